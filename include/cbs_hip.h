@@ -1,0 +1,148 @@
+/*
+ * cbs_hip.h -- C ABI of the MI355X convergent Born series (CBS) solver.
+ *
+ * Drop-in native backend for the CBS forward solver of BMOLKAIST/Inverse_solver.
+ * Each entry point replaces one action of the reference's CudaBorn MEX protocol
+ * (Codes/CUDA_CODE/main.cpp:53-213) or one step of the MATLAB class
+ * FORWARD_SOLVER_CONVERGENT_BORN that the MEX serves.  Plain pointers and
+ * sizes only; no exceptions cross this boundary.  Every function returns a
+ * CBS_* status code; on failure cbs_last_error() returns a message (thread local).
+ *
+ * Arrays are complex64 (interleaved re/im float pairs), column-major with x
+ * fastest, exactly as MATLAB passes single complex arrays with -R2018a
+ * (Codes/CUDA_CODE/compile_CudaBorn.m:4).  A batch of n_rhs arrays is stored
+ * back to back (the right-hand-side index is the slowest).
+ *
+ * Threading: one handle serialises its own calls on its own HIP stream;
+ * distinct handles are independent.  Calls on one handle from several threads
+ * at once are not allowed (as with the reference MEX).
+ */
+#ifndef CBS_HIP_H
+#define CBS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CBS_ABI_VERSION 1
+
+typedef struct cbs_solver* cbs_handle;
+
+enum {
+  CBS_OK = 0,
+  CBS_ERR_ARG = 1,          /* invalid argument or size mismatch */
+  CBS_ERR_HANDLE = 2,       /* unknown or destroyed handle */
+  CBS_ERR_HIP = 3,          /* HIP runtime error */
+  CBS_ERR_UNSUPPORTED = 4,  /* grid shape without a kernel plan */
+  CBS_ERR_STATE = 5,        /* call out of order (e.g. solve before set_potential) */
+  CBS_ERR_MEMORY = 6        /* device allocation failed or over the admission limit */
+};
+
+enum { CBS_MEM_HOST = 0, CBS_MEM_DEVICE = 1 };
+enum { CBS_FORM_FUSED = 0, CBS_FORM_PLAIN = 1 };
+
+/* Geometry of a solver instance.
+ * Replaces the struct of CudaBorn action 0 (main.cpp:57-101: simulation_size,
+ * RI_size, field_size, conv_size, green_size, used_gpu).  green_size is not
+ * needed because the I/O kernels are passed whole (cbs_set_io_kernels). */
+typedef struct {
+  uint32_t simulation_size[3]; /* RI grid + 2 * boundary pixels */
+  uint32_t RI_size[3];
+  uint32_t field_size[3];      /* xy size of the 2D fields; [2] ignored */
+  uint32_t conv_size[3];       /* xy convolution grid (field + RI); [2] ignored */
+  int32_t device;              /* HIP device ordinal */
+} cbs_geometry;
+
+/* Scalars set with the potential.
+ * Replaces the struct of CudaBorn action 1 (main.cpp:102-138: k0_nm, eps_imag,
+ * fourier_res1..3, ROI, Bornmax, dV); the remaining fields carry what the
+ * MATLAB class derives from its parameters (FORWARD_SOLVER_CONVERGENT_BORN.m:
+ * boundary_sharpness :37, acyclic :39, the shifted-axis choice :300-315 and the
+ * phase-ramp axes :355-377). */
+typedef struct {
+  double k0_nm;               /* RI_bg / wavelength */
+  double eps_imag;            /* 1.01 * max |Vt| (> 0) */
+  double fourier_res[3];      /* 1 / (resolution * simulation_size) */
+  uint32_t ROI[6];            /* 1-based inclusive [x0 x1 y0 y1 z0 z1] of the RI grid */
+  uint32_t Bornmax;           /* number of Born iterations (FFT pairs) */
+  double dV;                  /* voxel volume */
+  double boundary_sharpness;  /* attenuation-mask sharpness (reference default 1) */
+  uint32_t acyclic;           /* 1: alternate +-1/4-bin shifted Green's function */
+  uint32_t shift_axes;        /* bit j set: axis j carries the 1/4-bin shift */
+  uint32_t ramp_axes;         /* bit j set: axis j carries the phase ramp */
+} cbs_potential_params;
+
+typedef struct {
+  int32_t form;         /* CBS_FORM_FUSED (two passes per iteration) or CBS_FORM_PLAIN */
+  int32_t split_a;      /* fused form: force ny = a*b split (0 = planner's choice) */
+  int32_t tile_t;       /* fused form: force the k-space x tile (0 = planner's choice) */
+  int32_t max_batch;    /* right-hand sides per launch (0 = all of the call) */
+  int32_t track_norms;  /* 1: accumulate FP64 |f_n|^2 per rhs and iteration */
+  int32_t use_graph;    /* 1: capture the iteration loop in a hipGraph and replay it */
+  int32_t time_kernels; /* 1: time each pass kind with HIP events (cbs_get_kernel_times) */
+} cbs_options;
+
+/* Creates a solver on geo->device.  Replaces CudaBorn action 0 (main.cpp:57-101). */
+int cbs_create(const cbs_geometry* geo, cbs_handle* out);
+
+/* Replaces CudaBorn action 3 (main.cpp:164-174): frees every device buffer. */
+int cbs_destroy(cbs_handle h);
+
+/* Options (no counterpart in the reference, which has a fixed cuFFT plan). */
+int cbs_set_options(cbs_handle h, const cbs_options* opt);
+int cbs_get_options(cbs_handle h, cbs_options* opt);
+
+/* Uploads V = Vt - i*eps (complex64, simulation grid) and the scalars.
+ * Replaces CudaBorn action 1 (main.cpp:102-138), i.e. the device half of
+ * FORWARD_SOLVER_CONVERGENT_BORN.set_RI (:180-198).  Resets the alternating
+ * shift state, as set_RI's init() does (:280-396). */
+int cbs_set_potential(cbs_handle h, const cbs_potential_params* prm, const void* V, int mem);
+
+/* Born loop from the incident 3D field: u_out = Field[ROI] + u_in for n_rhs
+ * right-hand sides (RI grid each).  Replaces CudaBorn action 2 (main.cpp:139-163)
+ * and the MATLAB loop FORWARD_SOLVER_CONVERGENT_BORN.solve_raw :564-647.
+ * u_in and u_out are both host or both device pointers (mem); they may alias. */
+int cbs_solve_3d(cbs_handle h, const void* u_in, uint32_t n_rhs, void* u_out, int mem);
+
+/* Uploads the refocusing and transmission kernels for the 2D input/output
+ * projections (each conv_x * conv_y * RI_z complex64, centred (kx, ky) order,
+ * z in real space, as FORWARD_SOLVER_CONVERGENT_BORN's refocusing_util and
+ * kernel_trans :80-115).  kernel_ref may be null (then reflection is not
+ * available).  Replaces the green octants of CudaBorn action 0 (main.cpp:93-94). */
+int cbs_set_io_kernels(cbs_handle h, const void* refocus, const void* kernel_trans, const void* kernel_ref, int mem);
+
+/* Full 2D-in / 2D-out solve for n_rhs angles.  source_2d: field grid, real
+ * space, already NA/Abbe filtered (what FORWARD_SOLVER_CONVERGENT_BORN_CUDA.m:122
+ * passes).  Outputs (any may be null): trans_2d and ref_2d on the field grid
+ * before the host adds the input spectrum and applies transform_field_2D
+ * (:133-145), field_3d on the RI grid.  Replaces CudaBorn action 4
+ * (main.cpp:175-210: execute_2D_in, execute, execute_2D_out). */
+int cbs_solve_2d(cbs_handle h, const void* source_2d, uint32_t n_rhs, void* trans_2d, void* ref_2d, void* field_3d,
+                 int mem);
+
+/* FP64 per-iteration |f_n|^2 of the last solve (track_norms=1): out[r * n_iter + j]. */
+int cbs_get_norms(cbs_handle h, double* out, uint32_t n_rhs, uint32_t n_iter);
+
+/* Per-kind kernel time of the last solve (time_kernels=1):
+ * out_ms[k], out_count[k] for k = 0 pass_real, 1 pass_kspace, 2 other. */
+int cbs_get_kernel_times(cbs_handle h, double* out_ms, uint64_t* out_count, uint32_t n);
+
+/* The HIP stream (hipStream_t) the handle launches on. */
+int cbs_get_stream(cbs_handle h, void** stream);
+
+/* Device bytes a solve of n_rhs right-hand sides needs (state + workspace). */
+int cbs_footprint(cbs_handle h, uint32_t n_rhs, uint64_t* bytes);
+
+/* Kernel plan chosen for the current options: form, a, b, tile_t. */
+int cbs_get_plan(cbs_handle h, int32_t* form, int32_t* a, int32_t* b, int32_t* tile_t);
+
+const char* cbs_last_error(void);
+int cbs_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CBS_HIP_H */

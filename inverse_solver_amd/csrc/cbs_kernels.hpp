@@ -1,0 +1,135 @@
+// Convergent Born series kernels for gfx950.
+//
+// Memory layout (all complex64, interleaved re/im, x fastest like MATLAB):
+//   V      [nz][ny][nx]            potential V = Vt - i*eps on the simulation grid
+//   Field  [rhs][nz][ny][nx]       accumulated series
+//   fstate [rhs][nz][ny][nx]       current increment f (and the source term p0 at start)
+//   spec   [rhs][nz][b][a][nx]     intermediate spectrum between the two passes
+//                                  (x in transposed k order, y split ny = a*b with
+//                                  the a-part in k space and the b-part in real space)
+//
+// One Born iteration (Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:577-643) is two launches:
+//   pass_real  : inverse cross-twiddle + inverse a-DFT (y) + inverse x-FFT, the
+//                pointwise update with ramp/mask/potential, Field accumulation,
+//                gamma*f with the next ramp, forward x-FFT + a-DFT + cross-twiddle
+//   pass_kspace: b-DFT (y) + z-FFT, Green's multiply with the +-1/4-bin shifted
+//                grid and 1/N, inverse z-FFT + inverse b-DFT
+#pragma once
+#include "fft_core.hpp"
+
+namespace cbs {
+
+// Update kinds for pass_real / update_plain (see the recurrence in solver.cpp).
+enum UpdKind : int {
+  UPD_NONE = 0,      // no inverse input: p comes from the source term
+  UPD_VQ_SET = 1,    // jj = 1 : f1 = A V q;            Field  = f1
+  UPD_VQ_ADD = 2,    // jj = 2 : f2 = A V q;            Field += f2; x = f2
+  UPD_FULL3 = 3,     // jj = 3 : f3 = A (x - gx + V q); Field += f3; x = f3 + F1
+  UPD_FULL = 4,      // jj > 3 : fn = A (x - gx + V q); Field += fn; x = fn
+};
+enum FwdKind : int {
+  FWD_NONE = 0,      // last iteration: nothing to transform
+  FWD_P0 = 1,        // p = source term stored in fstate (jj = 1, 2)
+  FWD_GAMMA = 2,     // p = gamma * x = (i/eps) V x
+};
+
+struct Tables {
+  const float2* tw_x;     // W_nx^e, e < nx
+  const float2* tw_a;     // W_a^e
+  const float2* tw_z;     // W_nz^e
+  const float2* cross;    // [b][a]: W_ny^(y2 * k1(p1))
+  const float2* ramp_x;   // sigma=+1 ramp per axis (1 where the axis has no ramp)
+  const float2* ramp_y;
+  const float2* ramp_z;
+  const float* mask_x;    // attenuation mask factors per axis
+  const float* mask_y;
+  const float* mask_z;
+  const float* green_x;   // [2][nx]  4 pi^2 (k + sh dk/4)^2 at x position p (index 0: sigma=+1, 1: flipped)
+  const float* green_y;   // [2][b][a] at (k2, p1)
+  const float* green_z;   // [2][nz]  at z position p
+};
+
+struct Consts {
+  int nx, ny, nz, a, b;
+  int nrhs;
+  long long nvox;
+  float inv_eps;
+  float eps;
+  float km2;
+  float inv_n;
+};
+
+struct StateArgs {
+  const float2* V;
+  float2* field;
+  float2* fstate;
+  float2* spec;
+  const uint8_t* parity;   // per-rhs flip parity (0/1)
+  long long rhs_stride;    // elements between consecutive rhs in field/fstate/spec
+  double* norms;           // optional [nrhs] FP64 |f_n|^2 accumulators (may be null)
+};
+
+// flip state of iteration jj for rhs r: 1 = Greenp flipped (sigma = -1), 0 = sigma = +1
+__device__ __forceinline__ int flip_of(int jj, const uint8_t* parity, int r) { return (jj + parity[r]) & 1; }
+
+__device__ __forceinline__ float2 ramp3(const Tables& tb, int x, int y, int z) {
+  return cmul(cmul(tb.ramp_x[x], tb.ramp_y[y]), tb.ramp_z[z]);
+}
+
+// The pointwise body shared by the fused and the plain forms.
+//   q      : G_s applied to the previous p, before the conj ramp (ignored for UPD_NONE)
+//   returns the new p (times the forward ramp) to be transformed, or 0
+__device__ __forceinline__ float2 born_point(int upd, int fwd, float2 q, long long g, int x, int y, int z,
+                                             const Tables& tb, const Consts& c, float2* __restrict__ F,
+                                             float2* __restrict__ f, const float2* __restrict__ V, int flip_inv,
+                                             int flip_fwd, double& nrm) {
+  const float2 v = V[g];
+  const float2 r = ramp3(tb, x, y, z);
+  float2 xnew = make_float2(0.f, 0.f);
+  if (upd != UPD_NONE) {
+    // q * conj(ramp_s): ramp_s = r (sigma=+1) or conj(r) (sigma=-1)
+    const float2 qq = flip_inv ? cmul(q, r) : cmulc(q, r);
+    const float m = tb.mask_x[x] * tb.mask_y[y] * tb.mask_z[z];
+    const float2 vq = cmul(v, qq);
+    if (upd == UPD_VQ_SET) {
+      const float2 fn = cscale(vq, m);
+      F[g] = fn;
+      nrm += (double)fn.x * fn.x + (double)fn.y * fn.y;
+    } else if (upd == UPD_VQ_ADD) {
+      const float2 fn = cscale(vq, m);
+      F[g] = cadd(F[g], fn);
+      xnew = fn;
+      nrm += (double)fn.x * fn.x + (double)fn.y * fn.y;
+    } else {
+      const float2 xo = f[g];
+      // gamma x = (i/eps) v x
+      const float2 vx = cmul(v, xo);
+      const float2 gx = make_float2(-vx.y * c.inv_eps, vx.x * c.inv_eps);
+      const float2 fn = cscale(cadd(csub(xo, gx), vq), m);
+      const float2 Fo = F[g];
+      F[g] = cadd(Fo, fn);
+      xnew = (upd == UPD_FULL3) ? cadd(fn, csub(Fo, xo)) : fn;
+      nrm += (double)fn.x * fn.x + (double)fn.y * fn.y;
+    }
+  }
+  if (fwd == FWD_NONE) return make_float2(0.f, 0.f);
+  float2 p;
+  if (fwd == FWD_P0) {
+    p = f[g];
+  } else {
+    if (upd != UPD_VQ_SET) f[g] = xnew;
+    const float2 vx = cmul(v, xnew);
+    p = make_float2(-vx.y * c.inv_eps, vx.x * c.inv_eps);
+  }
+  return flip_fwd ? cmulc(p, r) : cmul(p, r);
+}
+
+// Wave-level reduction of a double then one atomic per wave.
+__device__ __forceinline__ void flush_norm(double v, double* dst) {
+  if (!dst) return;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(dst, v);
+}
+
+}  // namespace cbs

@@ -1,0 +1,290 @@
+// Register-resident small DFTs and LDS-staged line FFTs for gfx950 (CDNA4).
+//
+// Building blocks for the convergent-Born-series passes:
+//   * Dft<R, INV>::run(v)   in-place R-point DFT on R complex values held in
+//     registers, natural order in and out; twiddles are compile-time constants.
+//   * LineFft<N>            two-step (four-step) plan N = R1 * R2 for a line FFT
+//     whose data sits in LDS.  The forward transform leaves X[k1 + R1*k2] at
+//     position R2*k1 + k2 (a transposed order); the inverse transform consumes
+//     exactly that order and returns natural order, so a forward/inverse pair
+//     never needs a reorder pass.  pos2freq() maps a position to its frequency.
+//
+// Sign convention: forward = exp(-2*pi*i*n*k/N), inverse = exp(+2*pi*i*n*k/N),
+// both unnormalised (the 1/N of the inverse is folded into the Green's multiply).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cbs {
+
+// ---------------------------------------------------------------- complex helpers
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+  return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.y, b.x, -a.x * b.y));
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+// multiply by -i (forward quarter turn) / +i
+__device__ __forceinline__ float2 cmul_mi(float2 a) { return make_float2(a.y, -a.x); }
+__device__ __forceinline__ float2 cmul_pi(float2 a) { return make_float2(-a.y, a.x); }
+
+// ---------------------------------------------------------------- compile-time trig
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+// sin/cos of 2*pi*j/n by Taylor series on an argument reduced to [-pi, pi];
+// accurate to ~1e-15, far below the float rounding applied afterwards.
+constexpr double ct_sin_red(double x) {
+  double term = x, sum = x, x2 = x * x;
+  for (int n = 1; n < 30; ++n) { term *= -x2 / double((2 * n) * (2 * n + 1)); sum += term; }
+  return sum;
+}
+constexpr double ct_cos_red(double x) {
+  double term = 1.0, sum = 1.0, x2 = x * x;
+  for (int n = 1; n < 30; ++n) { term *= -x2 / double((2 * n - 1) * (2 * n)); sum += term; }
+  return sum;
+}
+constexpr double ct_arg(long j, long n) {
+  long m = ((j % n) + n) % n;          // 0..n-1
+  if (2 * m > n) m -= n;                // -n/2..n/2
+  return 2.0 * kPi * double(m) / double(n);
+}
+constexpr double ct_cos2pi(long j, long n) { return ct_cos_red(ct_arg(j, n)); }
+constexpr double ct_sin2pi(long j, long n) { return ct_sin_red(ct_arg(j, n)); }
+
+template <int R>
+struct RootTable {
+  float c[R > 0 ? R : 1];
+  float s[R > 0 ? R : 1];
+  constexpr RootTable() : c(), s() {
+    for (int e = 0; e < R; ++e) {
+      c[e] = float(ct_cos2pi(e, R));
+      s[e] = float(ct_sin2pi(e, R));
+    }
+  }
+};
+template <int R>
+__device__ constexpr RootTable<R> kRoots = RootTable<R>();
+
+// v * W_R^e with W_R = exp(-2 pi i / R) (forward) or its conjugate (inverse).
+// e is a compile-time constant after unrolling; trivial rotations fold away.
+template <int R, bool INV>
+__device__ __forceinline__ float2 twiddle_ct(float2 v, int e) {
+  e %= R;
+  if (e == 0) return v;
+  if (4 * e == R) return INV ? cmul_pi(v) : cmul_mi(v);
+  if (2 * e == R) return make_float2(-v.x, -v.y);
+  if (4 * e == 3 * R) return INV ? cmul_mi(v) : cmul_pi(v);
+  const float c = kRoots<R>.c[e];
+  const float s = INV ? kRoots<R>.s[e] : -kRoots<R>.s[e];
+  return make_float2(fmaf(v.x, c, -v.y * s), fmaf(v.x, s, v.y * c));
+}
+
+// ---------------------------------------------------------------- small DFTs
+constexpr int first_factor(int r) {
+  return (r % 4 == 0 && r != 4) ? 4 : (r % 2 == 0 && r != 2) ? 2 : (r % 3 == 0 && r != 3) ? 3
+       : (r % 5 == 0 && r != 5) ? 5 : r;
+}
+
+template <int R, bool INV>
+struct Dft;
+
+template <bool INV>
+struct Dft<1, INV> { __device__ __forceinline__ static void run(float2*) {} };
+
+template <bool INV>
+struct Dft<2, INV> {
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <bool INV>
+struct Dft<3, INV> {
+  __device__ __forceinline__ static void run(float2* v) {
+    constexpr float c1 = -0.5f;
+    constexpr float s1 = INV ? 0.86602540378443864676f : -0.86602540378443864676f;
+    float2 a = v[0], b = v[1], c = v[2];
+    float2 t = cadd(b, c), d = csub(b, c);
+    float2 m = make_float2(fmaf(t.x, c1, a.x), fmaf(t.y, c1, a.y));
+    v[0] = cadd(a, t);
+    // (b - c) * i*s1
+    float2 r = make_float2(-d.y * s1, d.x * s1);
+    v[1] = cadd(m, r);
+    v[2] = csub(m, r);
+  }
+};
+
+template <bool INV>
+struct Dft<4, INV> {
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    float2 b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
+    b1 = INV ? cmul_pi(b1) : cmul_mi(b1);
+    v[0] = cadd(a0, b0);
+    v[2] = csub(a0, b0);
+    v[1] = cadd(a1, b1);
+    v[3] = csub(a1, b1);
+  }
+};
+
+template <bool INV>
+struct Dft<5, INV> {
+  __device__ __forceinline__ static void run(float2* v) {
+    // Winograd-style 5-point DFT
+    constexpr float c1 = 0.30901699437494742410f;   // cos(2pi/5)
+    constexpr float c2 = -0.80901699437494742410f;  // cos(4pi/5)
+    constexpr float s1 = INV ? 0.95105651629515357212f : -0.95105651629515357212f;
+    constexpr float s2 = INV ? 0.58778525229247312917f : -0.58778525229247312917f;
+    float2 x0 = v[0];
+    float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    float2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    float2 m1 = make_float2(x0.x + c1 * t1.x + c2 * t2.x, x0.y + c1 * t1.y + c2 * t2.y);
+    float2 m2 = make_float2(x0.x + c2 * t1.x + c1 * t2.x, x0.y + c2 * t1.y + c1 * t2.y);
+    // i*(s1*t3 + s2*t4) and i*(s2*t3 - s1*t4)
+    float2 n1 = make_float2(-(s1 * t3.y + s2 * t4.y), s1 * t3.x + s2 * t4.x);
+    float2 n2 = make_float2(-(s2 * t3.y - s1 * t4.y), s2 * t3.x - s1 * t4.x);
+    v[0] = cadd(x0, cadd(t1, t2));
+    v[1] = cadd(m1, n1);
+    v[4] = csub(m1, n1);
+    v[2] = cadd(m2, n2);
+    v[3] = csub(m2, n2);
+  }
+};
+
+// Composite R = F * M, four-step in registers:
+//   n = M*n1 + n2, k = k1 + F*k2;  X[k] = sum_n2 W_R^{n2 k1} W_M^{n2 k2} sum_n1 x[M n1 + n2] W_F^{n1 k1}
+template <int R, bool INV>
+struct Dft {
+  static constexpr int F = first_factor(R);
+  static constexpr int M = R / F;
+  static_assert(F > 1 && F < R, "radix not supported (needs factors 2, 3, 5)");
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 w[R];
+#pragma unroll
+    for (int n2 = 0; n2 < M; ++n2) {
+      float2 u[F];
+#pragma unroll
+      for (int n1 = 0; n1 < F; ++n1) u[n1] = v[M * n1 + n2];
+      Dft<F, INV>::run(u);
+#pragma unroll
+      for (int k1 = 0; k1 < F; ++k1) w[k1 * M + n2] = twiddle_ct<R, INV>(u[k1], n2 * k1);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < F; ++k1) {
+      float2 u[M];
+#pragma unroll
+      for (int n2 = 0; n2 < M; ++n2) u[n2] = w[k1 * M + n2];
+      Dft<M, INV>::run(u);
+#pragma unroll
+      for (int k2 = 0; k2 < M; ++k2) v[k1 + F * k2] = u[k2];
+    }
+  }
+};
+
+// ---------------------------------------------------------------- line FFT plans
+constexpr bool is_smooth(int n) {
+  while (n % 2 == 0) n /= 2;
+  while (n % 3 == 0) n /= 3;
+  while (n % 5 == 0) n /= 5;
+  return n == 1;
+}
+// R2 = largest divisor of n not above sqrt(n) (single step when n <= 16)
+constexpr int plan_r2(int n) {
+  if (n <= 16) return 1;
+  int best = 1;
+  for (int d = 1; d * d <= n; ++d)
+    if (n % d == 0) best = d;
+  return best;
+}
+
+template <int N>
+struct LineFft {
+  static_assert(is_smooth(N), "line length must factor into 2, 3 and 5");
+  static constexpr int R2 = plan_r2(N);
+  static constexpr int R1 = N / R2;
+  static_assert(R1 <= 36 && R2 <= 36, "line too long for a two-step plan");
+  // frequency held at position p after the forward transform
+  __host__ __device__ static constexpr int pos2freq(int p) {
+    return R2 == 1 ? p : (p / R2) + R1 * (p % R2);
+  }
+  __host__ __device__ static constexpr int freq2pos(int k) {
+    return R2 == 1 ? k : (k % R1) * R2 + (k / R1);
+  }
+};
+
+// Forward (INV=false) or inverse line FFTs on NL lines held in LDS.
+// addr(l, e) gives the LDS index of element e of line l.  tw points at the
+// forward twiddle table W_N^e = exp(-2 pi i e / N), e in [0, N), in any memory.
+// Must be called by all NT threads of the block; contains __syncthreads().
+template <int N, bool INV, int NL, int NT, class Addr>
+__device__ __forceinline__ void line_fft(float2* __restrict__ lds, const Addr& addr,
+                                         const float2* __restrict__ tw, int tid) {
+  using P = LineFft<N>;
+  constexpr int R1 = P::R1, R2 = P::R2;
+  if constexpr (R2 == 1) {
+    for (int l = tid; l < NL; l += NT) {
+      float2 v[N];
+#pragma unroll
+      for (int e = 0; e < N; ++e) v[e] = lds[addr(l, e)];
+      Dft<N, INV>::run(v);
+#pragma unroll
+      for (int e = 0; e < N; ++e) lds[addr(l, e)] = v[e];
+    }
+    __syncthreads();
+  } else if constexpr (!INV) {
+    for (int t = tid; t < NL * R2; t += NT) {
+      const int l = t % NL, n2 = t / NL;
+      float2 v[R1];
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) v[n1] = lds[addr(l, R2 * n1 + n2)];
+      Dft<R1, false>::run(v);
+#pragma unroll
+      for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], tw[n2 * k1]);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) lds[addr(l, R2 * k1 + n2)] = v[k1];
+    }
+    __syncthreads();
+    for (int t = tid; t < NL * R1; t += NT) {
+      const int l = t % NL, k1 = t / NL;
+      float2 v[R2];
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) v[n2] = lds[addr(l, R2 * k1 + n2)];
+      Dft<R2, false>::run(v);
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) lds[addr(l, R2 * k1 + k2)] = v[k2];
+    }
+    __syncthreads();
+  } else {
+    for (int t = tid; t < NL * R1; t += NT) {
+      const int l = t % NL, k1 = t / NL;
+      float2 v[R2];
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) v[k2] = lds[addr(l, R2 * k1 + k2)];
+      Dft<R2, true>::run(v);
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) lds[addr(l, R2 * k1 + n2)] = v[n2];
+    }
+    __syncthreads();
+    for (int t = tid; t < NL * R2; t += NT) {
+      const int l = t % NL, n2 = t / NL;
+      float2 v[R1];
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) v[k1] = lds[addr(l, R2 * k1 + n2)];
+#pragma unroll
+      for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmulc(v[k1], tw[n2 * k1]);
+      Dft<R1, true>::run(v);
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) lds[addr(l, R2 * n1 + n2)] = v[n1];
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace cbs

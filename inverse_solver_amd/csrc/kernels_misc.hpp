@@ -1,0 +1,38 @@
+#pragma once
+#include "cbs_kernels.hpp"
+
+namespace cbs {
+
+struct RoiGeom {
+  int ox, oy, oz;        // 0-based start of the RI grid inside the simulation grid
+  int nx, ny, nz;        // RI grid size
+  long long nvox;
+};
+
+__global__ void green_plain(float2* __restrict__ spec, Tables tb, Consts c, int flip);
+__global__ void update_plain(StateArgs st, Tables tb, Consts c, int upd, int fwd, int jj_inv, int jj_fwd, int rhs);
+__global__ void seed_source(const float2* __restrict__ u_in, const float2* __restrict__ V, float2* __restrict__ fstate,
+                            Consts c, RoiGeom rg, long long rhs_stride, int nrhs);
+__global__ void finalize_field(const float2* __restrict__ field, const float2* __restrict__ u_in, float2* __restrict__ u_out,
+                               Consts c, RoiGeom rg, long long rhs_stride, int nrhs);
+
+// ---- registry of templated pass kernels (passes_inst.hip)
+using PassRFn = void (*)(struct PassRArgs);
+using PassKFn = void (*)(struct PassKArgs);
+using RowsFn = void (*)(float2*, long long, const float2*, int);
+using StridedFn = void (*)(float2*, int, long long, long long, int, const float2*, int);
+
+struct PassRInfo { PassRFn fn; int nx, a, nt, lds; };
+struct PassKInfo { PassKFn fn; int b, nz, t, nt, lds; };
+struct RowsInfo { RowsFn fn; int n, rows, nt; };
+struct StridedInfo { StridedFn fn; int n, t, nt; };
+
+const PassRInfo* find_pass_real(int nx, int a);
+const PassKInfo* find_pass_kspace(int b, int nz, int t);
+const RowsInfo* find_fft_rows(int n);
+const StridedInfo* find_fft_strided(int n, int t);
+// enumerate for planning
+int list_pass_real(const PassRInfo** out);
+int list_pass_kspace(const PassKInfo** out);
+
+}  // namespace cbs

@@ -1,0 +1,80 @@
+// Explicit instantiations of the templated passes and a lookup registry.
+// The shape lists cover the simulation grids of the supported workloads
+// (sim 96^3, 160^3, 128x128x135, 128^3, 192^2x128, 288^2x160, 576^2x320) and the small grids
+// used by the tests; a grid whose line lengths are absent here is rejected by
+// the planner with a clear error.
+#include "cbs_passes.hpp"
+#include "kernels_misc.hpp"
+
+namespace cbs {
+
+constexpr int kNT = 256;
+
+#define PR_LIST(X) \
+  X(24, 24) X(24, 12) X(24, 6) X(25, 25) X(32, 32) X(32, 16) X(40, 40) X(40, 20) \
+  X(96, 96) X(96, 48) X(96, 32) \
+  X(128, 64) X(128, 32) \
+  X(160, 40) X(160, 32) X(160, 20) \
+  X(192, 48) X(192, 32) \
+  X(288, 32)
+
+#define PK_LIST(X) \
+  X(1, 18, 8) X(1, 24, 8) X(1, 24, 16) X(2, 24, 8) X(4, 24, 8) X(1, 25, 8) X(1, 27, 8) X(1, 20, 8) \
+  X(1, 32, 16) X(2, 32, 16) X(1, 40, 16) X(2, 40, 16) X(1, 30, 8) X(1, 36, 8) \
+  X(1, 96, 32) X(1, 96, 16) X(2, 96, 16) X(3, 96, 16) \
+  X(2, 135, 32) X(2, 135, 16) X(4, 135, 8) \
+  X(2, 128, 32) X(4, 128, 16) X(6, 128, 8) X(4, 128, 8) \
+  X(4, 160, 16) X(5, 160, 8) X(8, 160, 8) X(4, 160, 8) \
+  X(9, 160, 4)
+
+#define RW_LIST(X) \
+  X(16) X(18) X(20) X(24) X(25) X(27) X(30) X(32) X(36) X(40) X(48) X(64) X(96) X(128) X(135) X(160) X(192) \
+  X(256) X(288) X(320) X(512) X(576) X(1024)
+
+#define ST_LIST(X) \
+  X(16) X(18) X(20) X(24) X(25) X(27) X(30) X(32) X(36) X(40) X(48) X(64) X(96) X(128) X(135) X(160) X(192) \
+  X(256) X(288) X(320) X(512) X(576) X(1024)
+
+constexpr int rows_for(int n) { return n <= 64 ? 16 : (n <= 256 ? 8 : (n <= 576 ? 4 : 2)); }
+constexpr int strided_t(int n) { return n <= 512 ? 16 : 8; }
+
+#define PR_ENTRY(NX, A) {&pass_real<NX, A, kNT>, NX, A, kNT, (int)(A * row_pitch(NX) * sizeof(float2))},
+#define PK_ENTRY(B, NZ, T) {&pass_kspace<B, NZ, T, kNT>, B, NZ, T, kNT, (int)(NZ * B * T * sizeof(float2))},
+#define RW_ENTRY(N) {&fft_rows<N, rows_for(N), kNT>, N, rows_for(N), kNT},
+#define ST_ENTRY(N) {&fft_strided<N, strided_t(N), kNT>, N, strided_t(N), kNT},
+
+static const PassRInfo kPR[] = {PR_LIST(PR_ENTRY)};
+static const PassKInfo kPK[] = {PK_LIST(PK_ENTRY)};
+static const RowsInfo kRW[] = {RW_LIST(RW_ENTRY)};
+static const StridedInfo kST[] = {ST_LIST(ST_ENTRY)};
+
+const PassRInfo* find_pass_real(int nx, int a) {
+  for (const auto& e : kPR)
+    if (e.nx == nx && e.a == a) return &e;
+  return nullptr;
+}
+const PassKInfo* find_pass_kspace(int b, int nz, int t) {
+  for (const auto& e : kPK)
+    if (e.b == b && e.nz == nz && e.t == t) return &e;
+  return nullptr;
+}
+const RowsInfo* find_fft_rows(int n) {
+  for (const auto& e : kRW)
+    if (e.n == n) return &e;
+  return nullptr;
+}
+const StridedInfo* find_fft_strided(int n, int t) {
+  for (const auto& e : kST)
+    if (e.n == n && (t <= 0 || e.t == t)) return &e;
+  return nullptr;
+}
+int list_pass_real(const PassRInfo** out) {
+  *out = kPR;
+  return (int)(sizeof(kPR) / sizeof(kPR[0]));
+}
+int list_pass_kspace(const PassKInfo** out) {
+  *out = kPK;
+  return (int)(sizeof(kPK) / sizeof(kPK[0]));
+}
+
+}  // namespace cbs

@@ -1,0 +1,777 @@
+// Host side of the CBS solver: handle registry, kernel planning, FP64 table
+// construction, the iteration schedule and the C ABI (include/cbs_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cbs_hip.h"
+#include "cbs_passes.hpp"
+#include "io_kernels.hpp"
+#include "kernels_misc.hpp"
+
+namespace cbs {
+
+// ------------------------------------------------------------------ errors
+struct CbsError : std::runtime_error {
+  int code;
+  CbsError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+static thread_local std::string g_last_error;
+
+#define HIP_CHECK(x)                                                                                    \
+  do {                                                                                                  \
+    hipError_t e_ = (x);                                                                                \
+    if (e_ != hipSuccess) throw CbsError(CBS_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+static void require(bool cond, int code, const std::string& msg) {
+  if (!cond) throw CbsError(code, msg);
+}
+
+// ------------------------------------------------------------------ device buffers
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void ensure(size_t count) {
+    if (count <= n && p) return;
+    release();
+    hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      throw CbsError(CBS_ERR_MEMORY, "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed: " +
+                                         hipGetErrorString(e));
+    }
+    n = count;
+  }
+  void upload(const std::vector<T>& v) {
+    ensure(v.size());
+    HIP_CHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  }
+};
+
+// ------------------------------------------------------------------ FFT position maps
+static int pos2freq(int n, int p) {
+  const int r2 = plan_r2(n);
+  if (r2 == 1) return p;
+  const int r1 = n / r2;
+  return (p / r2) + r1 * (p % r2);
+}
+
+// ------------------------------------------------------------------ table set
+struct TableSet {
+  int a = 0, b = 0;
+  DevBuf<float2> tw_x, tw_a, tw_z, cross, ramp_x, ramp_y, ramp_z;
+  DevBuf<float> mask_x, mask_y, mask_z, green_x, green_y, green_z;
+  Tables view() const {
+    Tables t;
+    t.tw_x = tw_x.p; t.tw_a = tw_a.p; t.tw_z = tw_z.p; t.cross = cross.p;
+    t.ramp_x = ramp_x.p; t.ramp_y = ramp_y.p; t.ramp_z = ramp_z.p;
+    t.mask_x = mask_x.p; t.mask_y = mask_y.p; t.mask_z = mask_z.p;
+    t.green_x = green_x.p; t.green_y = green_y.p; t.green_z = green_z.p;
+    return t;
+  }
+};
+
+static std::vector<float2> twiddles(int n) {
+  std::vector<float2> t(n);
+  for (int e = 0; e < n; ++e) {
+    const double ang = -2.0 * M_PI * (double)e / (double)n;
+    t[e] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+  }
+  return t;
+}
+
+struct Plan {
+  int form = CBS_FORM_FUSED;
+  int a = 0, b = 0, t = 0;
+  const PassRInfo* pr = nullptr;
+  const PassKInfo* pk = nullptr;
+  const RowsInfo* rw = nullptr;
+  const StridedInfo* sy = nullptr;
+  const StridedInfo* sz = nullptr;
+};
+
+// ------------------------------------------------------------------ solver
+struct Solver {
+  int device = 0;
+  int nx = 0, ny = 0, nz = 0;
+  int rx = 0, ry = 0, rz = 0;     // RI grid
+  int fx = 0, fy = 0;             // field grid
+  int cx = 0, cy = 0;             // convolution grid
+  long long nvox = 0, nri = 0;
+  hipStream_t stream = nullptr;
+  cbs_options opt{};
+  bool have_potential = false;
+  cbs_potential_params prm{};
+  DevBuf<float2> V;
+  Plan plan;
+  TableSet tabs;        // tables of the active plan
+  int flip_state = 0;   // persistent alternating-shift parity (reference handle semantics)
+  // workspace
+  DevBuf<float2> field, fstate, spec, uin, uout;
+  DevBuf<uint8_t> parity;
+  DevBuf<double> norms;
+  std::vector<double> last_norms;
+  int last_norm_rhs = 0, last_norm_iter = 0;
+  // kernel timing
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, int>> ev_used;   // (kind, index of start event)
+  double ktime_ms[3] = {0, 0, 0};
+  uint64_t kcount[3] = {0, 0, 0};
+  // graph cache
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph = nullptr;
+  std::vector<long long> graph_key;
+  // I/O projection state
+  IoState io;
+
+  ~Solver() {
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(device);
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto e : ev_pool) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+    (void)hipSetDevice(prev);
+  }
+
+  Consts consts(int nb) const {
+    Consts c;
+    c.nx = nx; c.ny = ny; c.nz = nz; c.a = plan.a; c.b = plan.b; c.nrhs = nb; c.nvox = nvox;
+    c.eps = (float)prm.eps_imag;
+    c.inv_eps = (float)(1.0 / prm.eps_imag);
+    const double km = 2.0 * M_PI * prm.k0_nm;
+    c.km2 = (float)(km * km);
+    c.inv_n = (float)(1.0 / (double)nvox);
+    return c;
+  }
+  RoiGeom roi() const {
+    RoiGeom g;
+    g.ox = (int)prm.ROI[0] - 1; g.oy = (int)prm.ROI[2] - 1; g.oz = (int)prm.ROI[4] - 1;
+    g.nx = rx; g.ny = ry; g.nz = rz; g.nvox = nri;
+    return g;
+  }
+
+  // ---- planning
+  void make_plan() {
+    Plan p;
+    p.form = opt.form;
+    if (p.form == CBS_FORM_FUSED) {
+      const PassRInfo* prs = nullptr;
+      const int npr = list_pass_real(&prs);
+      const PassKInfo* pks = nullptr;
+      const int npk = list_pass_kspace(&pks);
+      int best_a = -1;
+      for (int i = 0; i < npr; ++i) {
+        const PassRInfo& r = prs[i];
+        if (r.nx != nx || ny % r.a != 0) continue;
+        if (opt.split_a > 0 && r.a != opt.split_a) continue;
+        const int b = ny / r.a;
+        const PassKInfo* k = nullptr;
+        for (int j = 0; j < npk; ++j) {
+          const PassKInfo& q = pks[j];
+          if (q.b != b || q.nz != nz) continue;
+          if (opt.tile_t > 0 && q.t != opt.tile_t) continue;
+          if (!k || q.t > k->t) k = &q;
+        }
+        if (!k) continue;
+        if (r.a > best_a) {
+          best_a = r.a;
+          p.pr = &r;
+          p.pk = k;
+        }
+      }
+      if (p.pr) {
+        p.a = p.pr->a;
+        p.b = ny / p.a;
+        p.t = p.pk->t;
+        plan = p;
+        return;
+      }
+      if (opt.split_a > 0 || opt.tile_t > 0)
+        throw CbsError(CBS_ERR_UNSUPPORTED, "no fused kernel plan for the forced split/tile on grid " + shape_str());
+      p.form = CBS_FORM_PLAIN;   // grids without a fused plan run the per-axis form
+    }
+    p.rw = find_fft_rows(nx);
+    p.sy = find_fft_strided(ny, 0);
+    p.sz = find_fft_strided(nz, 0);
+    if (!p.rw || !p.sy || !p.sz)
+      throw CbsError(CBS_ERR_UNSUPPORTED, "no FFT kernels for simulation grid " + shape_str() +
+                                              " (line lengths must be instantiated 2,3,5-smooth sizes)");
+    p.a = ny;
+    p.b = 1;
+    p.t = p.sy->t;
+    plan = p;
+  }
+  std::string shape_str() const {
+    return std::to_string(nx) + "x" + std::to_string(ny) + "x" + std::to_string(nz);
+  }
+
+  // ---- FP64 tables for the active plan
+  void build_tables() {
+    const int a = plan.a, b = plan.b;
+    TableSet& T = tabs;
+    T.a = a; T.b = b;
+    T.tw_x.upload(twiddles(nx));
+    T.tw_a.upload(twiddles(a));
+    T.tw_z.upload(twiddles(nz));
+    {
+      std::vector<float2> cr((size_t)a * b);
+      for (int y2 = 0; y2 < b; ++y2)
+        for (int p1 = 0; p1 < a; ++p1) {
+          const double ang = -2.0 * M_PI * (double)y2 * (double)pos2freq(a, p1) / (double)ny;
+          cr[(size_t)y2 * a + p1] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+        }
+      T.cross.upload(cr);
+    }
+    const int dims[3] = {nx, ny, nz};
+    DevBuf<float2>* ramps[3] = {&T.ramp_x, &T.ramp_y, &T.ramp_z};
+    DevBuf<float>* masks[3] = {&T.mask_x, &T.mask_y, &T.mask_z};
+    for (int j = 0; j < 3; ++j) {
+      const int n = dims[j];
+      // phase ramp exp(-i pi (d - floor(n/2)) / (2 n))  (FORWARD_SOLVER_CONVERGENT_BORN.m:355-377)
+      std::vector<float2> rp(n);
+      const bool on = prm.acyclic && (prm.ramp_axes >> j & 1u);
+      for (int d = 0; d < n; ++d) {
+        const double ang = -M_PI * (double)(d - n / 2) / (2.0 * n);
+        rp[d] = on ? make_float2((float)std::cos(ang), (float)std::sin(ang)) : make_float2(1.f, 0.f);
+      }
+      ramps[j]->upload(rp);
+      // attenuation mask (create_boundary_RI :251-271)
+      const int bpx = (int)prm.ROI[2 * j] - 1;
+      std::vector<float> mk(n);
+      const double sh = prm.boundary_sharpness;
+      for (int d = 0; d < n; ++d) {
+        // x(i) = |i - (floor(n/2)+1) + 0.5| - 0.5 taken after circshift(x, -floor(n/2)),
+        // i.e. the distance of d to the nearest edge of the axis
+        const int dd = (d + n / 2) % n;
+        double xv = std::fabs((double)(dd + 1) - std::floor(n / 2.0 + 1.0) + 0.5) - 0.5;
+        double val0;
+        if (bpx == 0) {
+          val0 = 0.0;
+        } else {
+          xv = xv / ((double)bpx - 0.5);
+          val0 = std::fabs(xv) >= 1.0 ? 1.0 : std::fabs(xv);
+          val0 = 1.0 - val0;
+          if (val0 > sh) val0 = sh;
+          val0 = val0 / sh;
+        }
+        mk[d] = (float)(1.0 - val0);
+      }
+      masks[j]->upload(mk);
+    }
+    // Green's function per-axis terms 4 pi^2 (k + sh dk/4)^2 (init :299-351), position ordered,
+    // index 1 = flipped grid (fft_flip(...,[1 1 1],false) :580): G1(d) = G0((n - d) mod n).
+    auto g0 = [&](int j, int n, int d) {
+      const int m = ((d + n / 2) % n) - n / 2;
+      const double sh = (prm.acyclic && (prm.shift_axes >> j & 1u)) ? 0.25 : 0.0;
+      const double k = ((double)m + sh) * prm.fourier_res[j];
+      return 4.0 * M_PI * M_PI * k * k;
+    };
+    auto gfl = [&](int j, int n, int flip, int d) { return flip ? g0(j, n, (n - d) % n) : g0(j, n, d); };
+    {
+      std::vector<float> gx(2 * (size_t)nx + 64, 0.f);
+      for (int f = 0; f < 2; ++f)
+        for (int p = 0; p < nx; ++p) gx[(size_t)f * nx + p] = (float)gfl(0, nx, f, pos2freq(nx, p));
+      T.green_x.upload(gx);
+      std::vector<float> gy(2 * (size_t)ny, 0.f);
+      for (int f = 0; f < 2; ++f)
+        for (int k2 = 0; k2 < b; ++k2)
+          for (int p1 = 0; p1 < a; ++p1) {
+            const int ky = (plan.form == CBS_FORM_FUSED) ? pos2freq(a, p1) + a * k2 : pos2freq(ny, p1);
+            gy[(size_t)f * ny + (size_t)k2 * a + p1] = (float)gfl(1, ny, f, ky);
+          }
+      T.green_y.upload(gy);
+      std::vector<float> gz(2 * (size_t)nz, 0.f);
+      for (int f = 0; f < 2; ++f)
+        for (int p = 0; p < nz; ++p) gz[(size_t)f * nz + p] = (float)gfl(2, nz, f, pos2freq(nz, p));
+      T.green_z.upload(gz);
+    }
+  }
+
+  // ---- kernel timing
+  hipEvent_t event_at(size_t i) {
+    while (ev_pool.size() <= i) {
+      hipEvent_t e;
+      HIP_CHECK(hipEventCreate(&e));
+      ev_pool.push_back(e);
+    }
+    return ev_pool[i];
+  }
+  template <class F>
+  void timed(int kind, F&& launch) {
+    if (!opt.time_kernels) {
+      launch();
+      return;
+    }
+    const size_t i = 2 * ev_used.size();
+    hipEvent_t e0 = event_at(i), e1 = event_at(i + 1);
+    HIP_CHECK(hipEventRecord(e0, stream));
+    launch();
+    HIP_CHECK(hipEventRecord(e1, stream));
+    ev_used.emplace_back(kind, (int)i);
+  }
+  void resolve_times() {
+    for (int k = 0; k < 3; ++k) { ktime_ms[k] = 0; kcount[k] = 0; }
+    for (auto& u : ev_used) {
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, ev_pool[u.second], ev_pool[u.second + 1]));
+      ktime_ms[u.first] += ms;
+      kcount[u.first] += 1;
+    }
+    ev_used.clear();
+  }
+
+  // ---- launches
+  StateArgs state_args(int nb) {
+    (void)nb;
+    StateArgs s;
+    s.V = V.p; s.field = field.p; s.fstate = fstate.p; s.spec = spec.p; s.parity = parity.p;
+    s.rhs_stride = nvox;
+    s.norms = nullptr;
+    return s;
+  }
+
+  static int upd_of(int jj) { return jj == 1 ? UPD_VQ_SET : jj == 2 ? UPD_VQ_ADD : jj == 3 ? UPD_FULL3 : UPD_FULL; }
+  int fwd_of(int jj, int M) const { return jj == M ? FWD_NONE : (jj == 1 ? FWD_P0 : FWD_GAMMA); }
+
+  // fused schedule for one batch already seeded in fstate
+  void run_fused(int nb, int M) {
+    const PassRInfo* pr = plan.pr;
+    const PassKInfo* pk = plan.pk;
+    const Consts c = consts(nb);
+    const Tables tb = tabs.view();
+    auto pass_r = [&](int upd, int fwd, int jj_inv, int jj_fwd) {
+      PassRArgs args;
+      args.tb = tb; args.c = c; args.st = state_args(nb);
+      if (opt.track_norms && jj_inv > 0) {
+        args.st.norms = norms.p + (size_t)(jj_inv - 1) * nb;
+      }
+      args.upd = upd; args.fwd = fwd; args.jj_inv = jj_inv; args.jj_fwd = jj_fwd;
+      const dim3 grid((unsigned)((long long)nb * plan.b * nz));
+      timed(0, [&] { hipLaunchKernelGGL(pr->fn, grid, dim3(pr->nt), 0, stream, args); });
+    };
+    auto pass_k = [&](int jj) {
+      PassKArgs args;
+      args.tb = tb; args.c = c; args.spec = spec.p; args.parity = parity.p; args.rhs_stride = nvox; args.jj = jj;
+      const int ntile = (nx + pk->t - 1) / pk->t;
+      const dim3 grid((unsigned)((long long)nb * plan.a * ntile));
+      timed(1, [&] { hipLaunchKernelGGL(pk->fn, grid, dim3(pk->nt), 0, stream, args); });
+    };
+    pass_r(UPD_NONE, FWD_P0, 0, 1);
+    for (int jj = 1; jj <= M; ++jj) {
+      pass_k(jj);
+      const int fwd = fwd_of(jj, M);
+      pass_r(upd_of(jj), fwd, jj, fwd == FWD_NONE ? 0 : jj + 1);
+    }
+    HIP_CHECK(hipGetLastError());
+  }
+
+  // plain schedule for rhs r of a batch of nb (state arrays indexed by r)
+  void run_plain(int r, int nb, int M) {
+    const Consts c = consts(1);
+    const Tables tb = tabs.view();
+    const long long ro = (long long)r * nvox;
+    float2* sp = spec.p + ro;
+    const unsigned pw_grid = (unsigned)((nvox + 255) / 256);
+    auto update = [&](int upd, int fwd, int jj_inv, int jj_fwd) {
+      StateArgs s = state_args(1);
+      s.norms = (opt.track_norms && jj_inv > 0) ? norms.p + (size_t)(jj_inv - 1) * nb : nullptr;
+      timed(2, [&] { hipLaunchKernelGGL(update_plain, dim3(pw_grid), dim3(256), 0, stream, s, tb, c, upd, fwd, jj_inv,
+                                        jj_fwd, r); });
+    };
+    auto fft3 = [&](int inverse) {
+      const RowsInfo* rw = plan.rw;
+      const StridedInfo* sy = plan.sy;
+      const StridedInfo* sz = plan.sz;
+      auto fx = [&] {
+        const long long nrows = (long long)ny * nz;
+        const dim3 g((unsigned)((nrows + rw->rows - 1) / rw->rows));
+        timed(2, [&] { hipLaunchKernelGGL(rw->fn, g, dim3(rw->nt), 0, stream, sp, nrows, tabs.tw_x.p, inverse); });
+      };
+      auto fy = [&] {
+        const int nt = (nx + sy->t - 1) / sy->t;
+        const dim3 g((unsigned)(nt * nz));
+        timed(2, [&] { hipLaunchKernelGGL(sy->fn, g, dim3(sy->nt), 0, stream, sp, nx, (long long)nx,
+                                          (long long)nx * ny, nz, tabs.tw_a.p, inverse); });
+      };
+      auto fz = [&] {
+        const int nt = (nx + sz->t - 1) / sz->t;
+        const dim3 g((unsigned)(nt * ny));
+        timed(2, [&] { hipLaunchKernelGGL(sz->fn, g, dim3(sz->nt), 0, stream, sp, nx, (long long)nx * ny,
+                                          (long long)nx, ny, tabs.tw_z.p, inverse); });
+      };
+      if (!inverse) { fx(); fy(); fz(); }
+      else { fz(); fy(); fx(); }
+    };
+    update(UPD_NONE, FWD_P0, 0, 1);
+    for (int jj = 1; jj <= M; ++jj) {
+      fft3(0);
+      const int flip = (jj + (int)host_parity[r]) & 1;
+      timed(2, [&] { hipLaunchKernelGGL(green_plain, dim3(pw_grid), dim3(256), 0, stream, sp, tb, c, flip); });
+      fft3(1);
+      const int fwd = fwd_of(jj, M);
+      update(upd_of(jj), fwd, jj, fwd == FWD_NONE ? 0 : jj + 1);
+    }
+    HIP_CHECK(hipGetLastError());
+  }
+  std::vector<uint8_t> host_parity;
+
+  void ensure_workspace(int nb, bool host_io) {
+    const size_t n = (size_t)nb * nvox;
+    field.ensure(n);
+    fstate.ensure(n);
+    spec.ensure(n);
+    parity.ensure((size_t)nb);
+    if (host_io) {
+      uin.ensure((size_t)nb * nri);
+      uout.ensure((size_t)nb * nri);
+    }
+  }
+
+  uint64_t footprint(uint32_t n_rhs) const {
+    const int nb = batch_width(n_rhs);
+    uint64_t b = 3ull * nb * nvox * sizeof(float2);   // field, fstate, spec
+    b += 2ull * nb * nri * sizeof(float2);             // staging for host I/O
+    b += (uint64_t)nvox * sizeof(float2);              // V
+    b += io.footprint(nb);
+    return b;
+  }
+  int batch_width(uint32_t n_rhs) const {
+    if (opt.form == CBS_FORM_PLAIN || plan.form == CBS_FORM_PLAIN) return 1 > (int)n_rhs ? (int)n_rhs : 1;
+    int nb = (int)n_rhs;
+    if (opt.max_batch > 0 && opt.max_batch < nb) nb = opt.max_batch;
+    return nb;
+  }
+
+  // Born loop for a batch whose incident fields sit at d_uin (device, RI grid).
+  void born_batch(const float2* d_uin, float2* d_uout, int nb, int rhs_index0) {
+    const int M = (int)prm.Bornmax;
+    // the previous batch's kernels may still read the device parity array
+    HIP_CHECK(hipStreamSynchronize(stream));
+    host_parity.assign((size_t)nb, 0);
+    for (int r = 0; r < nb; ++r)
+      host_parity[r] = (uint8_t)((flip_state + (long long)(rhs_index0 + r) * M) & 1);
+    HIP_CHECK(hipMemcpy(parity.p, host_parity.data(), (size_t)nb, hipMemcpyHostToDevice));
+    const Consts c = consts(nb);
+    const RoiGeom g = roi();
+    if (opt.track_norms && M > 0) {
+      norms.ensure((size_t)M * nb);
+      HIP_CHECK(hipMemsetAsync(norms.p, 0, (size_t)M * nb * sizeof(double), stream));
+    }
+    {
+      const long long tot = (long long)nb * nvox;
+      timed(2, [&] { hipLaunchKernelGGL(seed_source, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream, d_uin,
+                                        V.p, fstate.p, c, g, nvox, nb); });
+    }
+    if (M == 0) {
+      HIP_CHECK(hipMemsetAsync(field.p, 0, (size_t)nb * nvox * sizeof(float2), stream));
+    } else if (plan.form == CBS_FORM_FUSED) {
+      run_fused(nb, M);
+    } else {
+      for (int r = 0; r < nb; ++r) run_plain(r, nb, M);
+    }
+    {
+      const long long tot = (long long)nb * nri;
+      timed(2, [&] { hipLaunchKernelGGL(finalize_field, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream,
+                                        field.p, d_uin, d_uout, c, g, nvox, nb); });
+    }
+    HIP_CHECK(hipGetLastError());
+    if (opt.track_norms && M > 0) {
+      std::vector<double> tmp((size_t)M * nb);
+      HIP_CHECK(hipMemcpyAsync(tmp.data(), norms.p, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+      HIP_CHECK(hipStreamSynchronize(stream));
+      for (int r = 0; r < nb; ++r)
+        for (int j = 0; j < M; ++j)
+          last_norms[(size_t)(rhs_index0 + r) * M + j] = tmp[(size_t)j * nb + r];
+    }
+  }
+
+  void solve_3d(const float2* u_in, uint32_t n_rhs, float2* u_out, int mem) {
+    require(have_potential, CBS_ERR_STATE, "cbs_set_potential must be called before solving");
+    require(n_rhs >= 1, CBS_ERR_ARG, "n_rhs must be >= 1");
+    require(u_in && u_out, CBS_ERR_ARG, "null field pointer");
+    require(mem == CBS_MEM_HOST || mem == CBS_MEM_DEVICE, CBS_ERR_ARG, "mem must be CBS_MEM_HOST or CBS_MEM_DEVICE");
+    const int nbw = batch_width(n_rhs);
+    ensure_workspace(nbw, mem == CBS_MEM_HOST);
+    const int M = (int)prm.Bornmax;
+    if (opt.track_norms) {
+      last_norms.assign((size_t)n_rhs * M, 0.0);
+      last_norm_rhs = (int)n_rhs;
+      last_norm_iter = M;
+    }
+    ev_used.clear();
+    for (uint32_t r0 = 0; r0 < n_rhs; r0 += nbw) {
+      const int nb = (int)std::min<uint32_t>(nbw, n_rhs - r0);
+      const float2* din;
+      float2* dout;
+      if (mem == CBS_MEM_HOST) {
+        HIP_CHECK(hipMemcpyAsync(uin.p, u_in + (size_t)r0 * nri, (size_t)nb * nri * sizeof(float2),
+                                 hipMemcpyHostToDevice, stream));
+        din = uin.p;
+        dout = uout.p;
+      } else {
+        din = u_in + (size_t)r0 * nri;
+        dout = u_out + (size_t)r0 * nri;
+      }
+      born_batch(din, dout, nb, (int)r0);
+      if (mem == CBS_MEM_HOST) {
+        HIP_CHECK(hipMemcpyAsync(u_out + (size_t)r0 * nri, uout.p, (size_t)nb * nri * sizeof(float2),
+                                 hipMemcpyDeviceToHost, stream));
+      }
+    }
+    HIP_CHECK(hipStreamSynchronize(stream));
+    if (opt.time_kernels) resolve_times();
+    if (prm.acyclic) flip_state = (int)((flip_state + (long long)n_rhs * M) & 1);
+  }
+
+  void set_potential(const cbs_potential_params* p, const float2* v, int mem) {
+    require(p != nullptr && v != nullptr, CBS_ERR_ARG, "null potential argument");
+    require(mem == CBS_MEM_HOST || mem == CBS_MEM_DEVICE, CBS_ERR_ARG, "mem must be CBS_MEM_HOST or CBS_MEM_DEVICE");
+    require(p->eps_imag > 0 && std::isfinite(p->eps_imag), CBS_ERR_ARG,
+            "eps_imag must be > 0 (a homogeneous RI has eps_imag = 0 and is not solvable by CBS)");
+    require(p->k0_nm > 0, CBS_ERR_ARG, "k0_nm must be > 0");
+    require(p->boundary_sharpness > 0, CBS_ERR_ARG, "boundary_sharpness must be > 0");
+    const int dims[3] = {nx, ny, nz};
+    const int rdims[3] = {rx, ry, rz};
+    for (int j = 0; j < 3; ++j) {
+      require(p->fourier_res[j] > 0, CBS_ERR_ARG, "fourier_res must be > 0");
+      const long lo = (long)p->ROI[2 * j], hi = (long)p->ROI[2 * j + 1];
+      require(lo >= 1 && hi <= dims[j] && hi - lo + 1 == rdims[j], CBS_ERR_ARG,
+              "ROI does not match RI_size inside simulation_size on axis " + std::to_string(j));
+      require(lo - 1 == dims[j] - hi, CBS_ERR_ARG, "ROI must be centred (equal boundary on both sides) on axis " +
+                                                       std::to_string(j));
+    }
+    prm = *p;
+    V.ensure((size_t)nvox);
+    HIP_CHECK(hipMemcpyAsync(V.p, v, (size_t)nvox * sizeof(float2),
+                             mem == CBS_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, stream));
+    make_plan();
+    build_tables();
+    HIP_CHECK(hipStreamSynchronize(stream));
+    flip_state = 0;
+    have_potential = true;
+  }
+};
+
+// ------------------------------------------------------------------ registry
+static std::mutex g_mu;
+static std::set<Solver*> g_live;
+
+static Solver* lookup(cbs_handle h) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Solver* s = reinterpret_cast<Solver*>(h);
+  if (!s || !g_live.count(s)) throw CbsError(CBS_ERR_HANDLE, "unknown or destroyed cbs handle");
+  return s;
+}
+
+struct DeviceGuard {
+  int prev = 0;
+  explicit DeviceGuard(int dev) {
+    (void)hipGetDevice(&prev);
+    HIP_CHECK(hipSetDevice(dev));
+  }
+  ~DeviceGuard() { (void)hipSetDevice(prev); }
+};
+
+template <class F>
+static int guarded(F&& f) {
+  try {
+    f();
+    return CBS_OK;
+  } catch (const CbsError& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return CBS_ERR_HIP;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return CBS_ERR_HIP;
+  }
+}
+
+}  // namespace cbs
+
+using namespace cbs;
+
+extern "C" {
+
+int cbs_abi_version(void) { return CBS_ABI_VERSION; }
+
+const char* cbs_last_error(void) { return g_last_error.c_str(); }
+
+int cbs_create(const cbs_geometry* geo, cbs_handle* out) {
+  return guarded([&] {
+    require(geo && out, CBS_ERR_ARG, "null argument");
+    *out = nullptr;
+    int ndev = 0;
+    HIP_CHECK(hipGetDeviceCount(&ndev));
+    require(geo->device >= 0 && geo->device < ndev, CBS_ERR_ARG, "device ordinal out of range");
+    for (int j = 0; j < 3; ++j) {
+      require(geo->simulation_size[j] >= 2 && geo->RI_size[j] >= 1 && geo->RI_size[j] <= geo->simulation_size[j],
+              CBS_ERR_ARG, "invalid simulation/RI size on axis " + std::to_string(j));
+    }
+    auto s = std::make_unique<Solver>();
+    s->device = geo->device;
+    DeviceGuard dg(s->device);
+    s->nx = (int)geo->simulation_size[0]; s->ny = (int)geo->simulation_size[1]; s->nz = (int)geo->simulation_size[2];
+    s->rx = (int)geo->RI_size[0]; s->ry = (int)geo->RI_size[1]; s->rz = (int)geo->RI_size[2];
+    s->fx = (int)geo->field_size[0]; s->fy = (int)geo->field_size[1];
+    s->cx = (int)geo->conv_size[0]; s->cy = (int)geo->conv_size[1];
+    s->nvox = (long long)s->nx * s->ny * s->nz;
+    s->nri = (long long)s->rx * s->ry * s->rz;
+    s->opt.form = CBS_FORM_FUSED;
+    HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    Solver* raw = s.release();
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_live.insert(raw);
+    *out = reinterpret_cast<cbs_handle>(raw);
+  });
+}
+
+int cbs_destroy(cbs_handle h) {
+  return guarded([&] {
+    Solver* s;
+    {
+      std::lock_guard<std::mutex> lk(g_mu);
+      s = reinterpret_cast<Solver*>(h);
+      if (!s || !g_live.count(s)) throw CbsError(CBS_ERR_HANDLE, "unknown or destroyed cbs handle");
+      g_live.erase(s);
+    }
+    {
+      DeviceGuard dg(s->device);
+      (void)hipStreamSynchronize(s->stream);
+    }
+    delete s;
+  });
+}
+
+int cbs_set_options(cbs_handle h, const cbs_options* opt) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    require(opt != nullptr, CBS_ERR_ARG, "null options");
+    require(opt->form == CBS_FORM_FUSED || opt->form == CBS_FORM_PLAIN, CBS_ERR_ARG, "unknown form");
+    require(opt->max_batch >= 0 && opt->split_a >= 0 && opt->tile_t >= 0, CBS_ERR_ARG, "negative option");
+    DeviceGuard dg(s->device);
+    s->opt = *opt;
+    if (s->have_potential) {
+      s->make_plan();
+      s->build_tables();
+    }
+  });
+}
+
+int cbs_get_options(cbs_handle h, cbs_options* opt) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    require(opt != nullptr, CBS_ERR_ARG, "null options");
+    *opt = s->opt;
+  });
+}
+
+int cbs_set_potential(cbs_handle h, const cbs_potential_params* prm, const void* V, int mem) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    DeviceGuard dg(s->device);
+    s->set_potential(prm, reinterpret_cast<const float2*>(V), mem);
+  });
+}
+
+int cbs_solve_3d(cbs_handle h, const void* u_in, uint32_t n_rhs, void* u_out, int mem) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    DeviceGuard dg(s->device);
+    s->solve_3d(reinterpret_cast<const float2*>(u_in), n_rhs, reinterpret_cast<float2*>(u_out), mem);
+  });
+}
+
+int cbs_get_norms(cbs_handle h, double* out, uint32_t n_rhs, uint32_t n_iter) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    require(out != nullptr, CBS_ERR_ARG, "null output");
+    require(s->opt.track_norms, CBS_ERR_STATE, "track_norms is off");
+    require((int)n_rhs == s->last_norm_rhs && (int)n_iter == s->last_norm_iter, CBS_ERR_ARG,
+            "n_rhs/n_iter do not match the last solve");
+    std::memcpy(out, s->last_norms.data(), s->last_norms.size() * sizeof(double));
+  });
+}
+
+int cbs_get_kernel_times(cbs_handle h, double* out_ms, uint64_t* out_count, uint32_t n) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    require(out_ms && out_count, CBS_ERR_ARG, "null output");
+    for (uint32_t k = 0; k < n && k < 3; ++k) {
+      out_ms[k] = s->ktime_ms[k];
+      out_count[k] = s->kcount[k];
+    }
+  });
+}
+
+int cbs_get_stream(cbs_handle h, void** stream) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    require(stream != nullptr, CBS_ERR_ARG, "null output");
+    *stream = (void*)s->stream;
+  });
+}
+
+int cbs_footprint(cbs_handle h, uint32_t n_rhs, uint64_t* bytes) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    require(bytes != nullptr, CBS_ERR_ARG, "null output");
+    *bytes = s->footprint(n_rhs);
+  });
+}
+
+int cbs_get_plan(cbs_handle h, int32_t* form, int32_t* a, int32_t* b, int32_t* tile_t) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    require(s->have_potential, CBS_ERR_STATE, "plan is chosen at cbs_set_potential");
+    if (form) *form = s->plan.form;
+    if (a) *a = s->plan.a;
+    if (b) *b = s->plan.b;
+    if (tile_t) *tile_t = s->plan.t;
+  });
+}
+
+int cbs_set_io_kernels(cbs_handle h, const void* refocus, const void* kernel_trans, const void* kernel_ref, int mem) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    (void)refocus; (void)kernel_trans; (void)kernel_ref; (void)mem;
+    (void)s;
+    throw CbsError(CBS_ERR_UNSUPPORTED, "2D input/output projections are not built yet");
+  });
+}
+
+int cbs_solve_2d(cbs_handle h, const void* source_2d, uint32_t n_rhs, void* trans_2d, void* ref_2d, void* field_3d,
+                 int mem) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    (void)source_2d; (void)n_rhs; (void)trans_2d; (void)ref_2d; (void)field_3d; (void)mem; (void)s;
+    throw CbsError(CBS_ERR_UNSUPPORTED, "2D input/output projections are not built yet");
+  });
+}
+
+}  // extern "C"

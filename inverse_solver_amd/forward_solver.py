@@ -1,0 +1,207 @@
+"""FORWARD_SOLVER_CONVERGENT_BORN, MI355X backend.
+
+Python mirror of the reference's MATLAB class API
+(Codes/FORWARD_SOLVER_CONVERGENT_BORN.m, Codes/FORWARD_SOLVER_CONVERGENT_BORN_CUDA.m):
+the same parameters (RI, wavelength, resolution, boundary_thickness,
+iterations_number, ...), set_RI(RI) and solve(input_field) returning the
+transmitted, reflected and 3D fields.  The host side here only prepares the
+potential and the geometry in FP64, as the MATLAB class does before calling its
+MEX; every solve runs in libcbs_hip.so on the GPU.
+
+Arrays use MATLAB index order: RI is (x, y, z); 2D fields (x, y, 1, A); 3D fields
+(x, y, z, 1, A).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+from .params import (basic_optical_parameter, update_struct, round_half_away, ri_to_potential)
+
+
+def get_default_parameters(init=None):
+    """FORWARD_SOLVER.get_default_parameters (Codes/FORWARD_SOLVER.m:9-21) followed by
+    FORWARD_SOLVER_CONVERGENT_BORN.get_default_parameters (:32-45)."""
+    p = basic_optical_parameter()
+    p.update(return_transmission=True, return_reflection=False, return_3D=False, use_GPU=True)
+    p.update(iterations_number=-1, boundary_thickness=[6, 6, 6], boundary_sharpness=1, verbose=False,
+             acyclic=True, RI_xy_size=[0, 0], RI_center=[0, 0])
+    return update_struct(p, init) if init is not None else p
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class FORWARD_SOLVER_CONVERGENT_BORN:
+    """Scalar CBS forward solver on one MI355X.
+
+    ``options`` maps to cbs_options (form, split_a, tile_t, max_batch,
+    track_norms, use_graph, time_kernels).
+    """
+
+    get_default_parameters = staticmethod(get_default_parameters)
+
+    def __init__(self, params, device=0, options=None):
+        p = dict(params)
+        if p.get('vector_simulation', False):
+            raise NotImplementedError('the dyadic (vector) Green\'s function is not implemented; '
+                                      'set vector_simulation=false')
+        rxy = list(p.get('RI_xy_size', [0, 0]))
+        rxy[0] = rxy[0] or p['size'][0]                                     # :54-59
+        rxy[1] = rxy[1] or p['size'][1]
+        p['RI_xy_size'] = rxy
+        self.parameters = p
+        self.expected_RI_size = [int(rxy[0]), int(rxy[1]), int(p['size'][2])]   # :60
+        bt = list(p['boundary_thickness'])
+        if len(bt) != 3:                                                    # :126-132
+            raise ValueError('Boundary in only one direction is not precise; enter "boundary_thickness" as an '
+                             'array of size 3 use size 0 for ciclic boundary')
+        self.cyclic_boundary_xy = (bt[0] == 0 and bt[1] == 0 and self.expected_RI_size[0] == p['size'][0]
+                                   and self.expected_RI_size[1] == p['size'][1])     # :63
+        self.boundary_thickness_pixel = round_half_away(
+            np.array(bt, float) * p['wavelength'] / p['RI_bg'] / (np.array(p['resolution'], float) * 2)
+        ).astype(int)                                                       # :129
+        self.simulation_size = [self.expected_RI_size[j] + 2 * int(self.boundary_thickness_pixel[j])
+                                for j in range(3)]
+        if self.cyclic_boundary_xy:
+            conv = [int(p['size'][0]), int(p['size'][1])]
+        else:
+            conv = [int(p['size'][0]) + self.expected_RI_size[0], int(p['size'][1]) + self.expected_RI_size[1]]
+        self.conv_size = conv
+        geo = _lib.Geometry()
+        for j in range(3):
+            geo.simulation_size[j] = self.simulation_size[j]
+            geo.RI_size[j] = self.expected_RI_size[j]
+        geo.field_size[0], geo.field_size[1], geo.field_size[2] = int(p['size'][0]), int(p['size'][1]), 1
+        geo.conv_size[0], geo.conv_size[1], geo.conv_size[2] = conv[0], conv[1], 1
+        geo.device = int(device)
+        h = C.c_void_p()
+        check(lib.cbs_create(C.byref(geo), C.byref(h)))
+        self._h = h
+        self.device = int(device)
+        self._io_ready = False
+        if options:
+            self.set_options(**options)
+
+    # ---------------------------------------------------------------- lifecycle
+    def close(self):
+        if getattr(self, '_h', None):
+            check(lib.cbs_destroy(self._h))
+            self._h = None
+
+    delete = close
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_options(self, **kw):
+        o = _lib.Options()
+        check(lib.cbs_get_options(self._h, C.byref(o)))
+        for k, v in kw.items():
+            if not hasattr(o, k):
+                raise KeyError(f'unknown option {k}')
+            setattr(o, k, int(v))
+        check(lib.cbs_set_options(self._h, C.byref(o)))
+
+    def plan(self):
+        f, a, b, t = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib.cbs_get_plan(self._h, C.byref(f), C.byref(a), C.byref(b), C.byref(t)))
+        return {'form': 'plain' if f.value == _lib.CBS_FORM_PLAIN else 'fused', 'a': a.value, 'b': b.value,
+                'tile_t': t.value}
+
+    # ---------------------------------------------------------------- potential (:180-396)
+    def set_RI(self, RI):
+        """Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:180-198 (scalar RI)."""
+        p = self.parameters
+        RI = np.asarray(RI)
+        if list(RI.shape[:3]) != self.expected_RI_size or (RI.ndim > 3 and int(np.prod(RI.shape[3:])) != 1):
+            raise ValueError('The refractiv index does not have the expected size : %d %d %d'
+                             % tuple(self.expected_RI_size))
+        RI = RI.reshape(self.expected_RI_size)
+        Vt = ri_to_potential(RI.astype(np.complex128), p['wavelength'], p['RI_bg'])
+        eps = float(np.max(np.abs(Vt)) * 1.01)                              # :205
+        k2 = (2 * np.pi * p['RI_bg'] / p['wavelength']) ** 2
+        if not eps > 1e-12 * k2:
+            raise ValueError('homogeneous RI: eps_imag = 0 and the Born series is undefined '
+                             '(the reference would divide by zero)')
+        b = self.boundary_thickness_pixel
+        Vpad = np.pad(Vt, [(int(b[0]),) * 2, (int(b[1]),) * 2, (int(b[2]),) * 2])   # :238-241
+        V = (Vpad - 1j * eps).astype(np.complex64)                          # :212-215
+        self.RI = RI
+        self.eps_imag = eps
+        self.Vt_roi = Vt
+        self.ROI = [b[0] + 1, b[0] + RI.shape[0], b[1] + 1, b[1] + RI.shape[1], b[2] + 1, b[2] + RI.shape[2]]
+        k0_nm = p['RI_bg'] / p['wavelength']
+        step = abs(2 * (2 * np.pi * k0_nm) / eps)                            # :326
+        res = np.array(p['resolution'], float)
+        bornmax_opt = np.ceil(np.linalg.norm(np.array(self.simulation_size) * res) / step / 2 + 1) * 2   # :327
+        it = p['iterations_number']
+        self.Bornmax = 0 if it == 0 else (int(bornmax_opt * abs(it)) if it < 0 else int(it))  # :330-336
+        acyclic = bool(p['acyclic'])
+        shift_axes = (0b100 if self.cyclic_boundary_xy else 0b111) if acyclic else 0        # :300-315
+        ramp_axes = (0b100 if (b[0] == 0 and b[1] == 0) else 0b111) if acyclic else 0       # :355-377
+        prm = _lib.PotentialParams()
+        prm.k0_nm = k0_nm
+        prm.eps_imag = eps
+        for j in range(3):
+            prm.fourier_res[j] = 1.0 / (res[j] * self.simulation_size[j])
+        for j in range(6):
+            prm.ROI[j] = int(self.ROI[j])
+        prm.Bornmax = self.Bornmax
+        prm.dV = float(np.prod(res))
+        prm.boundary_sharpness = float(p['boundary_sharpness'])
+        prm.acyclic = int(acyclic)
+        prm.shift_axes = shift_axes
+        prm.ramp_axes = ramp_axes
+        Vf = np.asfortranarray(V)
+        check(lib.cbs_set_potential(self._h, C.byref(prm), _fptr(Vf), _lib.CBS_MEM_HOST))
+        self._Vf = Vf
+
+    # ---------------------------------------------------------------- Born loop from a 3D incident field
+    def solve_incident(self, u_in):
+        """Born loop of :564-647 for incident fields u_in (x, y, z[, A]) on the RI grid;
+        returns Field[ROI] + u_in with the same shape (complex64)."""
+        u = np.asarray(u_in)
+        squeeze = u.ndim == 3
+        if squeeze:
+            u = u[..., None]
+        if list(u.shape[:3]) != self.expected_RI_size:
+            raise ValueError('incident field must have the RI grid size %s' % (self.expected_RI_size,))
+        A = u.shape[3]
+        uf = np.asfortranarray(u.astype(np.complex64))
+        out = np.empty_like(uf, order='F')
+        check(lib.cbs_solve_3d(self._h, _fptr(uf), A, _fptr(out), _lib.CBS_MEM_HOST))
+        return out[..., 0] if squeeze else out
+
+    def solve_incident_device(self, u_in_ptr, n_rhs, u_out_ptr):
+        """Same as solve_incident on device-resident complex64 buffers (pointers)."""
+        check(lib.cbs_solve_3d(self._h, C.c_void_p(int(u_in_ptr)), int(n_rhs), C.c_void_p(int(u_out_ptr)),
+                               _lib.CBS_MEM_DEVICE))
+
+    def norms(self, n_rhs):
+        out = np.zeros(n_rhs * self.Bornmax, dtype=np.float64)
+        check(lib.cbs_get_norms(self._h, out.ctypes.data_as(C.POINTER(C.c_double)), n_rhs, self.Bornmax))
+        return out.reshape(n_rhs, self.Bornmax)
+
+    def kernel_times(self):
+        ms = (C.c_double * 3)()
+        cnt = (C.c_uint64 * 3)()
+        check(lib.cbs_get_kernel_times(self._h, ms, cnt, 3))
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(['pass_real', 'pass_kspace', 'other'])}
+
+    def stream(self):
+        s = C.c_void_p()
+        check(lib.cbs_get_stream(self._h, C.byref(s)))
+        return s.value
+
+    def footprint(self, n_rhs):
+        b = C.c_uint64()
+        check(lib.cbs_footprint(self._h, int(n_rhs), C.byref(b)))
+        return b.value

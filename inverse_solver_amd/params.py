@@ -1,0 +1,71 @@
+"""Parameter structs and optics helpers of the forward-solver API.
+
+Mirrors the reference's chained get_default_parameters/update_struct config
+system (Codes/sub_code/BASIC_OPTICAL_PARAMETER.m:1-14, Codes/FORWARD_SOLVER.m:9-21,
+Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:32-45, Codes/sub_code/update_struct.m:1-15).
+"""
+from __future__ import annotations
+
+import copy
+import warnings
+
+import numpy as np
+
+
+def update_struct(base, update, warn=True):
+    """Merge ``update`` into a copy of ``base``; unknown keys warn and are still set
+    (Codes/sub_code/update_struct.m:1-15)."""
+    out = copy.deepcopy(base)
+    for k, v in (update or {}).items():
+        if k not in base and warn:
+            warnings.warn(f'Unexpected field : "{k}" was set ?! Verify the field name of your parameter structures.')
+        out[k] = copy.deepcopy(v)
+    return out
+
+
+def basic_optical_parameter(init=None):
+    """Codes/sub_code/BASIC_OPTICAL_PARAMETER.m:1-14."""
+    p = {'size': [100, 100, 100], 'wavelength': 0.532, 'NA': 1.2, 'RI_bg': 1.336,
+         'resolution': [0.1, 0.1, 0.1], 'vector_simulation': True, 'use_abbe_sine': True}
+    return update_struct(p, init) if init is not None else p
+
+
+def round_half_away(x):
+    """MATLAB round()."""
+    x = np.asarray(x, dtype=np.float64)
+    return np.sign(x) * np.floor(np.abs(x) + 0.5)
+
+
+def ri_to_potential(RI, wavelength, n_m):
+    """Vt = k^2 (RI^2 / n_m^2 - 1), k = 2 pi n_m / lambda (Codes/sub_code/RI2potential.m:1-4)."""
+    k = 2.0 * np.pi * n_m / wavelength
+    return k * k * (np.asarray(RI) ** 2 / (n_m * n_m) - 1.0)
+
+
+def fourier_grid(size, resolution):
+    """Centred spatial-frequency coordinates (cycles/um) per axis
+    (Codes/sub_code/DERIVE_OPTICAL_TOOL.m:34-47)."""
+    return [(np.arange(n) - n // 2) / (float(r) * n) for n, r in zip(size, resolution)]
+
+
+def pupil(size_xy, resolution, wavelength, NA, n_m):
+    """NA circle and cos(theta) on a centred 2D frequency grid
+    (Codes/sub_code/DERIVE_OPTICAL_TOOL.m:51-65)."""
+    kx, ky = fourier_grid(size_xy, resolution[:2])
+    kxy = np.sqrt(kx[:, None] ** 2 + ky[None, :] ** 2)
+    k0_nm = n_m / wavelength
+    na_circle = kxy < NA / wavelength
+    k3 = np.sqrt(np.maximum(k0_nm ** 2 - kxy ** 2, 0.0))
+    return na_circle, k3 / k0_nm, k3
+
+
+def centered_fft2(a):
+    """fftshift(fft2(ifftshift(a))) over the first two axes."""
+    ax = (0, 1)
+    return np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(a, axes=ax), axes=ax), axes=ax)
+
+
+def centered_ifft2(a):
+    """fftshift(ifft2(ifftshift(a))) over the first two axes."""
+    ax = (0, 1)
+    return np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(a, axes=ax), axes=ax), axes=ax)

@@ -1,0 +1,34 @@
+"""The C ABI library loads and exports every entry point include/cbs_hip.h
+declares; argument errors are reported without touching a GPU."""
+import ctypes as C
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HEADER = os.path.join(HERE, '..', 'include', 'cbs_hip.h')
+
+
+def _declared():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r'^\s*(?:int|const char\*)\s+(cbs_\w+)\s*\(', txt, re.M)))
+
+
+def test_header_declarations_are_exported():
+    from inverse_solver_amd import _lib
+    names = _declared()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(_lib.lib, n), n
+    assert set(names) == set(_lib.EXPORTED)
+    assert _lib.lib.cbs_abi_version() == 1
+
+
+def test_argument_errors_without_gpu():
+    from inverse_solver_amd import _lib
+    lib = _lib.lib
+    h = C.c_void_p()
+    assert lib.cbs_create(None, C.byref(h)) == _lib.CBS_ERR_ARG
+    assert 'null' in lib.cbs_last_error().decode()
+    assert lib.cbs_destroy(C.c_void_p(12345)) == _lib.CBS_ERR_HANDLE
+    assert 'handle' in lib.cbs_last_error().decode()
+    assert lib.cbs_solve_3d(None, None, 1, None, 0) == _lib.CBS_ERR_HANDLE

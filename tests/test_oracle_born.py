@@ -1,0 +1,154 @@
+"""Pins the oracle's Born loop (oracle/cbs.py) without reusing its formulas.
+
+* Dense brute force: on a tiny non-cubic grid the shifted Green's operators are
+  built from their closed form -- a direct sum over the explicitly shifted
+  frequency set S+ = {m + 1/4} (S- = -S+), no FFT and no phase ramp -- and the
+  two-chain series (F1 = A V G-(p0), F2 = A V G+(p0), f <- M_s f with f += F1
+  after the third step) is evaluated with matrices and compared with
+  born_loop() at every iteration count.
+* G- = G+^T: follows from S- = -S+ and the evenness of the Green's function;
+  it fixes how the flipped Greenp pairs with the conjugated ramp.
+* The attenuation mask, the ramp, eps and Bornmax against their closed forms.
+* Linearity in the incident field.
+"""
+import numpy as np
+import pytest
+
+from helpers import params, sphere_ri, random_field, rel
+from oracle.cbs import ForwardSolverConvergentBorn as Oracle
+
+
+def _tiny(ri=(4, 3, 5), iters=6, **kw):
+    p = params(ri, bt_px_target=1, iterations=iters, **kw)
+    rng = np.random.default_rng(0)
+    RI = 1.336 + 0.08 * rng.random(ri)
+    o = Oracle(p, io_kernels=False)
+    o.set_RI(RI)
+    return o, RI
+
+
+def _freq_set(n, sign):
+    """Shifted frequency indices: S+ = {m + 1/4 : m = -floor(n/2) .. n - floor(n/2) - 1}, S- = -S+."""
+    m = np.arange(n) - n // 2
+    return sign * (m + 0.25)
+
+
+def _dense_green(o, sign):
+    nx, ny, nz = o.V.shape
+    res = o.utility_border['fourier_res']
+    km2 = (2 * np.pi * o.utility_border['k0_nm']) ** 2
+    eps = o.eps_imag
+    kx, ky, kz = (_freq_set(n, sign) for n in (nx, ny, nz))
+    KX, KY, KZ = np.meshgrid(kx, ky, kz, indexing='ij')
+    g = 1.0 / (4 * np.pi ** 2 * ((KX * res[0]) ** 2 + (KY * res[1]) ** 2 + (KZ * res[2]) ** 2) - km2 - 1j * eps)
+    d = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing='ij'), -1).reshape(-1, 3)
+    N = nx * ny * nz
+    kap = np.stack([KX.ravel(), KY.ravel(), KZ.ravel()], -1)            # (N, 3)
+    nn = np.array([nx, ny, nz], float)
+    # K(d, d') = (1/N) sum_kappa g(kappa) exp(2 pi i kappa . (d - d') / n)
+    ph_d = np.exp(2j * np.pi * (d / nn) @ kap.T)                        # (N, N) over (d, kappa)
+    return (ph_d * g.ravel()[None, :]) @ ph_d.conj().T / N
+
+
+def test_green_minus_is_transpose_of_green_plus():
+    o, _ = _tiny()
+    Gp = _dense_green(o, +1)
+    Gm = _dense_green(o, -1)
+    np.testing.assert_allclose(Gm, Gp.T, rtol=0, atol=1e-12 * np.abs(Gp).max())
+
+
+@pytest.mark.parametrize('ri', [(4, 3, 5), (4, 4, 4)])
+def test_born_loop_equals_dense_series(ri):
+    iters = 7
+    o, RI0 = _tiny(ri, iters)
+    shp = o.V.shape
+    Gp = _dense_green(o, +1)
+    Gm = _dense_green(o, -1)
+    V = o.V.ravel(order='C')
+    # closed-form mask: min(dist / (b - 0.5), 1) per axis, dist to the nearest edge
+    b = o.boundary_thickness_pixel
+    A = np.ones(shp)
+    for j, n in enumerate(shp):
+        dist = np.minimum(np.arange(n), n - 1 - np.arange(n)).astype(float)
+        fac = np.minimum(dist / (b[j] - 0.5), 1.0) if b[j] > 0 else np.ones(n)
+        s = [1, 1, 1]
+        s[j] = -1
+        A = A * fac.reshape(s)
+    np.testing.assert_allclose(o.attenuation_mask * np.ones(shp), A, atol=1e-15)
+    A = A.ravel(order='C')
+    eps = o.eps_imag
+    gam = 1j * V / eps
+    R = o.ROI
+    roi = (slice(R[0] - 1, R[1]), slice(R[2] - 1, R[3]), slice(R[4] - 1, R[5]))
+    u_in = random_field(tuple(o.expected_RI_size), 3)
+    S = np.zeros(shp, complex)
+    S[roi] = (o.V[roi] + 1j * eps) * u_in
+    p0 = ((1j / eps) * S / 2).ravel(order='C')
+    F1 = A * V * (Gm @ p0)
+    F2 = A * V * (Gp @ p0)
+    M = {+1: lambda x: A * (x - gam * x + V * (Gp @ (gam * x))),
+         -1: lambda x: A * (x - gam * x + V * (Gm @ (gam * x)))}
+    for it in range(1, iters + 1):
+        Field = F1.copy()
+        if it >= 2:
+            Field = Field + F2
+            f = F2.copy()
+        for jj in range(3, it + 1):
+            s = -1 if jj % 2 else +1
+            f = M[s](f)
+            Field = Field + f
+            if jj == 3:
+                f = f + F1
+        expect = Field.reshape(shp, order='C')[roi] + u_in
+        o.set_RI(RI0)          # resets the persistent flip state
+        o.Bornmax = it
+        got = o.born_loop(u_in)
+        assert rel(got, expect) < 1e-12, (ri, it)
+
+
+def test_linear_in_incident_field():
+    o, _ = _tiny((4, 4, 4), 5)
+    R = o.ROI
+    roi = (slice(R[0] - 1, R[1]), slice(R[2] - 1, R[3]), slice(R[4] - 1, R[5]))
+    u1 = random_field((4, 4, 4), 1)
+    u2 = random_field((4, 4, 4), 2)
+    a, c = 0.7 - 0.2j, -1.3 + 0.5j
+    RI = o.RI[roi]
+    outs = []
+    for u in (u1, u2, a * u1 + c * u2):
+        o.set_RI(RI)            # resets the persistent flip state between calls
+        outs.append(o.born_loop(u) - u)
+    assert rel(outs[2], a * outs[0] + c * outs[1]) < 1e-12
+
+
+def test_homogeneous_ri_is_rejected():
+    p = params((4, 4, 4), bt_px_target=1, iterations=2)
+    o = Oracle(p, io_kernels=False)
+    with pytest.raises(ValueError):
+        o.set_RI(np.full((4, 4, 4), 1.336))
+
+
+def test_eps_ramp_and_bornmax_closed_forms():
+    ri = (6, 5, 4)
+    p = params(ri, bt_px_target=2, iterations=-2)
+    RI = sphere_ri(ri, 0.2, n_s=1.40)
+    o = Oracle(p, io_kernels=False)
+    o.set_RI(RI)
+    k = 2 * np.pi * 1.336 / 0.532
+    vt_max = k ** 2 * (1.40 ** 2 / 1.336 ** 2 - 1)
+    assert abs(o.eps_imag - 1.01 * vt_max) < 1e-9 * vt_max
+    n = np.array(o.V.shape)
+    step = 2 * k / o.eps_imag
+    bm = int(np.ceil(np.linalg.norm(n * 0.1) / step / 2 + 1) * 2) * 2
+    assert o.Bornmax == bm
+    # ramp: exp(-i pi (d - floor(n/2)) / (2 n)) per axis, unit at the centre
+    for j in range(3):
+        d = np.arange(n[j])
+        expect = np.exp(-1j * np.pi * (d - n[j] // 2) / (2 * n[j]))
+        idx = [n[0] // 2, n[1] // 2, n[2] // 2]
+        line = []
+        for dd in d:
+            ii = list(idx)
+            ii[j] = dd
+            line.append(o.phase_ramp[tuple(min(ii[q], o.phase_ramp.shape[q] - 1) for q in range(3))])
+        np.testing.assert_allclose(np.array(line), expect, atol=1e-14)
