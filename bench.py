@@ -1,0 +1,248 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched-angle convergent Born series forward solve on one MI355X.
+
+Workload (one "step" = one complete solve):
+  RI grid 128^3 at 0.1 um plus a 16-pixel absorbing boundary per side -> 160^3
+  simulation grid; 8 seeded non-overlapping spheres (radii U[0.5, 2] um,
+  n U[1.36, 1.48]) in n_m = 1.336 at lambda = 0.532 um; 49 plane-wave
+  illuminations (normal incidence + 48 on a circle at 0.95 * NA 1.2) solved as
+  right-hand sides sharing the potential; 200 Born iterations; complex64.
+  The incident 3D fields are resident in HBM when the timed region starts.
+
+Metric: Gvoxel-iter/s = 160^3 * 200 * 49 / seconds per solve (higher is better),
+with p50/p99 solve latency, the relative L2 error against the FP64 oracle, the
+HBM roofline of the dominant kernel (pass_real) and the complex64 CPU port of
+the reference loop timed on this host's cores on a bounded sample.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = 'Gvoxel-iter/s at ≤1e-5 rel err vs FP64 oracle; p50/p99 solve latency'
+
+
+def log(*a):
+    print('[bench]', *a, file=sys.stderr, flush=True)
+
+
+def workload_params(ri, iters, A_fields):
+    from inverse_solver_amd import get_default_parameters
+    return get_default_parameters(dict(
+        size=[ri[0], ri[1], ri[2]], wavelength=0.532, NA=1.2, RI_bg=1.336, resolution=[0.1, 0.1, 0.1],
+        vector_simulation=False, use_abbe_sine=True, iterations_number=iters,
+        boundary_thickness=[8, 8, 8], boundary_sharpness=1, acyclic=True,
+        return_transmission=True, return_reflection=False, return_3D=True))
+
+
+def pass_real_bytes(nvox, A, M):
+    """Algorithmic HBM bytes of every pass_real launch of one solve (complex64 = 8 B).
+
+    Per launch and voxel: the spectrum is read when the launch completes an
+    iteration and written when it starts the next one; f is read by the full
+    update (jj >= 3) and as the source term p0 (jj = 0, 1) and written when the
+    next p is gamma*f; Field is read by the accumulating updates (jj >= 2) and
+    written by every completing launch; V (shared by the A right-hand sides)
+    is read once per voxel."""
+    def one(jj):
+        upd = 0 if jj == 0 else (1 if jj == 1 else (2 if jj == 2 else 3))
+        fwd = 0 if jj == M else (1 if jj <= 1 else 2)
+        rd = (upd != 0) + (upd == 3 or fwd == 1) + (upd >= 2)
+        wr = (upd != 0) + (fwd == 2) + (fwd != 0)
+        return 8 * A * (rd + wr) + 8
+    return nvox * np.array([one(jj) for jj in range(0, M + 1)], dtype=np.float64)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--ri', type=int, nargs=3, default=[128, 128, 128])
+    ap.add_argument('--angles', type=int, default=49)
+    ap.add_argument('--iters', type=int, default=200)
+    ap.add_argument('--seed', type=int, default=2)
+    ap.add_argument('--check-iters', type=int, default=24,
+                    help='iterations of the FP64-oracle accuracy check (1 angle, same grid)')
+    ap.add_argument('--cpu-iters', type=int, default=30,
+                    help='iterations of the complex64 CPU-port sample (1 angle, same grid)')
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline and the oracle check')
+    ap.add_argument('--no-plain', action='store_true', help='skip the plain single-angle comparison')
+    ap.add_argument('--no-tune', action='store_true', help='use the planner default instead of the autotuner')
+    ap.add_argument('--copy-probe-mb', type=int, default=0,
+                    help='run one device-to-device copy of this many MiB first (calibrates HBM byte counters)')
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({'error': 'this build runs on one GPU; --gpus %d is not supported (the angle-parallel '
+                                   'sharding across GPUs is described in DESIGN.md)' % args.gpus}))
+        sys.exit(2)
+
+    from inverse_solver_amd import FORWARD_SOLVER_CONVERGENT_BORN
+    from inverse_solver_amd import device as dv
+    from inverse_solver_amd.phantoms import spheres, incident_plane_waves
+
+    ri = list(args.ri)
+    A, M = args.angles, args.iters
+    log('building phantom', ri, 'angles', A, 'iterations', M)
+    RI, _ = spheres(tuple(ri), 8, (0.5, 2.0), (1.36, 1.48), seed=args.seed)
+    u = incident_plane_waves(ri, A, 1.2, 0.532, 1.336, [0.1, 0.1, 0.1])
+    p = workload_params(ri, M, A)
+    s = FORWARD_SOLVER_CONVERGENT_BORN(p, options=dict(time_kernels=1))
+    s.set_RI(RI)
+    sim = s.simulation_size
+    nvox = int(np.prod(sim))
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        s.autotune(A, 6)
+        log('autotune %.1f s' % (time.perf_counter() - t0))
+    plan = s.plan()
+    log('sim grid', sim, 'plan', plan, 'device', dv.device_name(0))
+
+    if args.copy_probe_mb > 0:
+        n = args.copy_probe_mb * (1 << 20) // 8
+        ca, cb = dv.DeviceArray((n,), np.complex64), dv.DeviceArray((n,), np.complex64)
+        cb.copy_from(ca)
+        dv.synchronize()
+        log('copy probe: %d MiB device-to-device' % args.copy_probe_mb)
+        ca.free()
+        cb.free()
+
+    # inputs resident in HBM: per-rhs x-fastest blocks
+    u_host = np.ascontiguousarray(np.transpose(u, (3, 2, 1, 0)))
+    d_in = dv.DeviceArray.from_host(u_host)
+    d_out = dv.DeviceArray(u_host.shape, np.complex64)
+    stream = s.stream()
+
+    for w in range(args.warmup):
+        s.solve_incident_device(d_in.ptr, A, d_out.ptr)
+        log('warmup', w + 1, 'done')
+    dv.synchronize()
+
+    step_ms = []
+    pr_ms = pr_n = pk_ms = pk_n = 0.0
+    e0, e1 = dv.Event(), dv.Event()
+    for k in range(args.steps):
+        e0.record(stream)
+        s.solve_incident_device(d_in.ptr, A, d_out.ptr)
+        e1.record(stream)
+        e1.synchronize()
+        step_ms.append(e0.elapsed_ms(e1))
+        kt = s.kernel_times()
+        pr_ms += kt['pass_real'][0]
+        pr_n += kt['pass_real'][1]
+        pk_ms += kt['pass_kspace'][0]
+        pk_n += kt['pass_kspace'][1]
+        log('step', k + 1, '%.1f ms' % step_ms[-1])
+    dv.synchronize()
+    ms = float(np.mean(step_ms))
+    vox_iters = float(nvox) * M * A
+    value = vox_iters / (ms / 1e3) / 1e9
+    p50, p99 = float(np.percentile(step_ms, 50)), float(np.percentile(step_ms, 99))
+
+    # roofline of the dominant kernel
+    bytes_pr = float(pass_real_bytes(nvox, A, M).sum())   # per solve
+    pr_avg_ms = pr_ms / max(pr_n, 1)
+    pr_launches = pr_n / args.steps
+    achieved_pr = bytes_pr / pr_launches / (pr_avg_ms / 1e3) / 1e9
+    bytes_pk_launch = 16.0 * A * nvox
+    pk_avg_ms = pk_ms / max(pk_n, 1)
+    achieved_pk = bytes_pk_launch / (pk_avg_ms / 1e3) / 1e9
+    log('pass_real avg %.3f ms  %.0f GB/s; pass_kspace avg %.3f ms  %.0f GB/s'
+        % (pr_avg_ms, achieved_pr, pk_avg_ms, achieved_pk))
+
+    result_gpu = d_out.to_host()
+    out = {
+        'metric': METRIC, 'value': round(value, 3), 'unit': 'Gvoxel-iter/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': round(ms, 3), 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'complex64', 'data': 'synthetic',
+        'config': {'workload': 'batched-angle CBS forward solve: %dx%dx%d RI + %d-px boundary = %dx%dx%d sim, '
+                               '%d plane-wave RHS sharing V, %d iterations, 8 seeded spheres'
+                               % (ri[0], ri[1], ri[2], (sim[0] - ri[0]) // 2, sim[0], sim[1], sim[2], A, M),
+                   'sim_grid': sim, 'ri_grid': ri, 'n_rhs': A, 'iterations': M, 'seed': args.seed,
+                   'plan': plan, 'parallelism': 'single GPU, all angles in each launch'},
+        'latency_ms': {'p50': round(p50, 3), 'p99': round(p99, 3)},
+        'roofline': {'bound': 'hbm', 'achieved': round(achieved_pr, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': round(achieved_pr / HBM_PEAK_GBS, 4), 'traffic': None,
+                     'kernel': 'pass_real', 'avg_launch_ms': round(pr_avg_ms, 4),
+                     'algorithmic_bytes_per_launch': round(bytes_pr / pr_launches),
+                     'pass_kspace': {'achieved': round(achieved_pk, 1), 'avg_launch_ms': round(pk_avg_ms, 4),
+                                     'algorithmic_bytes_per_launch': bytes_pk_launch}},
+    }
+    s.close()
+
+    if not args.no_plain:
+        # plain single-angle form (unfused per-axis kernels, batch 1) on the same inputs
+        sp_ = FORWARD_SOLVER_CONVERGENT_BORN(p, options=dict(form=1))
+        sp_.set_RI(RI)
+        d1 = dv.DeviceArray.from_host(u_host[:1])
+        d1o = dv.DeviceArray(u_host[:1].shape, np.complex64)
+        sp_.solve_incident_device(d1.ptr, 1, d1o.ptr)
+        dv.synchronize()
+        st = sp_.stream()
+        e0.record(st)
+        sp_.solve_incident_device(d1.ptr, 1, d1o.ptr)
+        e1.record(st)
+        e1.synchronize()
+        pms = e0.elapsed_ms(e1)
+        plain_val = float(nvox) * M / (pms / 1e3) / 1e9
+        out['plain_single_angle'] = {'value': round(plain_val, 3), 'unit': 'Gvoxel-iter/s',
+                                     'ms_per_angle': round(pms, 3),
+                                     'speedup_of_batched_fused': round(value / plain_val, 2)}
+        diff = np.linalg.norm(d1o.to_host() - result_gpu[:1]) / np.linalg.norm(result_gpu[:1])
+        out['plain_single_angle']['rel_diff_vs_batched'] = float(diff)
+        sp_.close()
+        log('plain single angle %.1f ms -> %.2f Gvoxel-iter/s' % (pms, plain_val))
+
+    if not args.no_cpu:
+        from oracle.cbs import ForwardSolverConvergentBorn as Oracle
+        cores = os.cpu_count() or 1
+        u0 = u[..., 0]
+        # accuracy: 1 angle, check_iters iterations, GPU vs FP64 oracle on the same grid
+        pc = workload_params(ri, args.check_iters, 1)
+        t0 = time.perf_counter()
+        o = Oracle(pc, io_kernels=False, workers=cores)
+        o.set_RI(RI)
+        ref = o.born_loop(u0)
+        t_or = time.perf_counter() - t0
+        sg = FORWARD_SOLVER_CONVERGENT_BORN(pc)
+        sg.set_RI(RI)
+        got = sg.solve_incident(u0)
+        sg.close()
+        err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        out['accuracy'] = {'rel_l2_err_vs_fp64_oracle': err, 'angles': 1, 'iterations': args.check_iters,
+                           'grid': sim, 'tolerance': 1e-5, 'pass': err <= 1e-5,
+                           'oracle_seconds': round(t_or, 2)}
+        log('accuracy rel err %.3e (%d iterations, oracle %.1f s)' % (err, args.check_iters, t_or))
+        # CPU baseline: complex64 op-for-op port of the reference loop on this host
+        pcpu = workload_params(ri, args.cpu_iters, 1)
+        oc = Oracle(pcpu, dtype=np.complex64, io_kernels=False, workers=cores)
+        oc.set_RI(RI)
+        oc.born_loop(u0)          # warm-up (plans, page-in)
+        oc.set_RI(RI)
+        t0 = time.perf_counter()
+        oc.born_loop(u0)
+        t_cpu = time.perf_counter() - t0
+        cpu_val = float(nvox) * args.cpu_iters / t_cpu / 1e9
+        out['cpu_baseline'] = {'value': round(cpu_val, 5), 'unit': 'Gvoxel-iter/s', 'cores': cores, 'kind': 'port',
+                               'sample': '1 angle x %d iterations on the %dx%dx%d grid (complex64 NumPy/SciPy port '
+                                         'of solve_raw, scipy.fft workers=%d), %.1f s'
+                                         % (args.cpu_iters, sim[0], sim[1], sim[2], cores, t_cpu)}
+        log('cpu baseline %.4f Gvoxel-iter/s on %d cores' % (cpu_val, cores))
+
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

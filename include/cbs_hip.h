@@ -82,6 +82,8 @@ typedef struct {
   int32_t track_norms;  /* 1: accumulate FP64 |f_n|^2 per rhs and iteration */
   int32_t use_graph;    /* 1: capture the iteration loop in a hipGraph and replay it */
   int32_t time_kernels; /* 1: time each pass kind with HIP events (cbs_get_kernel_times) */
+  int32_t threads_real;   /* fused form: force threads per block of the real-space pass (0 = planner) */
+  int32_t threads_kspace; /* fused form: force threads per block of the k-space pass (0 = planner) */
 } cbs_options;
 
 /* Creates a solver on geo->device.  Replaces CudaBorn action 0 (main.cpp:57-101). */
@@ -137,6 +139,14 @@ int cbs_footprint(cbs_handle h, uint32_t n_rhs, uint64_t* bytes);
 
 /* Kernel plan chosen for the current options: form, a, b, tile_t. */
 int cbs_get_plan(cbs_handle h, int32_t* form, int32_t* a, int32_t* b, int32_t* tile_t);
+
+/* Autotuner (no counterpart in the reference, whose cuFFT plan is fixed): times
+ * every fused kernel plan available for the grid on the current potential with
+ * n_rhs right-hand sides and `iterations` Born iterations each, and keeps the
+ * fastest as the handle's plan (overriding split_a/tile_t/threads options).
+ * Leaves the alternating-shift state untouched.  out_ms (may be null) receives
+ * the best time per solve of the trial. */
+int cbs_autotune(cbs_handle h, uint32_t n_rhs, uint32_t iterations, double* out_ms);
 
 const char* cbs_last_error(void);
 int cbs_abi_version(void);

@@ -19,7 +19,7 @@ CBS_FORM_FUSED, CBS_FORM_PLAIN = 0, 1
 EXPORTED = [
     'cbs_create', 'cbs_destroy', 'cbs_set_options', 'cbs_get_options', 'cbs_set_potential', 'cbs_solve_3d',
     'cbs_set_io_kernels', 'cbs_solve_2d', 'cbs_get_norms', 'cbs_get_kernel_times', 'cbs_get_stream',
-    'cbs_footprint', 'cbs_get_plan', 'cbs_last_error', 'cbs_abi_version',
+    'cbs_footprint', 'cbs_get_plan', 'cbs_autotune', 'cbs_last_error', 'cbs_abi_version',
 ]
 
 
@@ -43,7 +43,8 @@ class PotentialParams(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [('form', C.c_int32), ('split_a', C.c_int32), ('tile_t', C.c_int32), ('max_batch', C.c_int32),
-                ('track_norms', C.c_int32), ('use_graph', C.c_int32), ('time_kernels', C.c_int32)]
+                ('track_norms', C.c_int32), ('use_graph', C.c_int32), ('time_kernels', C.c_int32),
+                ('threads_real', C.c_int32), ('threads_kspace', C.c_int32)]
 
 
 def _load():
@@ -67,6 +68,7 @@ def _load():
         'cbs_footprint': ([vp, u32, C.POINTER(C.c_uint64)], i32),
         'cbs_get_plan': ([vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                           C.POINTER(C.c_int32)], i32),
+        'cbs_autotune': ([vp, u32, u32, C.POINTER(C.c_double)], i32),
         'cbs_last_error': ([], C.c_char_p),
         'cbs_abi_version': ([], i32),
     }

@@ -19,12 +19,13 @@
 
 namespace cbs {
 
-// Update kinds for pass_real / update_plain (see the recurrence in solver.cpp).
+// Update kinds of the pointwise step (the two-chain recurrence of
+// Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:587-642).
 enum UpdKind : int {
   UPD_NONE = 0,      // no inverse input: p comes from the source term
   UPD_VQ_SET = 1,    // jj = 1 : f1 = A V q;            Field  = f1
   UPD_VQ_ADD = 2,    // jj = 2 : f2 = A V q;            Field += f2; x = f2
-  UPD_FULL3 = 3,     // jj = 3 : f3 = A (x - gx + V q); Field += f3; x = f3 + F1
+  UPD_FULL3 = 3,     // jj = 3 : f3 = A (x - gx + V q); Field += f3; x = f3 + F1 (F1 = Field - x)
   UPD_FULL = 4,      // jj > 3 : fn = A (x - gx + V q); Field += fn; x = fn
 };
 enum FwdKind : int {
@@ -72,54 +73,45 @@ struct StateArgs {
 // flip state of iteration jj for rhs r: 1 = Greenp flipped (sigma = -1), 0 = sigma = +1
 __device__ __forceinline__ int flip_of(int jj, const uint8_t* parity, int r) { return (jj + parity[r]) & 1; }
 
-__device__ __forceinline__ float2 ramp3(const Tables& tb, int x, int y, int z) {
-  return cmul(cmul(tb.ramp_x[x], tb.ramp_y[y]), tb.ramp_z[z]);
-}
+// Which state arrays a pointwise step reads.
+__host__ __device__ constexpr bool reads_f(int upd, int fwd) { return upd == UPD_FULL3 || upd == UPD_FULL || fwd == FWD_P0; }
+__host__ __device__ constexpr bool reads_F(int upd) { return upd == UPD_VQ_ADD || upd == UPD_FULL3 || upd == UPD_FULL; }
 
-// The pointwise body shared by the fused and the plain forms.
-//   q      : G_s applied to the previous p, before the conj ramp (ignored for UPD_NONE)
-//   returns the new p (times the forward ramp) to be transformed, or 0
-__device__ __forceinline__ float2 born_point(int upd, int fwd, float2 q, long long g, int x, int y, int z,
-                                             const Tables& tb, const Consts& c, float2* __restrict__ F,
-                                             float2* __restrict__ f, const float2* __restrict__ V, int flip_inv,
-                                             int flip_fwd, double& nrm) {
-  const float2 v = V[g];
-  const float2 r = ramp3(tb, x, y, z);
-  float2 xnew = make_float2(0.f, 0.f);
+// Pointwise step on one voxel.
+//   q   : G_s applied to the previous p, before the conj ramp (unused for UPD_NONE)
+//   r   : sigma=+1 ramp at the voxel, m: attenuation mask, v: potential
+//   xo  : fstate value (p0 for FWD_P0, x for the full update), Fo: Field value
+// Outputs: Fn (new Field, when upd != NONE), xn (new fstate, when fwd == GAMMA),
+// returns the next p times the forward ramp (0 when fwd == NONE).
+__device__ __forceinline__ float2 born_step(int upd, int fwd, float2 q, float2 r, float m, float2 v, float2 xo,
+                                            float2 Fo, float inv_eps, int flip_inv, int flip_fwd, float2& Fn,
+                                            float2& xn, double& nrm) {
+  xn = make_float2(0.f, 0.f);
   if (upd != UPD_NONE) {
     // q * conj(ramp_s): ramp_s = r (sigma=+1) or conj(r) (sigma=-1)
     const float2 qq = flip_inv ? cmul(q, r) : cmulc(q, r);
-    const float m = tb.mask_x[x] * tb.mask_y[y] * tb.mask_z[z];
     const float2 vq = cmul(v, qq);
-    if (upd == UPD_VQ_SET) {
-      const float2 fn = cscale(vq, m);
-      F[g] = fn;
-      nrm += (double)fn.x * fn.x + (double)fn.y * fn.y;
-    } else if (upd == UPD_VQ_ADD) {
-      const float2 fn = cscale(vq, m);
-      F[g] = cadd(F[g], fn);
-      xnew = fn;
-      nrm += (double)fn.x * fn.x + (double)fn.y * fn.y;
+    float2 fn;
+    if (upd == UPD_VQ_SET || upd == UPD_VQ_ADD) {
+      fn = cscale(vq, m);
+      Fn = (upd == UPD_VQ_SET) ? fn : cadd(Fo, fn);
+      xn = fn;
     } else {
-      const float2 xo = f[g];
-      // gamma x = (i/eps) v x
       const float2 vx = cmul(v, xo);
-      const float2 gx = make_float2(-vx.y * c.inv_eps, vx.x * c.inv_eps);
-      const float2 fn = cscale(cadd(csub(xo, gx), vq), m);
-      const float2 Fo = F[g];
-      F[g] = cadd(Fo, fn);
-      xnew = (upd == UPD_FULL3) ? cadd(fn, csub(Fo, xo)) : fn;
-      nrm += (double)fn.x * fn.x + (double)fn.y * fn.y;
+      const float2 gx = make_float2(-vx.y * inv_eps, vx.x * inv_eps);   // gamma x = (i/eps) v x
+      fn = cscale(cadd(csub(xo, gx), vq), m);
+      Fn = cadd(Fo, fn);
+      xn = (upd == UPD_FULL3) ? cadd(fn, csub(Fo, xo)) : fn;
     }
+    nrm += (double)fn.x * fn.x + (double)fn.y * fn.y;
   }
   if (fwd == FWD_NONE) return make_float2(0.f, 0.f);
   float2 p;
   if (fwd == FWD_P0) {
-    p = f[g];
+    p = xo;
   } else {
-    if (upd != UPD_VQ_SET) f[g] = xnew;
-    const float2 vx = cmul(v, xnew);
-    p = make_float2(-vx.y * c.inv_eps, vx.x * c.inv_eps);
+    const float2 vx = cmul(v, xn);
+    p = make_float2(-vx.y * inv_eps, vx.x * inv_eps);
   }
   return flip_fwd ? cmulc(p, r) : cmul(p, r);
 }
@@ -131,5 +123,9 @@ __device__ __forceinline__ void flush_norm(double v, double* dst) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(dst, v);
 }
+
+__device__ __forceinline__ float2 lo2(float4 v) { return make_float2(v.x, v.y); }
+__device__ __forceinline__ float2 hi2(float4 v) { return make_float2(v.z, v.w); }
+__device__ __forceinline__ float4 pack2(float2 a, float2 b) { return make_float4(a.x, a.y, b.x, b.y); }
 
 }  // namespace cbs

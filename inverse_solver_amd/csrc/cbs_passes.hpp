@@ -10,7 +10,10 @@ namespace cbs {
 __host__ __device__ constexpr int row_pitch(int n) { return (n % 2 == 0) ? n + 1 : n; }
 
 // ------------------------------------------------------------------ pass_real
-// One block per (rhs, y2, z).  Tile = a rows (y1) of nx points.
+// One block per (rhs, y2, z).  Tile = A rows (y1) of NX points.
+// Global traffic is issued in bulk: each thread loads its share of the tile as
+// 16-byte pairs before touching LDS, and the pointwise step loads V, f and Field
+// for CH pairs at a time.  Twiddles and per-axis ramp/mask factors live in LDS.
 struct PassRArgs {
   Tables tb;
   Consts c;
@@ -20,8 +23,20 @@ struct PassRArgs {
 
 template <int NX, int A, int NT>
 __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
+  static_assert(NX % 2 == 0, "pass_real processes x in pairs");
   constexpr int PR = row_pitch(NX);
+  constexpr int E2 = A * NX / 2;              // complex pairs in the tile
+  constexpr int L = (E2 + NT - 1) / NT;       // pairs per thread
+  constexpr int CH = L < 4 ? L : 4;           // pointwise chunk (pairs)
   __shared__ float2 lds[A * PR];
+  __shared__ float2 s_twx[NX];
+  __shared__ float2 s_twa[A];
+  __shared__ float2 s_rx[NX];
+  __shared__ float s_mx[NX];
+  __shared__ float2 s_ryz[A];
+  __shared__ float s_myz[A];
+  __shared__ float2 s_cr[A];
+
   const Consts& c = args.c;
   const Tables& tb = args.tb;
   const int tid = threadIdx.x;
@@ -31,54 +46,134 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
   const int y2 = tile % c.b;
   const int z = tile / c.b;
   const long long roff = (long long)r * args.st.rhs_stride;
-  float2* __restrict__ spec = args.st.spec + roff + (long long)(z * c.b + y2) * (A * NX);
-  float2* __restrict__ F = args.st.field + roff;
-  float2* __restrict__ f = args.st.fstate + roff;
+  float4* __restrict__ spec4 = reinterpret_cast<float4*>(args.st.spec + roff + (long long)(z * c.b + y2) * (A * NX));
+  const int upd = args.upd, fwd = args.fwd;
   const int flip_inv = args.jj_inv ? flip_of(args.jj_inv, args.st.parity, r) : 0;
   const int flip_fwd = args.jj_fwd ? flip_of(args.jj_fwd, args.st.parity, r) : 0;
+
+  // bulk load of the spectrum tile (all loads in flight before LDS writes)
+  float4 reg[L];
+  if (upd != UPD_NONE) {
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+      const int i = tid + k * NT;
+      if (i < E2) reg[k] = spec4[i];
+    }
+  }
+  for (int i = tid; i < NX; i += NT) {
+    s_twx[i] = tb.tw_x[i];
+    s_rx[i] = tb.ramp_x[i];
+    s_mx[i] = tb.mask_x[i];
+  }
+  for (int i = tid; i < A; i += NT) {
+    s_twa[i] = tb.tw_a[i];
+    const int y = i * c.b + y2;
+    s_ryz[i] = cmul(tb.ramp_y[y], tb.ramp_z[z]);
+    s_myz[i] = tb.mask_y[y] * tb.mask_z[z];
+    s_cr[i] = tb.cross[y2 * A + i];
+  }
+  __syncthreads();
 
   auto row = [](int l, int e) { return l * PR + e; };
   auto col = [](int l, int e) { return e * PR + l; };
 
-  if (args.upd != UPD_NONE) {
-    const float2* __restrict__ cr = tb.cross + y2 * A;
-    for (int i = tid; i < A * NX; i += NT) {
-      const int p1 = i / NX, px = i - p1 * NX;
-      float2 v = spec[i];
-      if (c.b > 1) v = cmulc(v, cr[p1]);
-      lds[p1 * PR + px] = v;
+  if (upd != UPD_NONE) {
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+      const int i = tid + k * NT;
+      if (i < E2) {
+        const int e = 2 * i;
+        const int p1 = e / NX, px = e - p1 * NX;
+        float2 v0 = lo2(reg[k]), v1 = hi2(reg[k]);
+        if (c.b > 1) {
+          const float2 w = s_cr[p1];
+          v0 = cmulc(v0, w);
+          v1 = cmulc(v1, w);
+        }
+        lds[p1 * PR + px] = v0;
+        lds[p1 * PR + px + 1] = v1;
+      }
     }
     __syncthreads();
-    if constexpr (A > 1) line_fft<A, true, NX, NT>(lds, col, tb.tw_a, tid);
-    line_fft<NX, true, A, NT>(lds, row, tb.tw_x, tid);
+    if constexpr (A > 1) line_fft<A, true, NX, NT>(lds, col, s_twa, tid);
+    line_fft<NX, true, A, NT>(lds, row, s_twx, tid);
   }
+
+  // pointwise update, CH pairs per thread at a time
+  const bool rf = reads_f(upd, fwd), rF = reads_F(upd);
+  const bool wF = upd != UPD_NONE, wf = fwd == FWD_GAMMA;
+  const float2* __restrict__ V = args.st.V;
+  float2* __restrict__ F = args.st.field + roff;
+  float2* __restrict__ f = args.st.fstate + roff;
   double nrm = 0.0;
-  for (int i = tid; i < A * NX; i += NT) {
-    const int y1 = i / NX, x = i - y1 * NX;
-    const int y = y1 * c.b + y2;
-    const long long g = ((long long)z * c.ny + y) * NX + x;
-    const float2 q = (args.upd != UPD_NONE) ? lds[y1 * PR + x] : make_float2(0.f, 0.f);
-    const float2 p = born_point(args.upd, args.fwd, q, g, x, y, z, tb, c, F, f, args.st.V, flip_inv, flip_fwd, nrm);
-    if (args.fwd != FWD_NONE) lds[y1 * PR + x] = p;
+#pragma unroll
+  for (int k0 = 0; k0 < L; k0 += CH) {
+    float4 vv[CH], xx[CH], FF[CH];
+    long long gi[CH];
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      const int i = tid + (k0 + k) * NT;
+      gi[k] = -1;
+      if (k0 + k < L && i < E2) {
+        const int e = 2 * i;
+        const int y1 = e / NX, x = e - y1 * NX;
+        const long long g = ((long long)z * c.ny + (y1 * c.b + y2)) * NX + x;
+        gi[k] = g;
+        vv[k] = *reinterpret_cast<const float4*>(V + g);
+        if (rf) xx[k] = *reinterpret_cast<const float4*>(f + g);
+        if (rF) FF[k] = *reinterpret_cast<const float4*>(F + g);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      if (gi[k] < 0) continue;
+      const int i = tid + (k0 + k) * NT;
+      const int e = 2 * i;
+      const int y1 = e / NX, x = e - y1 * NX;
+      const float2 ryz = s_ryz[y1];
+      const float myz = s_myz[y1];
+      const float4 xo4 = rf ? xx[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float4 Fo4 = rF ? FF[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+      float2 Fn0, Fn1, xn0, xn1;
+      const float2 q0 = upd != UPD_NONE ? lds[y1 * PR + x] : make_float2(0.f, 0.f);
+      const float2 q1 = upd != UPD_NONE ? lds[y1 * PR + x + 1] : make_float2(0.f, 0.f);
+      const float2 p0 = born_step(upd, fwd, q0, cmul(s_rx[x], ryz), s_mx[x] * myz, lo2(vv[k]), lo2(xo4), lo2(Fo4),
+                                  c.inv_eps, flip_inv, flip_fwd, Fn0, xn0, nrm);
+      const float2 p1 = born_step(upd, fwd, q1, cmul(s_rx[x + 1], ryz), s_mx[x + 1] * myz, hi2(vv[k]), hi2(xo4),
+                                  hi2(Fo4), c.inv_eps, flip_inv, flip_fwd, Fn1, xn1, nrm);
+      if (wF) *reinterpret_cast<float4*>(F + gi[k]) = pack2(Fn0, Fn1);
+      if (wf) *reinterpret_cast<float4*>(f + gi[k]) = pack2(xn0, xn1);
+      if (fwd != FWD_NONE) {
+        lds[y1 * PR + x] = p0;
+        lds[y1 * PR + x + 1] = p1;
+      }
+    }
   }
   if (args.st.norms) flush_norm(nrm, args.st.norms + r);
-  if (args.fwd == FWD_NONE) return;
+  if (fwd == FWD_NONE) return;
   __syncthreads();
-  line_fft<NX, false, A, NT>(lds, row, tb.tw_x, tid);
-  if constexpr (A > 1) line_fft<A, false, NX, NT>(lds, col, tb.tw_a, tid);
-  {
-    const float2* __restrict__ cr = tb.cross + y2 * A;
-    for (int i = tid; i < A * NX; i += NT) {
-      const int p1 = i / NX, px = i - p1 * NX;
-      float2 v = lds[p1 * PR + px];
-      if (c.b > 1) v = cmul(v, cr[p1]);
-      spec[i] = v;
+  line_fft<NX, false, A, NT>(lds, row, s_twx, tid);
+  if constexpr (A > 1) line_fft<A, false, NX, NT>(lds, col, s_twa, tid);
+#pragma unroll
+  for (int k = 0; k < L; ++k) {
+    const int i = tid + k * NT;
+    if (i < E2) {
+      const int e = 2 * i;
+      const int p1 = e / NX, px = e - p1 * NX;
+      float2 v0 = lds[p1 * PR + px], v1 = lds[p1 * PR + px + 1];
+      if (c.b > 1) {
+        const float2 w = s_cr[p1];
+        v0 = cmul(v0, w);
+        v1 = cmul(v1, w);
+      }
+      spec4[i] = pack2(v0, v1);
     }
   }
 }
 
 // ------------------------------------------------------------------ pass_kspace
 // One block per (rhs, p1, x tile of T).  Tile = T x-positions * B (y2) * NZ.
+// Loads are 16-byte pairs of x positions, all issued before the b-DFTs.
 struct PassKArgs {
   Tables tb;
   Consts c;
@@ -90,7 +185,13 @@ struct PassKArgs {
 
 template <int B, int NZ, int T, int NT>
 __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
-  constexpr int LW = B * T;   // lines of the z-FFT
+  static_assert(T % 2 == 0, "x tile must hold pairs");
+  constexpr int LW = B * T;                   // lines of the z-FFT
+  constexpr int TP = T / 2;                   // pairs per z row
+  constexpr int NTASK = NZ * TP;              // (z, pair) tasks
+  constexpr int L = (NTASK + NT - 1) / NT;
+  // the tile alone fills the LDS budget (two blocks per CU at 80 KiB), so the
+  // small per-axis tables are read through the L1 cache instead
   __shared__ float2 lds[NZ * LW];
   const Consts& c = args.c;
   const Tables& tb = args.tb;
@@ -104,47 +205,88 @@ __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
   const long long rowstride = (long long)c.a * c.nx;     // between consecutive y2
   float2* __restrict__ spec = args.spec + (long long)r * args.rhs_stride + (long long)p1 * c.nx + x0;
   const int flip = flip_of(args.jj, args.parity, r);
+  const int tvalid = c.nx - x0 < T ? c.nx - x0 : T;      // even: nx and x0 are even
 
-  const int tvalid = c.nx - x0 < T ? c.nx - x0 : T;
-  for (int task = tid; task < NZ * T; task += NT) {
-    const int t = task % T, z = task / T;
-    float2 v[B];
-    const float2* src = spec + (long long)z * B * rowstride + t;
+  float4 reg[L][B];
 #pragma unroll
-    for (int y2 = 0; y2 < B; ++y2) v[y2] = t < tvalid ? src[y2 * rowstride] : make_float2(0.f, 0.f);
-    Dft<B, false>::run(v);
+  for (int k = 0; k < L; ++k) {
+    const int task = tid + k * NT;
+    if (task < NTASK) {
+      const int tp = task % TP, z = task / TP;
+      const float2* src = spec + (long long)z * B * rowstride + 2 * tp;
 #pragma unroll
-    for (int k2 = 0; k2 < B; ++k2) lds[z * LW + k2 * T + t] = v[k2];
+      for (int y2 = 0; y2 < B; ++y2)
+        reg[k][y2] = (2 * tp < tvalid) ? *reinterpret_cast<const float4*>(src + y2 * rowstride)
+                                       : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < L; ++k) {
+    const int task = tid + k * NT;
+    if (task < NTASK) {
+      const int tp = task % TP, z = task / TP;
+      float2 u0[B], u1[B];
+#pragma unroll
+      for (int y2 = 0; y2 < B; ++y2) {
+        u0[y2] = lo2(reg[k][y2]);
+        u1[y2] = hi2(reg[k][y2]);
+      }
+      Dft<B, false>::run(u0);
+      Dft<B, false>::run(u1);
+#pragma unroll
+      for (int k2 = 0; k2 < B; ++k2) {
+        lds[z * LW + k2 * T + 2 * tp] = u0[k2];
+        lds[z * LW + k2 * T + 2 * tp + 1] = u1[k2];
+      }
+    }
   }
   __syncthreads();
   auto zl = [](int l, int e) { return e * LW + l; };
   line_fft<NZ, false, LW, NT>(lds, zl, tb.tw_z, tid);
   {
-    const float* gx = tb.green_x + flip * c.nx + x0;
-    const float* gy = tb.green_y + flip * c.ny + p1;
+    // (1/N) / (d - i eps) = (d + i eps) / (N (d^2 + eps^2)),  d = 4 pi^2 |k_s|^2 - k_m^2
     const float* gz = tb.green_z + flip * c.nz;
-    for (int i = tid; i < NZ * LW; i += NT) {
-      const int pz = i / LW, l = i - pz * LW;
+    if constexpr (NT % LW == 0) {
+      const int l = tid % LW;
       const int k2 = l / T, t = l - k2 * T;
-      const float d = gx[t < tvalid ? t : 0] + gy[k2 * c.a] + gz[pz] - c.km2;
-      // (1/N) / (d - i eps) = (d + i eps) / (N (d^2 + eps^2))
-      const float s = c.inv_n / fmaf(d, d, c.eps * c.eps);
-      const float2 g = make_float2(d * s, c.eps * s);
-      lds[i] = cmul(lds[i], g);
+      const float gxy = tb.green_x[flip * c.nx + x0 + (t < tvalid ? t : 0)] + tb.green_y[flip * c.ny + k2 * c.a + p1] -
+                        c.km2;
+      for (int pz = tid / LW; pz < NZ; pz += NT / LW) {
+        const float d = gxy + gz[pz];
+        const float s = c.inv_n / fmaf(d, d, c.eps * c.eps);
+        lds[pz * LW + l] = cmul(lds[pz * LW + l], make_float2(d * s, c.eps * s));
+      }
+    } else {
+      for (int i = tid; i < NZ * LW; i += NT) {
+        const int pz = i / LW, l = i - pz * LW;
+        const int k2 = l / T, t = l - k2 * T;
+        const float d = tb.green_x[flip * c.nx + x0 + (t < tvalid ? t : 0)] + tb.green_y[flip * c.ny + k2 * c.a + p1] +
+                        gz[pz] - c.km2;
+        const float s = c.inv_n / fmaf(d, d, c.eps * c.eps);
+        lds[i] = cmul(lds[i], make_float2(d * s, c.eps * s));
+      }
     }
   }
   __syncthreads();
   line_fft<NZ, true, LW, NT>(lds, zl, tb.tw_z, tid);
-  for (int task = tid; task < NZ * T; task += NT) {
-    const int t = task % T, z = task / T;
-    if (t >= tvalid) continue;
-    float2 v[B];
 #pragma unroll
-    for (int k2 = 0; k2 < B; ++k2) v[k2] = lds[z * LW + k2 * T + t];
-    Dft<B, true>::run(v);
-    float2* dst = spec + (long long)z * B * rowstride + t;
+  for (int k = 0; k < L; ++k) {
+    const int task = tid + k * NT;
+    if (task < NTASK) {
+      const int tp = task % TP, z = task / TP;
+      if (2 * tp >= tvalid) continue;
+      float2 u0[B], u1[B];
 #pragma unroll
-    for (int y2 = 0; y2 < B; ++y2) dst[y2 * rowstride] = v[y2];
+      for (int k2 = 0; k2 < B; ++k2) {
+        u0[k2] = lds[z * LW + k2 * T + 2 * tp];
+        u1[k2] = lds[z * LW + k2 * T + 2 * tp + 1];
+      }
+      Dft<B, true>::run(u0);
+      Dft<B, true>::run(u1);
+      float2* dst = spec + (long long)z * B * rowstride + 2 * tp;
+#pragma unroll
+      for (int y2 = 0; y2 < B; ++y2) *reinterpret_cast<float4*>(dst + y2 * rowstride) = pack2(u0[y2], u1[y2]);
+    }
   }
 }
 

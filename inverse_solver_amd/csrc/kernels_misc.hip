@@ -31,11 +31,20 @@ __global__ void update_plain(StateArgs st, Tables tb, Consts c, int upd, int fwd
     const int z = (int)(yz / c.ny);
     const long long ro = (long long)rhs * st.rhs_stride;
     float2* spec = st.spec + ro;
+    float2* F = st.field + ro;
+    float2* f = st.fstate + ro;
     const int flip_inv = jj_inv ? flip_of(jj_inv, st.parity, rhs) : 0;
     const int flip_fwd = jj_fwd ? flip_of(jj_fwd, st.parity, rhs) : 0;
-    const float2 q = (upd != UPD_NONE) ? spec[i] : make_float2(0.f, 0.f);
-    const float2 p = born_point(upd, fwd, q, i, x, y, z, tb, c, st.field + ro, st.fstate + ro, st.V, flip_inv,
-                                flip_fwd, nrm);
+    const float2 r = cmul(cmul(tb.ramp_x[x], tb.ramp_y[y]), tb.ramp_z[z]);
+    const float m = tb.mask_x[x] * tb.mask_y[y] * tb.mask_z[z];
+    const float2 zero = make_float2(0.f, 0.f);
+    const float2 q = (upd != UPD_NONE) ? spec[i] : zero;
+    const float2 xo = reads_f(upd, fwd) ? f[i] : zero;
+    const float2 Fo = reads_F(upd) ? F[i] : zero;
+    float2 Fn, xn;
+    const float2 p = born_step(upd, fwd, q, r, m, st.V[i], xo, Fo, c.inv_eps, flip_inv, flip_fwd, Fn, xn, nrm);
+    if (upd != UPD_NONE) F[i] = Fn;
+    if (fwd == FWD_GAMMA) f[i] = xn;
     if (fwd != FWD_NONE) spec[i] = p;
   }
   if (st.norms) flush_norm(nrm, st.norms + rhs);

@@ -10,22 +10,25 @@ namespace cbs {
 
 constexpr int kNT = 256;
 
+// (NX, A, NT): x length, rows of the y split, threads per block
 #define PR_LIST(X) \
-  X(24, 24) X(24, 12) X(24, 6) X(25, 25) X(32, 32) X(32, 16) X(40, 40) X(40, 20) \
-  X(96, 96) X(96, 48) X(96, 32) \
-  X(128, 64) X(128, 32) \
-  X(160, 40) X(160, 32) X(160, 20) \
-  X(192, 48) X(192, 32) \
-  X(288, 32)
+  X(24, 24, 256) X(24, 12, 256) X(24, 6, 256) X(32, 32, 256) X(32, 16, 256) X(40, 40, 256) X(40, 20, 256) \
+  X(96, 96, 256) X(96, 96, 512) X(96, 48, 256) X(96, 48, 512) X(96, 32, 256) \
+  X(128, 64, 256) X(128, 64, 512) X(128, 32, 256) \
+  X(160, 40, 256) X(160, 40, 512) X(160, 32, 256) X(160, 32, 512) X(160, 20, 256) X(160, 80, 512) \
+  X(192, 48, 256) X(192, 48, 512) X(192, 32, 256) \
+  X(288, 32, 256) X(288, 32, 512)
 
+// (B, NZ, T, NT): b-part of the y split, z length, x tile, threads per block
 #define PK_LIST(X) \
-  X(1, 18, 8) X(1, 24, 8) X(1, 24, 16) X(2, 24, 8) X(4, 24, 8) X(1, 25, 8) X(1, 27, 8) X(1, 20, 8) \
-  X(1, 32, 16) X(2, 32, 16) X(1, 40, 16) X(2, 40, 16) X(1, 30, 8) X(1, 36, 8) \
-  X(1, 96, 32) X(1, 96, 16) X(2, 96, 16) X(3, 96, 16) \
-  X(2, 135, 32) X(2, 135, 16) X(4, 135, 8) \
-  X(2, 128, 32) X(4, 128, 16) X(6, 128, 8) X(4, 128, 8) \
-  X(4, 160, 16) X(5, 160, 8) X(8, 160, 8) X(4, 160, 8) \
-  X(9, 160, 4)
+  X(1, 18, 8, 256) X(1, 24, 8, 256) X(1, 24, 16, 256) X(2, 24, 8, 256) X(4, 24, 8, 256) X(1, 20, 8, 256) \
+  X(1, 32, 16, 256) X(2, 32, 16, 256) X(1, 40, 16, 256) X(2, 40, 16, 256) X(1, 30, 8, 256) X(1, 36, 8, 256) \
+  X(1, 96, 32, 256) X(1, 96, 32, 512) X(1, 96, 16, 256) X(2, 96, 16, 256) X(2, 96, 32, 512) X(3, 96, 16, 256) \
+  X(2, 135, 32, 256) X(2, 135, 32, 512) X(2, 135, 16, 256) X(4, 135, 8, 256) \
+  X(2, 128, 32, 256) X(4, 128, 16, 256) X(6, 128, 8, 256) X(4, 128, 8, 256) \
+  X(4, 160, 16, 256) X(4, 160, 16, 512) X(5, 160, 8, 256) X(8, 160, 8, 256) X(4, 160, 8, 256) \
+  X(2, 160, 32, 512) X(2, 160, 32, 256) \
+  X(9, 160, 4, 256)
 
 #define RW_LIST(X) \
   X(16) X(18) X(20) X(24) X(25) X(27) X(30) X(32) X(36) X(40) X(48) X(64) X(96) X(128) X(135) X(160) X(192) \
@@ -38,8 +41,8 @@ constexpr int kNT = 256;
 constexpr int rows_for(int n) { return n <= 64 ? 16 : (n <= 256 ? 8 : (n <= 576 ? 4 : 2)); }
 constexpr int strided_t(int n) { return n <= 512 ? 16 : 8; }
 
-#define PR_ENTRY(NX, A) {&pass_real<NX, A, kNT>, NX, A, kNT, (int)(A * row_pitch(NX) * sizeof(float2))},
-#define PK_ENTRY(B, NZ, T) {&pass_kspace<B, NZ, T, kNT>, B, NZ, T, kNT, (int)(NZ * B * T * sizeof(float2))},
+#define PR_ENTRY(NX, A, NT) {&pass_real<NX, A, NT>, NX, A, NT, (int)(A * row_pitch(NX) * sizeof(float2))},
+#define PK_ENTRY(B, NZ, T, NT) {&pass_kspace<B, NZ, T, NT>, B, NZ, T, NT, (int)(NZ * B * T * sizeof(float2))},
 #define RW_ENTRY(N) {&fft_rows<N, rows_for(N), kNT>, N, rows_for(N), kNT},
 #define ST_ENTRY(N) {&fft_strided<N, strided_t(N), kNT>, N, strided_t(N), kNT},
 

@@ -182,22 +182,32 @@ struct Solver {
       const int npr = list_pass_real(&prs);
       const PassKInfo* pks = nullptr;
       const int npk = list_pass_kspace(&pks);
-      int best_a = -1;
+      // default preference: the largest a whose LDS tile keeps two blocks per CU
+      // (<= 80 KiB), then the widest x tile, then 256 threads; cbs_autotune
+      // replaces this choice by measurement.
+      // (measured on MI355X at 160^3 x 49 rhs: 512-thread blocks beat 256 for
+      // tiles of 40 KiB and more)
+      auto score_r = [&](const PassRInfo& r) {
+        return (r.lds <= 81920 ? 1000000 : 0) + r.a * 10 + ((r.nt == 512) == (r.lds >= 40960) ? 1 : 0);
+      };
+      auto score_k = [&](const PassKInfo& q) {
+        return (q.lds <= 81920 ? 1000 : 0) + (q.t >= 16 ? 100 : 0) - q.t + ((q.nt == 512) == (q.lds >= 40960) ? 1 : 0);
+      };
+      long best = -1;
       for (int i = 0; i < npr; ++i) {
         const PassRInfo& r = prs[i];
-        if (r.nx != nx || ny % r.a != 0) continue;
-        if (opt.split_a > 0 && r.a != opt.split_a) continue;
+        if (!fused_ok_r(r)) continue;
         const int b = ny / r.a;
         const PassKInfo* k = nullptr;
         for (int j = 0; j < npk; ++j) {
           const PassKInfo& q = pks[j];
-          if (q.b != b || q.nz != nz) continue;
-          if (opt.tile_t > 0 && q.t != opt.tile_t) continue;
-          if (!k || q.t > k->t) k = &q;
+          if (!fused_ok_k(q, b)) continue;
+          if (!k || score_k(q) > score_k(*k)) k = &q;
         }
         if (!k) continue;
-        if (r.a > best_a) {
-          best_a = r.a;
+        const long sc = (long)score_r(r) * 10000 + score_k(*k);
+        if (sc > best) {
+          best = sc;
           p.pr = &r;
           p.pk = k;
         }
@@ -209,8 +219,9 @@ struct Solver {
         plan = p;
         return;
       }
-      if (opt.split_a > 0 || opt.tile_t > 0)
-        throw CbsError(CBS_ERR_UNSUPPORTED, "no fused kernel plan for the forced split/tile on grid " + shape_str());
+      if (opt.split_a > 0 || opt.tile_t > 0 || opt.threads_real > 0 || opt.threads_kspace > 0)
+        throw CbsError(CBS_ERR_UNSUPPORTED,
+                       "no fused kernel plan for the forced split/tile/threads on grid " + shape_str());
       p.form = CBS_FORM_PLAIN;   // grids without a fused plan run the per-axis form
     }
     p.rw = find_fft_rows(nx);
@@ -223,6 +234,18 @@ struct Solver {
     p.b = 1;
     p.t = p.sy->t;
     plan = p;
+  }
+  bool fused_ok_r(const PassRInfo& r) const {
+    if (r.nx != nx || ny % r.a != 0 || nx % 2 != 0) return false;   // pairs of x per 16-byte access
+    if (opt.split_a > 0 && r.a != opt.split_a) return false;
+    if (opt.threads_real > 0 && r.nt != opt.threads_real) return false;
+    return true;
+  }
+  bool fused_ok_k(const PassKInfo& q, int b) const {
+    if (q.b != b || q.nz != nz) return false;
+    if (opt.tile_t > 0 && q.t != opt.tile_t) return false;
+    if (opt.threads_kspace > 0 && q.nt != opt.threads_kspace) return false;
+    return true;
   }
   std::string shape_str() const {
     return std::to_string(nx) + "x" + std::to_string(ny) + "x" + std::to_string(nz);
@@ -546,6 +569,99 @@ struct Solver {
     if (prm.acyclic) flip_state = (int)((flip_state + (long long)n_rhs * M) & 1);
   }
 
+  // Times every fused plan for the grid and keeps the fastest.
+  double autotune(uint32_t n_rhs, uint32_t iters) {
+    require(have_potential, CBS_ERR_STATE, "cbs_set_potential must be called before cbs_autotune");
+    require(n_rhs >= 1 && iters >= 3, CBS_ERR_ARG, "autotune needs n_rhs >= 1 and iterations >= 3");
+    const PassRInfo* prs = nullptr;
+    const int npr = list_pass_real(&prs);
+    const PassKInfo* pks = nullptr;
+    const int npk = list_pass_kspace(&pks);
+    const cbs_options saved = opt;
+    opt.split_a = opt.tile_t = opt.threads_real = opt.threads_kspace = 0;
+    opt.form = CBS_FORM_FUSED;
+    std::vector<std::pair<const PassRInfo*, const PassKInfo*>> cands;
+    for (int i = 0; i < npr; ++i) {
+      if (!fused_ok_r(prs[i])) continue;
+      for (int j = 0; j < npk; ++j)
+        if (fused_ok_k(pks[j], ny / prs[i].a)) cands.emplace_back(&prs[i], &pks[j]);
+    }
+    if (cands.empty()) {
+      opt = saved;
+      throw CbsError(CBS_ERR_UNSUPPORTED, "no fused kernel plan to tune for grid " + shape_str());
+    }
+    const int nb = batch_width(n_rhs);
+    ensure_workspace(nb, false);
+    DevBuf<float2> src, dst;
+    src.ensure((size_t)nb * nri);
+    dst.ensure((size_t)nb * nri);
+    {
+      std::vector<float2> h((size_t)nb * nri);
+      for (size_t i = 0; i < h.size(); ++i) h[i] = make_float2(std::cos(0.37 * (double)i), std::sin(0.21 * (double)i));
+      HIP_CHECK(hipMemcpy(src.p, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice));
+    }
+    const uint32_t saved_iters = prm.Bornmax;
+    const int saved_flip = flip_state;
+    const int saved_time = opt.time_kernels, saved_norm = opt.track_norms;
+    opt.time_kernels = 0;
+    opt.track_norms = 0;
+    prm.Bornmax = iters;
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    double best = 1e30;
+    Plan best_plan;
+    try {
+      for (auto& cd : cands) {
+        Plan p;
+        p.form = CBS_FORM_FUSED;
+        p.pr = cd.first;
+        p.pk = cd.second;
+        p.a = cd.first->a;
+        p.b = ny / p.a;
+        p.t = cd.second->t;
+        plan = p;
+        build_tables();
+        born_batch(src.p, dst.p, nb, 0);      // warm-up
+        HIP_CHECK(hipEventRecord(e0, stream));
+        born_batch(src.p, dst.p, nb, 0);
+        HIP_CHECK(hipEventRecord(e1, stream));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+          best = ms;
+          best_plan = p;
+        }
+      }
+    } catch (...) {
+      prm.Bornmax = saved_iters;
+      flip_state = saved_flip;
+      opt = saved;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      make_plan();
+      build_tables();
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    prm.Bornmax = saved_iters;
+    flip_state = saved_flip;
+    opt = saved;
+    opt.form = CBS_FORM_FUSED;
+    opt.split_a = best_plan.a;
+    opt.tile_t = best_plan.t;
+    opt.threads_real = best_plan.pr->nt;
+    opt.threads_kspace = best_plan.pk->nt;
+    opt.time_kernels = saved_time;
+    opt.track_norms = saved_norm;
+    plan = best_plan;
+    build_tables();
+    HIP_CHECK(hipStreamSynchronize(stream));
+    return best;
+  }
+
   void set_potential(const cbs_potential_params* p, const float2* v, int mem) {
     require(p != nullptr && v != nullptr, CBS_ERR_ARG, "null potential argument");
     require(mem == CBS_MEM_HOST || mem == CBS_MEM_DEVICE, CBS_ERR_ARG, "mem must be CBS_MEM_HOST or CBS_MEM_DEVICE");
@@ -753,6 +869,15 @@ int cbs_get_plan(cbs_handle h, int32_t* form, int32_t* a, int32_t* b, int32_t* t
     if (a) *a = s->plan.a;
     if (b) *b = s->plan.b;
     if (tile_t) *tile_t = s->plan.t;
+  });
+}
+
+int cbs_autotune(cbs_handle h, uint32_t n_rhs, uint32_t iterations, double* out_ms) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    DeviceGuard dg(s->device);
+    const double ms = s->autotune(n_rhs, iterations);
+    if (out_ms) *out_ms = ms;
   });
 }
 

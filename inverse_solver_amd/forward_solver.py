@@ -40,7 +40,7 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
     """Scalar CBS forward solver on one MI355X.
 
     ``options`` maps to cbs_options (form, split_a, tile_t, max_batch,
-    track_norms, use_graph, time_kernels).
+    track_norms, use_graph, time_kernels, threads_real, threads_kspace).
     """
 
     get_default_parameters = staticmethod(get_default_parameters)
@@ -110,11 +110,19 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
             setattr(o, k, int(v))
         check(lib.cbs_set_options(self._h, C.byref(o)))
 
+    def autotune(self, n_rhs, iterations=6):
+        """Measure every fused kernel plan on the current potential and keep the fastest."""
+        ms = C.c_double()
+        check(lib.cbs_autotune(self._h, int(n_rhs), int(iterations), C.byref(ms)))
+        return ms.value
+
     def plan(self):
         f, a, b, t = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
         check(lib.cbs_get_plan(self._h, C.byref(f), C.byref(a), C.byref(b), C.byref(t)))
+        o = _lib.Options()
+        check(lib.cbs_get_options(self._h, C.byref(o)))
         return {'form': 'plain' if f.value == _lib.CBS_FORM_PLAIN else 'fused', 'a': a.value, 'b': b.value,
-                'tile_t': t.value}
+                'tile_t': t.value, 'threads_real': o.threads_real, 'threads_kspace': o.threads_kspace}
 
     # ---------------------------------------------------------------- potential (:180-396)
     def set_RI(self, RI):
