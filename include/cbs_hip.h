@@ -53,6 +53,7 @@ typedef struct {
   uint32_t field_size[3];      /* xy size of the 2D fields; [2] ignored */
   uint32_t conv_size[3];       /* xy convolution grid (field + RI); [2] ignored */
   int32_t device;              /* HIP device ordinal */
+  int32_t RI_center[2];        /* xy offset of the RI grid in the field (RI_center, :40-41) */
 } cbs_geometry;
 
 /* Scalars set with the potential.

@@ -31,7 +31,7 @@ class CbsError(RuntimeError):
 
 class Geometry(C.Structure):
     _fields_ = [('simulation_size', C.c_uint32 * 3), ('RI_size', C.c_uint32 * 3), ('field_size', C.c_uint32 * 3),
-                ('conv_size', C.c_uint32 * 3), ('device', C.c_int32)]
+                ('conv_size', C.c_uint32 * 3), ('device', C.c_int32), ('RI_center', C.c_int32 * 2)]
 
 
 class PotentialParams(C.Structure):

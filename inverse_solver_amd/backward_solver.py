@@ -1,0 +1,124 @@
+"""BACKWARD_SOLVER_MULTI, MI355X backend: gradient of the transmitted-field
+misfit through a forward and an adjoint CBS solve, and the TV-regularised FISTA
+reconstruction loop (Codes/BACKWARD_SOLVER_MULTI.m).
+
+The forward potential V and the adjoint potential V' (from conj(flip_z(RI)),
+:48) live on the device in two solver handles that share one set of 2D
+projection kernels, so a gradient needs no potential re-upload beyond the two
+set_RI calls the reference also makes.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from .forward_solver import FORWARD_SOLVER_CONVERGENT_BORN, get_default_parameters as fwd_defaults
+from .params import update_struct
+from .tv import tv_fista_inner
+
+
+def get_default_parameters(init=None):
+    """Codes/BACKWARD_SOLVER_MULTI.m:12-29 (on top of BACKWARD_SOLVER's optical defaults)."""
+    p = fwd_defaults()
+    p.update(forward_solver_parameters=fwd_defaults(dict(return_transmission=True, return_reflection=True,
+                                                         return_3D=True, vector_simulation=False)),
+             step=0.01, tv_param=0.001, use_non_negativity=False, nmin=-np.inf, nmax=np.inf, kappamax=0,
+             inner_itt=100, itter_max=100, num_scan_per_iteration=0, verbose=True)
+    return update_struct(p, init) if init is not None else p
+
+
+class BACKWARD_SOLVER_MULTI:
+    get_default_parameters = staticmethod(get_default_parameters)
+
+    def __init__(self, params, device=0, options=None):
+        self.parameters = dict(params)
+        fp = dict(self.parameters['forward_solver_parameters'])
+        if not (fp.get('return_transmission') and fp.get('return_reflection') and fp.get('return_3D')):
+            raise ValueError('need to set all return parameter to true in the forward solver')   # :57-66
+        fp['return_reflection'] = False          # the gradient reads only the transmitted and 3D fields
+        self.forward = FORWARD_SOLVER_CONVERGENT_BORN(fp, device=device, options=options)
+        self.adjoint = FORWARD_SOLVER_CONVERGENT_BORN(fp, device=device, options=options)
+        self.RI_inter = None
+        self.err_list = []
+
+    def close(self):
+        self.forward.close()
+        self.adjoint.close()
+
+    def _share_io(self):
+        self.forward._ensure_io()
+        if not self.adjoint._io_ready:
+            self.adjoint._io_arrays = self.forward._io_arrays
+            from ._lib import check, lib, CBS_MEM_HOST
+            from .forward_solver import _fptr
+            check(lib.cbs_set_io_kernels(self.adjoint._h, *[_fptr(a) for a in self.adjoint._io_arrays],
+                                         CBS_MEM_HOST))
+            self.adjoint._io_ready = True
+
+    def get_gradiant_static(self, RI, input_field, output_field):
+        """Codes/BACKWARD_SOLVER_MULTI.m:30-54: forward solve, residual, adjoint solve on
+        conj(flip_z(RI)) with conj(residual), gradient -i mean(conj(u) w) (FP64 reduction)."""
+        self._share_io()
+        RI = np.asarray(RI)
+        self.forward.set_RI(RI)                                                   # :32
+        trans, _, u3d = self.forward.solve(input_field)                           # :34
+        diff = trans.astype(np.complex128) - np.asarray(output_field)            # :36
+        err = float(np.sum(np.abs(diff) ** 2))                                    # :37
+        self.adjoint.set_RI(np.conj(np.flip(RI, axis=2)))                         # :48
+        _, _, w3d = self.adjoint.solve(np.conj(diff))                             # :49
+        w3d = np.flip(np.conj(w3d), axis=2)                                       # :50
+        grad = -(1j * np.mean(np.conj(u3d.astype(np.complex128)) * w3d, axis=(3, 4)))   # :53
+        return grad, err
+
+    def get_gradiant(self, RI, input_field, output_field, rng=None):
+        """:80-91 (random angle subsets when num_scan_per_iteration > 0)."""
+        n = self.parameters['num_scan_per_iteration']
+        A = np.asarray(input_field).shape[3]
+        if n == 0:
+            scan = np.arange(A)
+        else:
+            rng = rng or np.random.default_rng()
+            scan = np.unique(np.concatenate([[0], rng.permutation(A)[:n]]))[:n]
+        return self.get_gradiant_static(RI, np.asarray(input_field)[:, :, :, scan],
+                                        np.asarray(output_field)[:, :, :, scan])
+
+    def solve(self, input_field, output_field, init_RI=None, init_solver=None):
+        """:92-196.  The reference starts from init_solver.solve(input, output) (a Rytov
+        reconstruction); pass init_RI (or an object with .solve) instead."""
+        p = self.parameters
+        if init_RI is None:
+            if init_solver is None:
+                raise ValueError('give init_RI or an init_solver with .solve(input_field, output_field)')
+            init_RI = init_solver.solve(input_field, output_field)
+        RI = np.asarray(init_RI).astype(np.complex128)
+        n_m = p['RI_bg']
+        alpha = 1.0 / p['step']
+        t_np = 1.0
+        u_n = RI ** 2 / n_m ** 2 - 1
+        x_n = u_n.copy()
+        if RI.shape[2] % 2 == 0:
+            raise ValueError('need to be odd size')                                # :117-119
+        Vmin = (p['nmin'] ** 2 - p['kappamax'] ** 2) / n_m ** 2 - 1
+        Vmax = p['nmax'] ** 2 / n_m ** 2 - 1
+        Vimag_max = 2 * p['kappamax'] * p['nmax'] / n_m ** 2
+        self.err_list = []
+        self.timings = []
+        for ii in range(int(p['itter_max'])):
+            t0 = time.perf_counter()
+            grad, err = self.get_gradiant(RI, input_field, output_field)
+            t1 = time.perf_counter()
+            self.err_list.append(err)
+            t_n = t_np
+            s_n = tv_fista_inner(u_n - (1 / alpha) * grad, p['tv_param'] / alpha, Vmin, Vmax, Vimag_max,
+                                 p['use_non_negativity'], False, p['inner_itt'])
+            t_np = (1 + np.sqrt(1 + 4 * t_n ** 2)) / 2
+            u_n = s_n + (t_n - 1) / t_np * (s_n - x_n)
+            x_n = s_n
+            RI = n_m * np.sqrt(u_n.astype(np.complex128) + 1)
+            self.RI_inter = RI
+            self.timings.append((t1 - t0, time.perf_counter() - t1))
+            if p.get('verbose'):
+                print('itter : %d  err %.6e  gradient %.2f s  TV %.2f s' % (ii + 1, err, t1 - t0,
+                                                                             time.perf_counter() - t1))
+        return RI

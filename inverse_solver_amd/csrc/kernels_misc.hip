@@ -1,40 +1,43 @@
 // Pointwise kernels: plain-form Green's multiply and update, source seeding and
-// the final Field[ROI] + u_in assembly.
+// the final Field[ROI] + u_in assembly.  Grid-stride loops (the largest grids
+// exceed 2^32 work items when launched one thread per element).
 #include "kernels_misc.hpp"
 
 namespace cbs {
 
+#define GRID_STRIDE(i, total) \
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (total); i += (long long)gridDim.x * blockDim.x)
+
 // Plain form: spectrum in natural [z][y][x] layout (each axis in LineFft
 // position order).  Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:609-613 (Greenp .* fftn)
-// with the 1/N of the following ifftn folded in, as Codes/CUDA_CODE/execution.cu:106 does.
+// with the 1/N of the following ifftn folded in.
 __global__ void green_plain(float2* __restrict__ spec, Tables tb, Consts c, int flip) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= c.nvox) return;
-  const int x = (int)(i % c.nx);
-  const long long yz = i / c.nx;
-  const int y = (int)(yz % c.ny);
-  const int z = (int)(yz / c.ny);
-  const float d = tb.green_x[flip * c.nx + x] + tb.green_y[flip * c.ny + y] + tb.green_z[flip * c.nz + z] - c.km2;
-  const float s = c.inv_n / fmaf(d, d, c.eps * c.eps);
-  spec[i] = cmul(spec[i], make_float2(d * s, c.eps * s));
+  GRID_STRIDE(i, c.nvox) {
+    const int x = (int)(i % c.nx);
+    const long long yz = i / c.nx;
+    const int y = (int)(yz % c.ny);
+    const int z = (int)(yz / c.ny);
+    const float d = tb.green_x[flip * c.nx + x] + tb.green_y[flip * c.ny + y] + tb.green_z[flip * c.nz + z] - c.km2;
+    const float s = c.inv_n / fmaf(d, d, c.eps * c.eps);
+    spec[i] = cmul(spec[i], make_float2(d * s, c.eps * s));
+  }
 }
 
 // Plain form: the pointwise part of one iteration on natural-layout arrays
 // (Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:595-642).
 __global__ void update_plain(StateArgs st, Tables tb, Consts c, int upd, int fwd, int jj_inv, int jj_fwd, int rhs) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   double nrm = 0.0;
-  if (i < c.nvox) {
+  const long long ro = (long long)rhs * st.rhs_stride;
+  float2* spec = st.spec + ro;
+  float2* F = st.field + ro;
+  float2* f = st.fstate + ro;
+  const int flip_inv = jj_inv ? flip_of(jj_inv, st.parity, rhs) : 0;
+  const int flip_fwd = jj_fwd ? flip_of(jj_fwd, st.parity, rhs) : 0;
+  GRID_STRIDE(i, c.nvox) {
     const int x = (int)(i % c.nx);
     const long long yz = i / c.nx;
     const int y = (int)(yz % c.ny);
     const int z = (int)(yz / c.ny);
-    const long long ro = (long long)rhs * st.rhs_stride;
-    float2* spec = st.spec + ro;
-    float2* F = st.field + ro;
-    float2* f = st.fstate + ro;
-    const int flip_inv = jj_inv ? flip_of(jj_inv, st.parity, rhs) : 0;
-    const int flip_fwd = jj_fwd ? flip_of(jj_fwd, st.parity, rhs) : 0;
     const float2 r = cmul(cmul(tb.ramp_x[x], tb.ramp_y[y]), tb.ramp_z[z]);
     const float m = tb.mask_x[x] * tb.mask_y[y] * tb.mask_z[z];
     const float2 zero = make_float2(0.f, 0.f);
@@ -51,47 +54,43 @@ __global__ void update_plain(StateArgs st, Tables tb, Consts c, int upd, int fwd
 }
 
 // p0 = (i/eps) * S / 2 with S = (V + i eps) .* u_in on the ROI, zero elsewhere
-// (Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:566,596).  One thread per sim voxel per rhs.
+// (Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:566,596).
 __global__ void seed_source(const float2* __restrict__ u_in, const float2* __restrict__ V, float2* __restrict__ fstate,
                             Consts c, RoiGeom rg, long long rhs_stride, int nrhs) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long total = c.nvox * nrhs;
-  if (i >= total) return;
-  const int r = (int)(i / c.nvox);
-  const long long g = i - (long long)r * c.nvox;
-  const int x = (int)(g % c.nx);
-  const long long yz = g / c.nx;
-  const int y = (int)(yz % c.ny);
-  const int z = (int)(yz / c.ny);
-  const int xr = x - rg.ox, yr = y - rg.oy, zr = z - rg.oz;
-  float2 p = make_float2(0.f, 0.f);
-  if (xr >= 0 && xr < rg.nx && yr >= 0 && yr < rg.ny && zr >= 0 && zr < rg.nz) {
-    const long long gi = ((long long)zr * rg.ny + yr) * rg.nx + xr;
-    const float2 u = u_in[(long long)r * rg.nvox + gi];
-    const float2 v = V[g];
-    const float2 vt = make_float2(v.x, v.y + c.eps);          // V + i eps = Vt
-    const float2 s = cmul(vt, u);
-    // (i / eps) * s / 2
-    const float h = 0.5f * c.inv_eps;
-    p = make_float2(-s.y * h, s.x * h);
+  GRID_STRIDE(i, c.nvox * nrhs) {
+    const int r = (int)(i / c.nvox);
+    const long long g = i - (long long)r * c.nvox;
+    const int x = (int)(g % c.nx);
+    const long long yz = g / c.nx;
+    const int y = (int)(yz % c.ny);
+    const int z = (int)(yz / c.ny);
+    const int xr = x - rg.ox, yr = y - rg.oy, zr = z - rg.oz;
+    float2 p = make_float2(0.f, 0.f);
+    if (xr >= 0 && xr < rg.nx && yr >= 0 && yr < rg.ny && zr >= 0 && zr < rg.nz) {
+      const long long gi = ((long long)zr * rg.ny + yr) * rg.nx + xr;
+      const float2 u = u_in[(long long)r * rg.nvox + gi];
+      const float2 v = V[g];
+      const float2 s = cmul(make_float2(v.x, v.y + c.eps), u);   // (V + i eps) u = Vt u
+      const float h = 0.5f * c.inv_eps;
+      p = make_float2(-s.y * h, s.x * h);                         // (i / eps) s / 2
+    }
+    fstate[(long long)r * rhs_stride + g] = p;
   }
-  fstate[(long long)r * rhs_stride + g] = p;
 }
 
 // u = Field[ROI] + u_in (Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:646-647).
 __global__ void finalize_field(const float2* __restrict__ field, const float2* __restrict__ u_in, float2* __restrict__ u_out,
                                Consts c, RoiGeom rg, long long rhs_stride, int nrhs) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long total = rg.nvox * nrhs;
-  if (i >= total) return;
-  const int r = (int)(i / rg.nvox);
-  const long long gi = i - (long long)r * rg.nvox;
-  const int xr = (int)(gi % rg.nx);
-  const long long yz = gi / rg.nx;
-  const int yr = (int)(yz % rg.ny);
-  const int zr = (int)(yz / rg.ny);
-  const long long g = ((long long)(zr + rg.oz) * c.ny + (yr + rg.oy)) * c.nx + (xr + rg.ox);
-  u_out[i] = cadd(field[(long long)r * rhs_stride + g], u_in[i]);
+  GRID_STRIDE(i, rg.nvox * nrhs) {
+    const int r = (int)(i / rg.nvox);
+    const long long gi = i - (long long)r * rg.nvox;
+    const int xr = (int)(gi % rg.nx);
+    const long long yz = gi / rg.nx;
+    const int yr = (int)(yz % rg.ny);
+    const int zr = (int)(yz / rg.ny);
+    const long long g = ((long long)(zr + rg.oz) * c.ny + (yr + rg.oy)) * c.nx + (xr + rg.ox);
+    u_out[i] = cadd(field[(long long)r * rhs_stride + g], u_in[i]);
+  }
 }
 
 }  // namespace cbs

@@ -144,6 +144,13 @@ struct Solver {
   std::vector<long long> graph_key;
   // I/O projection state
   IoState io;
+  int rc_x = 0, rc_y = 0;
+  IoGeom iog{};
+  DevBuf<float2> io_refocus_k, io_ktrans, io_kref, io_tw_cx, io_tw_cy;
+  DevBuf<int> io_kcx, io_kcy;
+  DevBuf<float2> io_buf3, io_spec2, io_out_t, io_out_r, io_uin, io_src, io_u3d, io_tr, io_rf;
+  const RowsInfo* io_rows = nullptr;
+  const StridedInfo* io_cols = nullptr;
 
   ~Solver() {
     int prev = 0;
@@ -417,7 +424,7 @@ struct Solver {
     const Tables tb = tabs.view();
     const long long ro = (long long)r * nvox;
     float2* sp = spec.p + ro;
-    const unsigned pw_grid = (unsigned)((nvox + 255) / 256);
+    const unsigned pw_grid = grid_for(nvox);
     auto update = [&](int upd, int fwd, int jj_inv, int jj_fwd) {
       StateArgs s = state_args(1);
       s.norms = (opt.track_norms && jj_inv > 0) ? norms.p + (size_t)(jj_inv - 1) * nb : nullptr;
@@ -478,7 +485,7 @@ struct Solver {
     uint64_t b = 3ull * nb * nvox * sizeof(float2);   // field, fstate, spec
     b += 2ull * nb * nri * sizeof(float2);             // staging for host I/O
     b += (uint64_t)nvox * sizeof(float2);              // V
-    b += io.footprint(nb);
+    b += io.footprint(nb, cx, cy, rz) + (io.ready ? (unsigned long long)nb * nri * 16ull : 0ull);
     return b;
   }
   int batch_width(uint32_t n_rhs) const {
@@ -505,7 +512,7 @@ struct Solver {
     }
     {
       const long long tot = (long long)nb * nvox;
-      timed(2, [&] { hipLaunchKernelGGL(seed_source, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream, d_uin,
+      timed(2, [&] { hipLaunchKernelGGL(seed_source, dim3(grid_for(tot)), dim3(256), 0, stream, d_uin,
                                         V.p, fstate.p, c, g, nvox, nb); });
     }
     if (M == 0) {
@@ -517,7 +524,7 @@ struct Solver {
     }
     {
       const long long tot = (long long)nb * nri;
-      timed(2, [&] { hipLaunchKernelGGL(finalize_field, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream,
+      timed(2, [&] { hipLaunchKernelGGL(finalize_field, dim3(grid_for(tot)), dim3(256), 0, stream,
                                         field.p, d_uin, d_uout, c, g, nvox, nb); });
     }
     HIP_CHECK(hipGetLastError());
@@ -562,6 +569,157 @@ struct Solver {
       if (mem == CBS_MEM_HOST) {
         HIP_CHECK(hipMemcpyAsync(u_out + (size_t)r0 * nri, uout.p, (size_t)nb * nri * sizeof(float2),
                                  hipMemcpyDeviceToHost, stream));
+      }
+    }
+    HIP_CHECK(hipStreamSynchronize(stream));
+    if (opt.time_kernels) resolve_times();
+    if (prm.acyclic) flip_state = (int)((flip_state + (long long)n_rhs * M) & 1);
+  }
+
+  static unsigned grid_for(long long n) {
+    const long long b = (n + 255) / 256;
+    return (unsigned)std::max<long long>(1, std::min<long long>(b, 1 << 20));
+  }
+
+  // ---- 2D input/output projections
+  void set_io_kernels(const float2* refocus, const float2* ktrans, const float2* kref, int mem) {
+    require(refocus != nullptr && ktrans != nullptr, CBS_ERR_ARG, "refocus and kernel_trans are required");
+    require(mem == CBS_MEM_HOST || mem == CBS_MEM_DEVICE, CBS_ERR_ARG, "mem must be CBS_MEM_HOST or CBS_MEM_DEVICE");
+    require(fx == rx && fy == ry, CBS_ERR_ARG,
+            "2D projections need the field xy size equal to the RI xy size (as the reference's crop_field2RI assumes)");
+    require(cx >= fx && cy >= fy, CBS_ERR_ARG, "conv_size must be at least field_size");
+    if (cx == fx) require(rc_x == 0 && rc_y == 0, CBS_ERR_ARG, "RI_center must be 0 with a cyclic xy boundary");
+    io_rows = find_fft_rows(cx);
+    io_cols = find_fft_strided(cy, 0);
+    require(io_rows && io_cols, CBS_ERR_UNSUPPORTED,
+            "no FFT kernels for the convolution grid " + std::to_string(cx) + "x" + std::to_string(cy));
+    const size_t n = (size_t)cx * cy * rz;
+    const hipMemcpyKind kind = mem == CBS_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+    io_refocus_k.ensure(n);
+    HIP_CHECK(hipMemcpy(io_refocus_k.p, refocus, n * sizeof(float2), kind));
+    io_ktrans.ensure(n);
+    HIP_CHECK(hipMemcpy(io_ktrans.p, ktrans, n * sizeof(float2), kind));
+    io.have_ref = kref != nullptr;
+    if (kref) {
+      io_kref.ensure(n);
+      HIP_CHECK(hipMemcpy(io_kref.p, kref, n * sizeof(float2), kind));
+    }
+    std::vector<int> kx(cx), ky(cy);
+    for (int p = 0; p < cx; ++p) kx[p] = (pos2freq(cx, p) + cx / 2) % cx;
+    for (int p = 0; p < cy; ++p) ky[p] = (pos2freq(cy, p) + cy / 2) % cy;
+    io_kcx.upload(kx);
+    io_kcy.upload(ky);
+    io_tw_cx.upload(twiddles(cx));
+    io_tw_cy.upload(twiddles(cy));
+    iog.fx = fx; iog.fy = fy; iog.cx = cx; iog.cy = cy; iog.rx = rx; iog.ry = ry; iog.rz = rz;
+    iog.pad_x = cx / 2 - fx / 2; iog.pad_y = cy / 2 - fy / 2;
+    iog.ri_x = iog.pad_x + rc_x; iog.ri_y = iog.pad_y + rc_y;
+    require(iog.ri_x >= 0 && iog.ri_y >= 0 && iog.ri_x + rx <= cx && iog.ri_y + ry <= cy, CBS_ERR_ARG,
+            "RI_center moves the RI grid outside the convolution grid");
+    io.ready = true;
+  }
+
+  // batched 2D FFT over nslices (cx x cy) planes
+  void fft2d(float2* buf, long long nslices, int inverse) {
+    const RowsInfo* rw = io_rows;
+    const StridedInfo* st = io_cols;
+    auto fx_ = [&] {
+      const long long nrows = nslices * cy;
+      const dim3 g((unsigned)((nrows + rw->rows - 1) / rw->rows));
+      timed(2, [&] { hipLaunchKernelGGL(rw->fn, g, dim3(rw->nt), 0, stream, buf, nrows, io_tw_cx.p, inverse); });
+    };
+    auto fy_ = [&] {
+      const int nt = (cx + st->t - 1) / st->t;
+      require((long long)nt * nslices < (1ll << 31), CBS_ERR_UNSUPPORTED, "2D FFT batch too large for one launch");
+      const dim3 g((unsigned)(nt * nslices));
+      timed(2, [&] { hipLaunchKernelGGL(st->fn, g, dim3(st->nt), 0, stream, buf, cx, (long long)cx,
+                                        (long long)cx * cy, (int)nslices, io_tw_cy.p, inverse); });
+    };
+    if (!inverse) { fx_(); fy_(); }
+    else { fy_(); fx_(); }
+    HIP_CHECK(hipGetLastError());
+  }
+
+  void solve_2d(const float2* src, uint32_t n_rhs, float2* trans, float2* refl, float2* f3d, int mem) {
+    require(have_potential, CBS_ERR_STATE, "cbs_set_potential must be called before solving");
+    require(io.ready, CBS_ERR_STATE, "cbs_set_io_kernels must be called before cbs_solve_2d");
+    require(n_rhs >= 1 && src != nullptr, CBS_ERR_ARG, "null source or n_rhs = 0");
+    require(mem == CBS_MEM_HOST || mem == CBS_MEM_DEVICE, CBS_ERR_ARG, "mem must be CBS_MEM_HOST or CBS_MEM_DEVICE");
+    require(refl == nullptr || io.have_ref, CBS_ERR_STATE, "reflection requested but no kernel_ref was uploaded");
+    const int nbw = batch_width(n_rhs);
+    ensure_workspace(nbw, false);
+    const long long n2 = (long long)cx * cy, nf = (long long)fx * fy;
+    io_buf3.ensure((size_t)nbw * n2 * rz);
+    io_spec2.ensure((size_t)nbw * n2);
+    if (trans) io_out_t.ensure((size_t)nbw * n2);
+    if (refl) io_out_r.ensure((size_t)nbw * n2);
+    io_uin.ensure((size_t)nbw * nri);
+    if (mem == CBS_MEM_HOST || !f3d) io_u3d.ensure((size_t)nbw * nri);
+    if (mem == CBS_MEM_HOST) {
+      io_src.ensure((size_t)nbw * nf);
+      if (trans) io_tr.ensure((size_t)nbw * nf);
+      if (refl) io_rf.ensure((size_t)nbw * nf);
+    }
+    const int M = (int)prm.Bornmax;
+    if (opt.track_norms) {
+      last_norms.assign((size_t)n_rhs * M, 0.0);
+      last_norm_rhs = (int)n_rhs;
+      last_norm_iter = M;
+    }
+    ev_used.clear();
+    const float sc = (float)(1.0 / (double)n2);
+    const Consts c = consts(1);
+    const RoiGeom rg = roi();
+    for (uint32_t r0 = 0; r0 < n_rhs; r0 += nbw) {
+      const int nb = (int)std::min<uint32_t>(nbw, n_rhs - r0);
+      const float2* s_d = src + (size_t)r0 * nf;
+      if (mem == CBS_MEM_HOST) {
+        HIP_CHECK(hipMemcpyAsync(io_src.p, s_d, (size_t)nb * nf * sizeof(float2), hipMemcpyHostToDevice, stream));
+        s_d = io_src.p;
+      }
+      // incident field: pad -> fft2 -> x refocus(z) -> ifft2 -> crop  (:543-554)
+      timed(2, [&] { hipLaunchKernelGGL(io_pad_in, dim3(grid_for(nb * n2)), dim3(256), 0, stream, s_d, io_spec2.p, iog, nb); });
+      fft2d(io_spec2.p, nb, 0);
+      timed(2, [&] { hipLaunchKernelGGL(io_refocus, dim3(grid_for(nb * n2 * rz)), dim3(256), 0, stream, io_spec2.p,
+                                        io_refocus_k.p, io_buf3.p, io_kcx.p, io_kcy.p, iog, nb, sc); });
+      fft2d(io_buf3.p, (long long)nb * rz, 1);
+      timed(2, [&] { hipLaunchKernelGGL(io_crop_in, dim3(grid_for(nb * nri)), dim3(256), 0, stream, io_buf3.p,
+                                        io_uin.p, iog, nb); });
+      float2* u3 = (mem == CBS_MEM_DEVICE && f3d) ? f3d + (size_t)r0 * nri : io_u3d.p;
+      born_batch(io_uin.p, u3, nb, (int)r0);
+      if (trans || refl) {
+        // emitter -> fft2 per z -> sum_z E K (FP64) -> ifft2 -> crop  (:444-511)
+        timed(2, [&] { hipLaunchKernelGGL(io_emitter, dim3(grid_for(nb * n2 * rz)), dim3(256), 0, stream, field.p,
+                                          io_uin.p, V.p, io_buf3.p, iog, c, rg.ox, rg.oy, rg.oz, nvox, nb,
+                                          (float)prm.dV); });
+        fft2d(io_buf3.p, (long long)nb * rz, 0);
+        timed(2, [&] { hipLaunchKernelGGL(io_zsum, dim3(grid_for(nb * n2)), dim3(256), 0, stream, io_buf3.p,
+                                          io_ktrans.p, io.have_ref ? io_kref.p : nullptr, trans ? io_out_t.p : nullptr,
+                                          refl ? io_out_r.p : nullptr, io_kcx.p, io_kcy.p, iog, nb); });
+        if (trans) {
+          fft2d(io_out_t.p, nb, 1);
+          float2* dst = mem == CBS_MEM_HOST ? io_tr.p : trans + (size_t)r0 * nf;
+          timed(2, [&] { hipLaunchKernelGGL(io_crop_out, dim3(grid_for(nb * nf)), dim3(256), 0, stream, io_out_t.p,
+                                            dst, iog, nb, sc); });
+        }
+        if (refl) {
+          fft2d(io_out_r.p, nb, 1);
+          float2* dst = mem == CBS_MEM_HOST ? io_rf.p : refl + (size_t)r0 * nf;
+          timed(2, [&] { hipLaunchKernelGGL(io_crop_out, dim3(grid_for(nb * nf)), dim3(256), 0, stream, io_out_r.p,
+                                            dst, iog, nb, sc); });
+        }
+      }
+      HIP_CHECK(hipGetLastError());
+      if (mem == CBS_MEM_HOST) {
+        if (f3d)
+          HIP_CHECK(hipMemcpyAsync(f3d + (size_t)r0 * nri, io_u3d.p, (size_t)nb * nri * sizeof(float2),
+                                   hipMemcpyDeviceToHost, stream));
+        if (trans)
+          HIP_CHECK(hipMemcpyAsync(trans + (size_t)r0 * nf, io_tr.p, (size_t)nb * nf * sizeof(float2),
+                                   hipMemcpyDeviceToHost, stream));
+        if (refl)
+          HIP_CHECK(hipMemcpyAsync(refl + (size_t)r0 * nf, io_rf.p, (size_t)nb * nf * sizeof(float2),
+                                   hipMemcpyDeviceToHost, stream));
       }
     }
     HIP_CHECK(hipStreamSynchronize(stream));
@@ -756,6 +914,7 @@ int cbs_create(const cbs_geometry* geo, cbs_handle* out) {
     s->rx = (int)geo->RI_size[0]; s->ry = (int)geo->RI_size[1]; s->rz = (int)geo->RI_size[2];
     s->fx = (int)geo->field_size[0]; s->fy = (int)geo->field_size[1];
     s->cx = (int)geo->conv_size[0]; s->cy = (int)geo->conv_size[1];
+    s->rc_x = geo->RI_center[0]; s->rc_y = geo->RI_center[1];
     s->nvox = (long long)s->nx * s->ny * s->nz;
     s->nri = (long long)s->rx * s->ry * s->rz;
     s->opt.form = CBS_FORM_FUSED;
@@ -884,9 +1043,9 @@ int cbs_autotune(cbs_handle h, uint32_t n_rhs, uint32_t iterations, double* out_
 int cbs_set_io_kernels(cbs_handle h, const void* refocus, const void* kernel_trans, const void* kernel_ref, int mem) {
   return guarded([&] {
     Solver* s = lookup(h);
-    (void)refocus; (void)kernel_trans; (void)kernel_ref; (void)mem;
-    (void)s;
-    throw CbsError(CBS_ERR_UNSUPPORTED, "2D input/output projections are not built yet");
+    DeviceGuard dg(s->device);
+    s->set_io_kernels(reinterpret_cast<const float2*>(refocus), reinterpret_cast<const float2*>(kernel_trans),
+                      reinterpret_cast<const float2*>(kernel_ref), mem);
   });
 }
 
@@ -894,8 +1053,9 @@ int cbs_solve_2d(cbs_handle h, const void* source_2d, uint32_t n_rhs, void* tran
                  int mem) {
   return guarded([&] {
     Solver* s = lookup(h);
-    (void)source_2d; (void)n_rhs; (void)trans_2d; (void)ref_2d; (void)field_3d; (void)mem; (void)s;
-    throw CbsError(CBS_ERR_UNSUPPORTED, "2D input/output projections are not built yet");
+    DeviceGuard dg(s->device);
+    s->solve_2d(reinterpret_cast<const float2*>(source_2d), n_rhs, reinterpret_cast<float2*>(trans_2d),
+                reinterpret_cast<float2*>(ref_2d), reinterpret_cast<float2*>(field_3d), mem);
   });
 }
 

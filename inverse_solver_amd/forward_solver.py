@@ -19,7 +19,9 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, lib
-from .params import (basic_optical_parameter, update_struct, round_half_away, ri_to_potential)
+from .params import (basic_optical_parameter, update_struct, round_half_away, ri_to_potential, pupil,
+                     centered_fft2, centered_ifft2)
+from .io_kernels import build_io_kernels
 
 
 def get_default_parameters(init=None):
@@ -79,6 +81,8 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
         geo.field_size[0], geo.field_size[1], geo.field_size[2] = int(p['size'][0]), int(p['size'][1]), 1
         geo.conv_size[0], geo.conv_size[1], geo.conv_size[2] = conv[0], conv[1], 1
         geo.device = int(device)
+        rc = list(p.get('RI_center', [0, 0]))
+        geo.RI_center[0], geo.RI_center[1] = int(rc[0]), int(rc[1])
         h = C.c_void_p()
         check(lib.cbs_create(C.byref(geo), C.byref(h)))
         self._h = h
@@ -171,6 +175,80 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
         Vf = np.asfortranarray(V)
         check(lib.cbs_set_potential(self._h, C.byref(prm), _fptr(Vf), _lib.CBS_MEM_HOST))
         self._Vf = Vf
+
+    # ---------------------------------------------------------------- 2D in / 2D out (:397-520)
+    def _ensure_io(self):
+        if self._io_ready:
+            return
+        ref, kt, kr = build_io_kernels(self.parameters, self.expected_RI_size, self.cyclic_boundary_xy)
+        self._io_arrays = [np.asfortranarray(a.astype(np.complex64)) for a in (ref, kt, kr)]
+        check(lib.cbs_set_io_kernels(self._h, *[_fptr(a) for a in self._io_arrays], _lib.CBS_MEM_HOST))
+        self._io_ready = True
+
+    def transform_field_3D(self, F):
+        """FORWARD_SOLVER.m:38-48 (scalar): NA circle, then 1/sqrt(cos theta) (Abbe sine)."""
+        p = self.parameters
+        na, cos_t, _ = pupil(p['size'][:2], p['resolution'], p['wavelength'], p['NA'], p['RI_bg'])
+        shp = na.shape + (1,) * (F.ndim - 2)
+        F = F * na.reshape(shp)
+        if p['use_abbe_sine']:
+            filt = np.where(na, 1.0 / np.sqrt(np.where(na, cos_t, 1.0)), 0.0)
+            F = F * filt.reshape(shp)
+        return F
+
+    def transform_field_2D(self, F):
+        """FORWARD_SOLVER.m:69-77 (scalar; the reflection variant :98-106 is identical for scalars)."""
+        p = self.parameters
+        na, cos_t, _ = pupil(p['size'][:2], p['resolution'], p['wavelength'], p['NA'], p['RI_bg'])
+        shp = na.shape + (1,) * (F.ndim - 2)
+        F = F * na.reshape(shp)
+        if p['use_abbe_sine']:
+            F = F * (np.sqrt(cos_t) * na).reshape(shp)
+        return F
+
+    transform_field_2D_reflection = transform_field_2D
+
+    def solve(self, input_field):
+        """Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:397-520 / _CUDA.m:69-148 (scalar).
+
+        input_field: (x, y, 1, A) (or (x, y, A) / (x, y)) on the field grid.
+        Returns (fields_trans (x, y, 1, A) or None, fields_ref (x, y, 1, A) or None,
+        fields_3D (x, y, z, 1, A) or None) as complex64, following return_* flags."""
+        p = self.parameters
+        inp = np.asarray(input_field)
+        if inp.ndim == 2:
+            inp = inp[:, :, None, None]
+        elif inp.ndim == 3:
+            inp = inp[:, :, None, :]
+        if inp.shape[2] > 1:
+            raise ValueError('the source is 2D but parameter indicate a vectorial simulation')
+        if list(inp.shape[:2]) != [int(p['size'][0]), int(p['size'][1])]:
+            raise ValueError('input field must have the field size %s' % (p['size'][:2],))
+        if not hasattr(self, 'Bornmax'):
+            raise RuntimeError('set_RI must be called before solve')
+        self._ensure_io()
+        A = inp.shape[3]
+        U = self.transform_field_3D(centered_fft2(inp.astype(np.complex128)))          # :416-418
+        src = np.asfortranarray(centered_ifft2(U)[:, :, 0, :].astype(np.complex64))    # _CUDA.m:122
+        fx, fy = int(p['size'][0]), int(p['size'][1])
+        rx, ry, rz = self.expected_RI_size
+        want_t, want_r, want_3 = bool(p['return_transmission']), bool(p['return_reflection']), bool(p['return_3D'])
+        t2 = np.empty((fx, fy, A), np.complex64, order='F') if want_t else None
+        r2 = np.empty((fx, fy, A), np.complex64, order='F') if want_r else None
+        f3 = np.empty((rx, ry, rz, A), np.complex64, order='F') if want_3 else None
+        nul = C.c_void_p(None)
+        check(lib.cbs_solve_2d(self._h, _fptr(src), A, _fptr(t2) if want_t else nul, _fptr(r2) if want_r else nul,
+                               _fptr(f3) if want_3 else nul, _lib.CBS_MEM_HOST))
+        trans = ref = field3d = None
+        if want_t:                                                                      # :481-492
+            T = centered_fft2(t2.astype(np.complex128)) + U[:, :, 0, :]
+            trans = centered_ifft2(self.transform_field_2D(T)).astype(np.complex64)[:, :, None, :]
+        if want_r:                                                                      # :505-508
+            R = centered_fft2(r2.astype(np.complex128))
+            ref = centered_ifft2(self.transform_field_2D_reflection(R)).astype(np.complex64)[:, :, None, :]
+        if want_3:
+            field3d = f3[:, :, :, None, :]
+        return trans, ref, field3d
 
     # ---------------------------------------------------------------- Born loop from a 3D incident field
     def solve_incident(self, u_in):
