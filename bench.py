@@ -64,6 +64,200 @@ def pass_real_bytes(nvox, A, M):
     return nvox * np.array([one(jj) for jj in range(0, M + 1)], dtype=np.float64)
 
 
+def _rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+def _line(metric, value, unit, workload, ms, args, extra, hib=True, dtype='complex64'):
+    out = {'metric': metric, 'value': value, 'unit': unit, 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+           'ms_per_step': round(ms, 3), 'higher_is_better': hib, 'scaling': 'weak', 'vs_baseline': None,
+           'dtype': dtype, 'data': 'synthetic', 'config': {'workload': workload}}
+    out.update(extra)
+    return out
+
+
+def workload_c1(args):
+    """64^3 RI + 16-px boundary (96^3 sim), centred SiO2 bead (n 1.46, 30-voxel
+    diameter), one normal-incidence plane wave, 100 iterations: the complete 2D-in /
+    2D-out solve on the GPU against the FP64 oracle and the complex64 CPU port."""
+    from inverse_solver_amd import FORWARD_SOLVER_CONVERGENT_BORN
+    from inverse_solver_amd.phantoms import bead_ri, circle_illumination
+    from oracle.cbs import ForwardSolverConvergentBorn as Oracle
+    ri = [64, 64, 64]
+    p = workload_params(ri, 100, 1)
+    p.update(return_reflection=True)
+    RI = bead_ri(ri, 30, 1.46, 1.336)
+    u = circle_illumination(ri[:2], 1, 1.2, 0.532, [0.1] * 3)
+    s = FORWARD_SOLVER_CONVERGENT_BORN(p, options=dict(time_kernels=1))
+    s.set_RI(RI)
+    s.solve(u)                                   # kernels, plans, warm-up
+    times = []
+    for _ in range(max(args.steps, 3)):
+        t0 = time.perf_counter()
+        t, r, f = s.solve(u)
+        times.append((time.perf_counter() - t0) * 1e3)
+    ms = float(np.median(times))
+    nvox = int(np.prod(s.simulation_size))
+    cores = os.cpu_count() or 1
+    t0 = time.perf_counter()
+    o = Oracle(p, workers=cores)
+    o.set_RI(RI)
+    tr, rr, fr = o.solve(u)
+    t_or = time.perf_counter() - t0
+    oc = Oracle(p, dtype=np.complex64, workers=cores)
+    oc.set_RI(RI)
+    oc.solve(u)
+    oc.set_RI(RI)
+    t0 = time.perf_counter()
+    oc.solve(u)
+    t_cpu = time.perf_counter() - t0
+    val = nvox * 100 / (ms / 1e3) / 1e9
+    return _line('Gvoxel-iter/s at <=1e-5 rel err vs FP64 oracle (full 2D solve, single angle)', round(val, 3),
+                 'Gvoxel-iter/s', 'C1: 64^3 RI + 16-px boundary = 96^3 sim, bead n=1.46 d=30 px, 1 angle, '
+                 '100 iterations, full 2D in/out solve', ms, args,
+                 {'latency_ms': {'p50': ms, 'p99': float(np.percentile(times, 99))},
+                  'accuracy': {'rel_l2_err_3d': _rel(f, fr), 'rel_l2_err_trans': _rel(t, tr),
+                               'rel_l2_err_ref': _rel(r, rr), 'tolerance': 1e-5},
+                  'plan': s.plan(),
+                  'cpu_baseline': {'value': round(nvox * 100 / t_cpu / 1e9, 5), 'unit': 'Gvoxel-iter/s',
+                                   'cores': cores, 'kind': 'port',
+                                   'sample': 'the whole C1 solve (complex64 NumPy port of solve/solve_raw), '
+                                             '%.1f s; FP64 oracle %.1f s' % (t_cpu, t_or)}})
+
+
+def workload_inverse(args):
+    """C3: RI 96x96x97 (+16/16/19 px -> 128x128x135 sim), 5 seeded spheres,
+    24 circle angles at NA 1.2, measured fields from the GPU forward solve plus
+    10 % complex Gaussian noise; BACKWARD_SOLVER_MULTI with TV (host prox)."""
+    from inverse_solver_amd import FORWARD_SOLVER_CONVERGENT_BORN
+    from inverse_solver_amd.backward_solver import BACKWARD_SOLVER_MULTI, get_default_parameters
+    from inverse_solver_amd.phantoms import spheres, circle_illumination
+    ri = [96, 96, 97]
+    bt_xy = 8.0
+    bt_z = 19 / (0.532 / 1.336 / 0.2)            # 19 px at 0.1 um
+    fp = workload_params(ri, -1, 24)
+    fp.update(boundary_thickness=[bt_xy, bt_xy, bt_z], return_reflection=True, return_3D=True)
+    RI_true, _ = spheres(tuple(ri), 5, (0.5, 2.0), (1.36, 1.48), seed=11)
+    u = circle_illumination(ri[:2], 24, 1.2, 0.532, [0.1] * 3)
+    f = FORWARD_SOLVER_CONVERGENT_BORN(fp)
+    f.set_RI(RI_true)
+    meas, _, _ = f.solve(u)
+    sim = f.simulation_size
+    f.close()
+    rng = np.random.default_rng(12)
+    sig = 0.1 * np.sqrt(np.mean(np.abs(meas) ** 2))
+    meas = meas + sig * (rng.standard_normal(meas.shape) + 1j * rng.standard_normal(meas.shape)) / np.sqrt(2)
+    bp = get_default_parameters(dict(forward_solver_parameters=fp, step=0.01, tv_param=0.001,
+                                     inner_itt=args.inner, itter_max=args.outer, verbose=False))
+    b = BACKWARD_SOLVER_MULTI(bp)
+    t0 = time.perf_counter()
+    RI = b.solve(u, meas, init_RI=np.full(ri, 1.336))
+    wall = time.perf_counter() - t0
+    g_t = float(np.mean([x[0] for x in b.timings]))
+    tv_t = float(np.mean([x[1] for x in b.timings]))
+    nvox = int(np.prod(sim))
+    M = b.forward.Bornmax
+    err = float(np.sum(np.abs(RI - RI_true) ** 2) / np.sum(np.abs(1.336 - RI_true) ** 2))
+    b.close()
+    return _line('seconds per gradient (forward + adjoint CBS over 24 angles)', round(g_t, 4), 's',
+                 'C3: 96x96x97 RI -> 128x128x135 sim, 24 angles, %d iterations per solve, %d outer '
+                 'iterations, TV inner %d' % (M, args.outer, args.inner), g_t * 1e3, args,
+                 {'gradient_gvoxel_iter_per_s': round(2 * 24 * nvox * M / g_t / 1e9, 3),
+                  'tv_prox_seconds': round(tv_t, 3), 'wall_seconds': round(wall, 2),
+                  'misfit_first_last': [b.err_list[0], b.err_list[-1]],
+                  'ri_rel_error_vs_phantom': err}, hib=False)
+
+
+def workload_serving(args):
+    """C4: Poisson arrivals of forward/adjoint requests; grids RI {64^3, 96^3, 128^3,
+    160^2x96, 256^2x128} + 16 px; angles U{1..61}; 1-20 spheres, contrast <= 0.15;
+    admission by HBM footprint."""
+    from inverse_solver_amd.serving import Scheduler, Request, SolverCache, estimate_footprint, summarize
+    from inverse_solver_amd.phantoms import spheres, circle_illumination
+    from inverse_solver_amd import device as dv
+    rng = np.random.default_rng(21)
+    grids = [(64, 64, 64), (96, 96, 96), (128, 128, 128), (160, 160, 96), (256, 256, 128)]
+    reqs = []
+    t = 0.0
+    for i in range(args.requests):
+        t += rng.exponential(1.0 / args.rate)
+        ri = grids[rng.integers(len(grids))]
+        A = int(rng.integers(1, 62))
+        ns = int(rng.integers(1, 21))
+        RI, _ = spheres(ri, ns, (0.3, 1.2), (1.336, 1.336 + 0.15), seed=1000 + i)
+        p = workload_params(list(ri), -1, A)
+        p.update(return_reflection=False, return_3D=False)
+        sim = [n + 32 for n in ri]
+        reqs.append(Request(arrival=t, rid=i, kind='adjoint' if rng.random() < 0.5 else 'forward', params=p,
+                            RI=RI, fields=circle_illumination(ri[:2], A, 1.2, 0.532, [0.1] * 3),
+                            footprint=estimate_footprint(sim, ri, [2 * ri[0], 2 * ri[1]], A)))
+    free, total = dv.mem_info()
+    cache = SolverCache(options=dict(max_batch=16))
+    sched = Scheduler(cache, hbm_budget=int(free * 0.8), slots=4)
+    done = sched.run(reqs, realtime=True)
+    cache.close()
+
+    def vi(r):
+        from inverse_solver_amd.params import round_half_away
+        sim = [n + 32 for n in r.RI.shape]
+        k = 2 * np.pi * 1.336 / 0.532
+        vt = np.max(np.abs(k * k * (r.RI ** 2 / 1.336 ** 2 - 1)))
+        step = 2 * k / (1.01 * vt)
+        M = np.ceil(np.linalg.norm(np.array(sim) * 0.1) / step / 2 + 1) * 2
+        return float(np.prod(sim)) * M * r.fields.shape[3]
+    summ = summarize(done, vi)
+    lat = summ.get('latency_ms', {})
+    return _line('serving throughput (Gvoxel-iter/s) with p50/p99 request latency', round(
+        summ.get('gvoxel_iter_per_s', 0.0), 3), 'Gvoxel-iter/s',
+        'C4: %d Poisson requests at %.1f/s, mixed forward/adjoint, 5 grid sizes, 1-61 angles' % (
+            args.requests, args.rate), lat.get('p50', float('nan')), args,
+        {'latency_ms': lat, 'queue_ms': summ.get('queue_ms'), 'completed': summ.get('completed'),
+         'failed': summ.get('failed'), 'span_s': summ.get('span_s'), 'hbm_budget_gb': round(free * 0.8 / 2 ** 30, 1)})
+
+
+def workload_large(args):
+    """C5: RI 512x512x256 Gaussian random field in an ellipsoid (n in [1.33, 1.40]),
+    32-px boundary -> 576x576x320 sim, 30 angles, 300 iterations (Born loop from
+    the analytic incident plane waves)."""
+    from inverse_solver_amd import FORWARD_SOLVER_CONVERGENT_BORN
+    from inverse_solver_amd import device as dv
+    from inverse_solver_amd.phantoms import gaussian_random_field, incident_plane_waves
+    from oracle.cbs import ForwardSolverConvergentBorn as Oracle
+    ri = [512, 512, 256]
+    A, M = 30, args.iters if args.iters != 200 else 300
+    RI = gaussian_random_field(tuple(ri), 0.5, (1.33, 1.40), seed=31)
+    p = workload_params(ri, M, A)
+    p.update(boundary_thickness=[16, 16, 16])
+    s = FORWARD_SOLVER_CONVERGENT_BORN(p, options=dict(time_kernels=1))
+    s.set_RI(RI)
+    u = incident_plane_waves(ri, A, 1.2, 0.532, 1.336, [0.1] * 3)
+    u_host = np.ascontiguousarray(np.transpose(u, (3, 2, 1, 0)))
+    del u
+    d_in = dv.DeviceArray.from_host(u_host)
+    d_out = dv.DeviceArray(u_host.shape, np.complex64)
+    t0 = time.perf_counter()
+    s.solve_incident_device(d_in.ptr, A, d_out.ptr)
+    dv.synchronize()
+    sec = time.perf_counter() - t0
+    nvox = int(np.prod(s.simulation_size))
+    val = nvox * M * A / sec / 1e9
+    # parity sample: 1 angle x 6 iterations against the FP64 oracle
+    pc = workload_params(ri, 6, 1)
+    pc.update(boundary_thickness=[16, 16, 16])
+    sg = FORWARD_SOLVER_CONVERGENT_BORN(pc)
+    sg.set_RI(RI)
+    u0 = u_host[0].transpose(2, 1, 0)
+    got = sg.solve_incident(u0)
+    sg.close()
+    o = Oracle(pc, io_kernels=False, workers=os.cpu_count())
+    o.set_RI(RI)
+    err = _rel(got, o.born_loop(u0))
+    return _line('Gvoxel-iter/s (large volume)', round(val, 3), 'Gvoxel-iter/s',
+                 'C5: 512x512x256 RI + 32-px boundary = 576x576x320 sim, 30 angles, %d iterations' % M,
+                 sec * 1e3, args, {'plan': s.plan(), 'accuracy': {'rel_l2_err_vs_fp64_oracle': err, 'angles': 1,
+                                                                   'iterations': 6}})
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     ap.add_argument('--gpus', type=int, default=1)
@@ -82,12 +276,25 @@ def main():
     ap.add_argument('--no-tune', action='store_true', help='use the planner default instead of the autotuner')
     ap.add_argument('--copy-probe-mb', type=int, default=0,
                     help='run one device-to-device copy of this many MiB first (calibrates HBM byte counters)')
+    ap.add_argument('--workload', default='batched', choices=['batched', 'c1', 'inverse', 'serving', 'large'],
+                    help='batched: the headline 49-angle solve (default); the others are the secondary '
+                         'configurations, each printing its own JSON line')
+    ap.add_argument('--requests', type=int, default=200, help='serving: number of requests')
+    ap.add_argument('--rate', type=float, default=4.0, help='serving: Poisson arrival rate (requests/s)')
+    ap.add_argument('--outer', type=int, default=20, help='inverse: outer iterations')
+    ap.add_argument('--inner', type=int, default=100, help='inverse: TV inner iterations')
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(json.dumps({'error': 'this build runs on one GPU; --gpus %d is not supported (the angle-parallel '
                                    'sharding across GPUs is described in DESIGN.md)' % args.gpus}))
         sys.exit(2)
+
+    if args.workload != 'batched':
+        out = {'c1': workload_c1, 'inverse': workload_inverse, 'serving': workload_serving,
+               'large': workload_large}[args.workload](args)
+        print(json.dumps(out), flush=True)
+        return
 
     from inverse_solver_amd import FORWARD_SOLVER_CONVERGENT_BORN
     from inverse_solver_amd import device as dv

@@ -125,6 +125,14 @@ int cbs_set_io_kernels(cbs_handle h, const void* refocus, const void* kernel_tra
 int cbs_solve_2d(cbs_handle h, const void* source_2d, uint32_t n_rhs, void* trans_2d, void* ref_2d, void* field_3d,
                  int mem);
 
+/* Isotropic 3D TV proximal step of the reconstruction loop, on the handle's
+ * device and stream: replaces Codes/sub_code/TV_FISTA_inner_v2.m for a real
+ * input (kappamax = 0), in single precision as the reference runs it.
+ * in/out: dims[0]*dims[1]*dims[2] floats (x fastest), host or device (mem);
+ * they may alias.  non_neg clamps to [vmin, vmax]; dirichlet zeroes the faces. */
+int cbs_tv_prox(cbs_handle h, const float* in, float* out, const uint32_t dims[3], double lambda, double vmin,
+                double vmax, int non_neg, int dirichlet, uint32_t inner_itt, int mem);
+
 /* FP64 per-iteration |f_n|^2 of the last solve (track_norms=1): out[r * n_iter + j]. */
 int cbs_get_norms(cbs_handle h, double* out, uint32_t n_rhs, uint32_t n_iter);
 

@@ -15,7 +15,7 @@ import numpy as np
 
 from .forward_solver import FORWARD_SOLVER_CONVERGENT_BORN, get_default_parameters as fwd_defaults
 from .params import update_struct
-from .tv import tv_fista_inner
+from .tv import tv_fista_inner, tv_fista_inner_device
 
 
 def get_default_parameters(init=None):
@@ -24,7 +24,7 @@ def get_default_parameters(init=None):
     p.update(forward_solver_parameters=fwd_defaults(dict(return_transmission=True, return_reflection=True,
                                                          return_3D=True, vector_simulation=False)),
              step=0.01, tv_param=0.001, use_non_negativity=False, nmin=-np.inf, nmax=np.inf, kappamax=0,
-             inner_itt=100, itter_max=100, num_scan_per_iteration=0, verbose=True)
+             inner_itt=100, itter_max=100, num_scan_per_iteration=0, verbose=True, tv_on_device=True)
     return update_struct(p, init) if init is not None else p
 
 
@@ -110,8 +110,12 @@ class BACKWARD_SOLVER_MULTI:
             t1 = time.perf_counter()
             self.err_list.append(err)
             t_n = t_np
-            s_n = tv_fista_inner(u_n - (1 / alpha) * grad, p['tv_param'] / alpha, Vmin, Vmax, Vimag_max,
-                                 p['use_non_negativity'], False, p['inner_itt'])
+            if Vimag_max == 0 and p.get('tv_on_device', True):
+                s_n = tv_fista_inner_device(self.forward, u_n - (1 / alpha) * grad, p['tv_param'] / alpha, Vmin,
+                                            Vmax, p['use_non_negativity'], False, p['inner_itt'])
+            else:
+                s_n = tv_fista_inner(u_n - (1 / alpha) * grad, p['tv_param'] / alpha, Vmin, Vmax, Vimag_max,
+                                     p['use_non_negativity'], False, p['inner_itt'])
             t_np = (1 + np.sqrt(1 + 4 * t_n ** 2)) / 2
             u_n = s_n + (t_n - 1) / t_np * (s_n - x_n)
             x_n = s_n

@@ -242,33 +242,21 @@ __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
   }
   __syncthreads();
   auto zl = [](int l, int e) { return e * LW + l; };
-  line_fft<NZ, false, LW, NT>(lds, zl, tb.tw_z, tid);
   {
-    // (1/N) / (d - i eps) = (d + i eps) / (N (d^2 + eps^2)),  d = 4 pi^2 |k_s|^2 - k_m^2
+    // (1/N) / (d - i eps) = (d + i eps) / (N (d^2 + eps^2)),  d = 4 pi^2 |k_s|^2 - k_m^2,
+    // applied between the forward and inverse z transforms of each line
     const float* gz = tb.green_z + flip * c.nz;
-    if constexpr (NT % LW == 0) {
-      const int l = tid % LW;
+    const float* gx = tb.green_x + flip * c.nx + x0;
+    const float* gy = tb.green_y + flip * c.ny + p1;
+    const float km2 = c.km2, eps2 = c.eps * c.eps, inv_n = c.inv_n, eps = c.eps;
+    auto green = [&](int l, int pz, float2 v) {
       const int k2 = l / T, t = l - k2 * T;
-      const float gxy = tb.green_x[flip * c.nx + x0 + (t < tvalid ? t : 0)] + tb.green_y[flip * c.ny + k2 * c.a + p1] -
-                        c.km2;
-      for (int pz = tid / LW; pz < NZ; pz += NT / LW) {
-        const float d = gxy + gz[pz];
-        const float s = c.inv_n / fmaf(d, d, c.eps * c.eps);
-        lds[pz * LW + l] = cmul(lds[pz * LW + l], make_float2(d * s, c.eps * s));
-      }
-    } else {
-      for (int i = tid; i < NZ * LW; i += NT) {
-        const int pz = i / LW, l = i - pz * LW;
-        const int k2 = l / T, t = l - k2 * T;
-        const float d = tb.green_x[flip * c.nx + x0 + (t < tvalid ? t : 0)] + tb.green_y[flip * c.ny + k2 * c.a + p1] +
-                        gz[pz] - c.km2;
-        const float s = c.inv_n / fmaf(d, d, c.eps * c.eps);
-        lds[i] = cmul(lds[i], make_float2(d * s, c.eps * s));
-      }
-    }
+      const float d = gx[t < tvalid ? t : 0] + gy[k2 * c.a] + gz[pz] - km2;
+      const float s = inv_n / fmaf(d, d, eps2);
+      return cmul(v, make_float2(d * s, eps * s));
+    };
+    line_fft_mul_ifft<NZ, LW, NT>(lds, zl, tb.tw_z, tid, green);
   }
-  __syncthreads();
-  line_fft<NZ, true, LW, NT>(lds, zl, tb.tw_z, tid);
 #pragma unroll
   for (int k = 0; k < L; ++k) {
     const int task = tid + k * NT;

@@ -287,4 +287,68 @@ __device__ __forceinline__ void line_fft(float2* __restrict__ lds, const Addr& a
   }
 }
 
+// Forward FFT, pointwise multiply, inverse FFT on NL lines in LDS, with the
+// last forward step, the multiply and the first inverse step fused in
+// registers (they act on the same R2 values of one task).  mul(l, p, v)
+// returns v times the factor for line l at position p (forward output order).
+// Result: the circular convolution, unnormalised (N times the usual inverse).
+template <int N, int NL, int NT, class Addr, class Mul>
+__device__ __forceinline__ void line_fft_mul_ifft(float2* __restrict__ lds, const Addr& addr,
+                                                  const float2* __restrict__ tw, int tid, const Mul& mul) {
+  using P = LineFft<N>;
+  constexpr int R1 = P::R1, R2 = P::R2;
+  if constexpr (R2 == 1) {
+    for (int l = tid; l < NL; l += NT) {
+      float2 v[N];
+#pragma unroll
+      for (int e = 0; e < N; ++e) v[e] = lds[addr(l, e)];
+      Dft<N, false>::run(v);
+#pragma unroll
+      for (int e = 0; e < N; ++e) v[e] = mul(l, e, v[e]);
+      Dft<N, true>::run(v);
+#pragma unroll
+      for (int e = 0; e < N; ++e) lds[addr(l, e)] = v[e];
+    }
+    __syncthreads();
+  } else {
+    for (int t = tid; t < NL * R2; t += NT) {           // forward step A
+      const int l = t % NL, n2 = t / NL;
+      float2 v[R1];
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) v[n1] = lds[addr(l, R2 * n1 + n2)];
+      Dft<R1, false>::run(v);
+#pragma unroll
+      for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], tw[n2 * k1]);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) lds[addr(l, R2 * k1 + n2)] = v[k1];
+    }
+    __syncthreads();
+    for (int t = tid; t < NL * R1; t += NT) {           // step B, multiply, inverse step B
+      const int l = t % NL, k1 = t / NL;
+      float2 v[R2];
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) v[n2] = lds[addr(l, R2 * k1 + n2)];
+      Dft<R2, false>::run(v);
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) v[k2] = mul(l, R2 * k1 + k2, v[k2]);
+      Dft<R2, true>::run(v);
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) lds[addr(l, R2 * k1 + n2)] = v[n2];
+    }
+    __syncthreads();
+    for (int t = tid; t < NL * R2; t += NT) {           // inverse step A
+      const int l = t % NL, n2 = t / NL;
+      float2 v[R1];
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) v[k1] = lds[addr(l, R2 * k1 + n2)];
+#pragma unroll
+      for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmulc(v[k1], tw[n2 * k1]);
+      Dft<R1, true>::run(v);
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) lds[addr(l, R2 * n1 + n2)] = v[n1];
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace cbs

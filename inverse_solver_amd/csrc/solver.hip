@@ -17,6 +17,7 @@
 #include "cbs_passes.hpp"
 #include "io_kernels.hpp"
 #include "kernels_misc.hpp"
+#include "tv_kernels.hpp"
 
 namespace cbs {
 
@@ -151,6 +152,14 @@ struct Solver {
   DevBuf<float2> io_buf3, io_spec2, io_out_t, io_out_r, io_uin, io_src, io_u3d, io_tr, io_rf;
   const RowsInfo* io_rows = nullptr;
   const StridedInfo* io_cols = nullptr;
+
+  void drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+    graph_key.clear();
+  }
 
   ~Solver() {
     int prev = 0;
@@ -518,7 +527,29 @@ struct Solver {
     if (M == 0) {
       HIP_CHECK(hipMemsetAsync(field.p, 0, (size_t)nb * nvox * sizeof(float2), stream));
     } else if (plan.form == CBS_FORM_FUSED) {
-      run_fused(nb, M);
+      if (opt.use_graph && !opt.time_kernels) {
+        // replay a captured iteration loop (the kernel arguments, including the
+        // potential's scalars, are baked in; the cache is dropped whenever the
+        // potential, the options or the plan change)
+        const std::vector<long long> key = {nb, M, (long long)field.p, (long long)fstate.p, (long long)spec.p,
+                                            (long long)parity.p, (long long)(opt.track_norms ? norms.p : nullptr),
+                                            (long long)plan.pr, (long long)plan.pk};
+        if (!graph_exec || key != graph_key) {
+          drop_graph();
+          HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+          run_fused(nb, M);
+          hipError_t e = hipStreamEndCapture(stream, &graph);
+          if (e != hipSuccess) {
+            graph = nullptr;
+            throw CbsError(CBS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+          }
+          HIP_CHECK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+          graph_key = key;
+        }
+        HIP_CHECK(hipGraphLaunch(graph_exec, stream));
+      } else {
+        run_fused(nb, M);
+      }
     } else {
       for (int r = 0; r < nb; ++r) run_plain(r, nb, M);
     }
@@ -735,6 +766,7 @@ struct Solver {
     const int npr = list_pass_real(&prs);
     const PassKInfo* pks = nullptr;
     const int npk = list_pass_kspace(&pks);
+    drop_graph();
     const cbs_options saved = opt;
     opt.split_a = opt.tile_t = opt.threads_real = opt.threads_kspace = 0;
     opt.form = CBS_FORM_FUSED;
@@ -820,6 +852,46 @@ struct Solver {
     return best;
   }
 
+  DevBuf<float> tv_in, tv_y, tv_P, tv_R;
+
+  void tv_prox(const float* in, float* out, const uint32_t* dims, double lam, double vmin, double vmax, int non_neg,
+               int dirichlet, uint32_t inner, int mem) {
+    require(in && out && dims, CBS_ERR_ARG, "null TV argument");
+    require(dims[0] >= 1 && dims[1] >= 1 && dims[2] >= 1, CBS_ERR_ARG, "TV dims must be >= 1");
+    require(lam > 0, CBS_ERR_ARG, "TV lambda must be > 0");
+    require(mem == CBS_MEM_HOST || mem == CBS_MEM_DEVICE, CBS_ERR_ARG, "mem must be CBS_MEM_HOST or CBS_MEM_DEVICE");
+    const long long n = (long long)dims[0] * dims[1] * dims[2];
+    tv_y.ensure((size_t)n);
+    tv_P.ensure((size_t)(3 * n));
+    tv_R.ensure((size_t)(3 * n));
+    const float* din = in;
+    if (mem == CBS_MEM_HOST) {
+      tv_in.ensure((size_t)n);
+      HIP_CHECK(hipMemcpyAsync(tv_in.p, in, (size_t)n * sizeof(float), hipMemcpyHostToDevice, stream));
+      din = tv_in.p;
+    }
+    TvArgs a;
+    a.nx = (int)dims[0]; a.ny = (int)dims[1]; a.nz = (int)dims[2];
+    a.lam = (float)lam; a.vmin = (float)vmin; a.vmax = (float)vmax;
+    a.non_neg = non_neg; a.dirichlet = dirichlet;
+    hipLaunchKernelGGL(tv_zero, dim3(grid_for(3 * n)), dim3(256), 0, stream, tv_P.p, 3 * n);
+    hipLaunchKernelGGL(tv_zero, dim3(grid_for(3 * n)), dim3(256), 0, stream, tv_R.p, 3 * n);
+    double t = 1.0;
+    for (uint32_t it = 0; it < inner; ++it) {
+      const double tn = (1.0 + std::sqrt(1.0 + 4.0 * t * t)) / 2.0;
+      hipLaunchKernelGGL(tv_div_project, dim3(grid_for(n)), dim3(256), 0, stream, din, tv_R.p, tv_y.p, a);
+      hipLaunchKernelGGL(tv_dual_update, dim3(grid_for(n)), dim3(256), 0, stream, tv_y.p, tv_P.p, tv_R.p, a,
+                         (float)((t - 1.0) / tn));
+      t = tn;
+    }
+    float* dout = mem == CBS_MEM_HOST ? tv_y.p : out;
+    hipLaunchKernelGGL(tv_div_project, dim3(grid_for(n)), dim3(256), 0, stream, din, tv_P.p, dout, a);
+    HIP_CHECK(hipGetLastError());
+    if (mem == CBS_MEM_HOST)
+      HIP_CHECK(hipMemcpyAsync(out, tv_y.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+  }
+
   void set_potential(const cbs_potential_params* p, const float2* v, int mem) {
     require(p != nullptr && v != nullptr, CBS_ERR_ARG, "null potential argument");
     require(mem == CBS_MEM_HOST || mem == CBS_MEM_DEVICE, CBS_ERR_ARG, "mem must be CBS_MEM_HOST or CBS_MEM_DEVICE");
@@ -838,6 +910,7 @@ struct Solver {
                                                        std::to_string(j));
     }
     prm = *p;
+    drop_graph();
     V.ensure((size_t)nvox);
     HIP_CHECK(hipMemcpyAsync(V.p, v, (size_t)nvox * sizeof(float2),
                              mem == CBS_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, stream));
@@ -951,6 +1024,7 @@ int cbs_set_options(cbs_handle h, const cbs_options* opt) {
     require(opt->max_batch >= 0 && opt->split_a >= 0 && opt->tile_t >= 0, CBS_ERR_ARG, "negative option");
     DeviceGuard dg(s->device);
     s->opt = *opt;
+    s->drop_graph();
     if (s->have_potential) {
       s->make_plan();
       s->build_tables();
@@ -979,6 +1053,15 @@ int cbs_solve_3d(cbs_handle h, const void* u_in, uint32_t n_rhs, void* u_out, in
     Solver* s = lookup(h);
     DeviceGuard dg(s->device);
     s->solve_3d(reinterpret_cast<const float2*>(u_in), n_rhs, reinterpret_cast<float2*>(u_out), mem);
+  });
+}
+
+int cbs_tv_prox(cbs_handle h, const float* in, float* out, const uint32_t dims[3], double lambda, double vmin,
+                double vmax, int non_neg, int dirichlet, uint32_t inner_itt, int mem) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    DeviceGuard dg(s->device);
+    s->tv_prox(in, out, dims, lambda, vmin, vmax, non_neg, dirichlet, inner_itt, mem);
   });
 }
 

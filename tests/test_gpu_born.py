@@ -97,3 +97,23 @@ def test_short_series(iters):
     s.set_RI(RI)
     assert rel(s.solve_incident(u), ref[..., 0]) < TOL
     s.close()
+
+
+def test_graph_replay_equals_eager():
+    """use_graph captures the iteration loop once and replays it; every replay
+    (with eager work and synchronisation in between) must equal the eager solve."""
+    ri = (16, 16, 16)
+    p = params(ri, iterations=14)
+    RI = sphere_ri(ri, 0.5)
+    fields = np.stack([random_field(ri, 20 + s) for s in range(2)], -1)
+    s = _gpu()(p)
+    s.set_RI(RI)
+    eager = s.solve_incident(fields)
+    g = _gpu()(p, options=dict(use_graph=1))
+    g.set_RI(RI)
+    for rep in range(4):
+        got = g.solve_incident(fields)
+        np.testing.assert_array_equal(got, eager)
+        s.solve_incident(fields[..., :1])       # eager work on another handle in between
+    s.close()
+    g.close()

@@ -128,3 +128,21 @@ def test_gpu_gradient_and_fista_match_oracle():
     assert d < 1e-4
     np.testing.assert_allclose(b.err_list, errs, rtol=1e-4)
     b.close()
+
+
+@pytest.mark.gpu
+def test_gpu_tv_prox_matches_oracle():
+    from inverse_solver_amd import FORWARD_SOLVER_CONVERGENT_BORN
+    from inverse_solver_amd.tv import tv_fista_inner_device
+    from oracle.backward import tv_fista_inner_v2
+    ri = (16, 16, 15)
+    s = FORWARD_SOLVER_CONVERGENT_BORN(params(ri, iterations=4))
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((20, 18, 17)) * 0.05
+    for args in [(0.02, -np.inf, np.inf, False), (0.01, -0.03, 0.04, True)]:
+        lam, lo, hi, nn = args
+        got = tv_fista_inner_device(s, x, lam, lo, hi, nn, False, 60)
+        ref = tv_fista_inner_v2(x, lam, lo, hi, 0, nn, False, 60)
+        print('TV device vs oracle', rel(got, ref))
+        assert rel(got, ref) < 2e-5
+    s.close()
