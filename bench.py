@@ -64,6 +64,16 @@ def pass_real_bytes(nvox, A, M):
     return nvox * np.array([one(jj) for jj in range(0, M + 1)], dtype=np.float64)
 
 
+def measured_traffic(sim, A, plan, alg_per_launch):
+    """HBM bytes per pass_real launch from the calibrated FETCH_SIZE/WRITE_SIZE runs
+    recorded in profiles/ for this exact workload and plan (None otherwise)."""
+    fn = os.path.join(ROOT, 'profiles', 'pass_real_traffic_160cubed_49rhs.json')
+    if not os.path.exists(fn) or list(sim) != [160, 160, 160] or A != 49 or plan.get('a') != 40:
+        return None
+    rec = json.load(open(fn))
+    return round(alg_per_launch * rec['measured_over_algorithmic'])
+
+
 def _rel(a, b):
     return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
 
@@ -269,7 +279,7 @@ def main():
     ap.add_argument('--seed', type=int, default=2)
     ap.add_argument('--check-iters', type=int, default=24,
                     help='iterations of the FP64-oracle accuracy check (1 angle, same grid)')
-    ap.add_argument('--cpu-iters', type=int, default=30,
+    ap.add_argument('--cpu-iters', type=int, default=250,
                     help='iterations of the complex64 CPU-port sample (1 angle, same grid)')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline and the oracle check')
     ap.add_argument('--no-plain', action='store_true', help='skip the plain single-angle comparison')
@@ -381,7 +391,7 @@ def main():
                    'plan': plan, 'parallelism': 'single GPU, all angles in each launch'},
         'latency_ms': {'p50': round(p50, 3), 'p99': round(p99, 3)},
         'roofline': {'bound': 'hbm', 'achieved': round(achieved_pr, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': round(achieved_pr / HBM_PEAK_GBS, 4), 'traffic': None,
+                     'frac': round(achieved_pr / HBM_PEAK_GBS, 4), 'traffic': measured_traffic(sim, A, plan, bytes_pr / pr_launches),
                      'kernel': 'pass_real', 'avg_launch_ms': round(pr_avg_ms, 4),
                      'algorithmic_bytes_per_launch': round(bytes_pr / pr_launches),
                      'pass_kspace': {'achieved': round(achieved_pk, 1), 'avg_launch_ms': round(pk_avg_ms, 4),
@@ -434,9 +444,10 @@ def main():
         log('accuracy rel err %.3e (%d iterations, oracle %.1f s)' % (err, args.check_iters, t_or))
         # CPU baseline: complex64 op-for-op port of the reference loop on this host
         pcpu = workload_params(ri, args.cpu_iters, 1)
+        ow = Oracle(workload_params(ri, 3, 1), dtype=np.complex64, io_kernels=False, workers=cores)
+        ow.set_RI(RI)
+        ow.born_loop(u0)          # warm-up (FFT plans, page-in)
         oc = Oracle(pcpu, dtype=np.complex64, io_kernels=False, workers=cores)
-        oc.set_RI(RI)
-        oc.born_loop(u0)          # warm-up (plans, page-in)
         oc.set_RI(RI)
         t0 = time.perf_counter()
         oc.born_loop(u0)

@@ -117,7 +117,7 @@ def test_gpu_mixed_stream_equals_alone():
     from inverse_solver_amd.serving import SolverCache
     from inverse_solver_amd.phantoms import circle_illumination
     rng = np.random.default_rng(0)
-    cfgs = [(16, 16, 15), (24, 24, 15)]
+    cfgs = [(16, 16, 17), (24, 24, 17)]
     reqs = []
     for i in range(6):
         ri = cfgs[i % 2]

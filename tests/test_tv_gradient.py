@@ -100,7 +100,7 @@ def test_gpu_gradient_and_fista_match_oracle():
     from inverse_solver_amd.phantoms import circle_illumination
     from oracle.cbs import ForwardSolverConvergentBorn as Oracle
     from oracle.backward import get_gradiant_static, backward_multi_solve
-    ri = (16, 16, 15)
+    ri = (16, 16, 17)
     p = params(ri, iterations=12)
     RI_true = sphere_ri(ri, 0.5, n_s=1.39)
     u = circle_illumination(ri[:2], 4, 1.2, 0.532, [0.1] * 3)
@@ -135,7 +135,7 @@ def test_gpu_tv_prox_matches_oracle():
     from inverse_solver_amd import FORWARD_SOLVER_CONVERGENT_BORN
     from inverse_solver_amd.tv import tv_fista_inner_device
     from oracle.backward import tv_fista_inner_v2
-    ri = (16, 16, 15)
+    ri = (16, 16, 17)
     s = FORWARD_SOLVER_CONVERGENT_BORN(params(ri, iterations=4))
     rng = np.random.default_rng(4)
     x = rng.standard_normal((20, 18, 17)) * 0.05
