@@ -133,6 +133,25 @@ int cbs_solve_2d(cbs_handle h, const void* source_2d, uint32_t n_rhs, void* tran
 int cbs_tv_prox(cbs_handle h, const float* in, float* out, const uint32_t dims[3], double lambda, double vmin,
                 double vmax, int non_neg, int dirichlet, uint32_t inner_itt, int mem);
 
+/* The same proximal step for a complex potential: Codes/sub_code/TV_FISTA_inner_v2.m
+ * when Vimag_max != 0, which includes the reference defaults (nmax = Inf and
+ * kappamax = 0 give Vimag_max = 0*Inf = NaN at BACKWARD_SOLVER_MULTI.m:135, so
+ * :10-12 keeps the imaginary part).  in/out: complex64 (re, im interleaved).
+ * project_non_neg (:57-75): negative imaginary parts become 0, and when
+ * vimag_max > 0 the imaginary part is capped at it (NaN means no cap); the dual
+ * norm is sqrt(sum_k |P_k|^2) over complex components (project_TV :79-91). */
+int cbs_tv_prox_complex(cbs_handle h, const void* in, void* out, const uint32_t dims[3], double lambda, double vmin,
+                        double vmax, double vimag_max, int non_neg, int dirichlet, uint32_t inner_itt, int mem);
+
+/* Misfit gradient of BACKWARD_SOLVER_MULTI.get_gradiant_static (:50-53) from the
+ * 3D fields the forward (u) and adjoint (w) solves return through
+ * cbs_solve_2d's field_3d: grad = -i/n_rhs * sum_r conj(u_r) * flip_z(conj(w_r)),
+ * summed in FP64.  u, w: n_rhs blocks of dims[0]*dims[1]*dims[2] complex64
+ * (x fastest); grad: dims complex128.  The reference forms it from gpuArrays in
+ * MATLAB; this keeps both fields in device memory. */
+int cbs_field_gradient(cbs_handle h, const void* u, const void* w, uint32_t n_rhs, const uint32_t dims[3], void* grad,
+                       int mem);
+
 /* FP64 per-iteration |f_n|^2 of the last solve (track_norms=1): out[r * n_iter + j]. */
 int cbs_get_norms(cbs_handle h, double* out, uint32_t n_rhs, uint32_t n_iter);
 

@@ -43,6 +43,8 @@ class BACKWARD_SOLVER_MULTI:
         self.err_list = []
 
     def close(self):
+        self._bufs = None
+        self._nbuf = None
         self.forward.close()
         self.adjoint.close()
 
@@ -56,19 +58,39 @@ class BACKWARD_SOLVER_MULTI:
                                          CBS_MEM_HOST))
             self.adjoint._io_ready = True
 
+    def _fields(self, A):
+        """Device buffers for the forward and adjoint 3D fields of A angles (kept between calls)."""
+        from . import device as dv
+        nri = int(np.prod(self.forward.expected_RI_size))
+        if getattr(self, '_nbuf', None) != A:
+            self._bufs = None
+            self._bufs = (dv.DeviceArray((A * nri,)), dv.DeviceArray((A * nri,)),
+                          dv.DeviceArray((nri,), np.complex128))
+            self._nbuf = A
+        return self._bufs
+
     def get_gradiant_static(self, RI, input_field, output_field):
         """Codes/BACKWARD_SOLVER_MULTI.m:30-54: forward solve, residual, adjoint solve on
-        conj(flip_z(RI)) with conj(residual), gradient -i mean(conj(u) w) (FP64 reduction)."""
+        conj(flip_z(RI)) with conj(residual), gradient -i mean(conj(u) w).  Both 3D fields
+        stay in device memory; cbs_field_gradient forms the gradient in FP64 there."""
+        import ctypes as C
+        from ._lib import check, lib, CBS_MEM_DEVICE
         self._share_io()
         RI = np.asarray(RI)
+        inp = np.asarray(input_field)
+        A = inp.shape[-1] if inp.ndim >= 3 else 1
+        du, dw, dg = self._fields(A)
         self.forward.set_RI(RI)                                                   # :32
-        trans, _, u3d = self.forward.solve(input_field)                           # :34
+        trans, _ = self.forward.solve_keep_3d(inp, du)                            # :34
         diff = trans.astype(np.complex128) - np.asarray(output_field)            # :36
         err = float(np.sum(np.abs(diff) ** 2))                                    # :37
         self.adjoint.set_RI(np.conj(np.flip(RI, axis=2)))                         # :48
-        _, _, w3d = self.adjoint.solve(np.conj(diff))                             # :49
-        w3d = np.flip(np.conj(w3d), axis=2)                                       # :50
-        grad = -(1j * np.mean(np.conj(u3d.astype(np.complex128)) * w3d, axis=(3, 4)))   # :53
+        self.adjoint.solve_keep_3d(np.conj(diff), dw, return_2d=False)            # :49
+        dims = (C.c_uint32 * 3)(*self.forward.expected_RI_size)
+        check(lib.cbs_field_gradient(self.adjoint._h, C.c_void_p(du.ptr), C.c_void_p(dw.ptr), A, dims,
+                                     C.c_void_p(dg.ptr), CBS_MEM_DEVICE))         # :50-53
+        rx, ry, rz = self.forward.expected_RI_size
+        grad = dg.to_host().reshape(rz, ry, rx).transpose(2, 1, 0)
         return grad, err
 
     def get_gradiant(self, RI, input_field, output_field, rng=None):
@@ -110,9 +132,9 @@ class BACKWARD_SOLVER_MULTI:
             t1 = time.perf_counter()
             self.err_list.append(err)
             t_n = t_np
-            if Vimag_max == 0 and p.get('tv_on_device', True):
+            if p.get('tv_on_device', True):
                 s_n = tv_fista_inner_device(self.forward, u_n - (1 / alpha) * grad, p['tv_param'] / alpha, Vmin,
-                                            Vmax, p['use_non_negativity'], False, p['inner_itt'])
+                                            Vmax, Vimag_max, p['use_non_negativity'], False, p['inner_itt'])
             else:
                 s_n = tv_fista_inner(u_n - (1 / alpha) * grad, p['tv_param'] / alpha, Vmin, Vmax, Vimag_max,
                                      p['use_non_negativity'], False, p['inner_itt'])

@@ -852,43 +852,88 @@ struct Solver {
     return best;
   }
 
-  DevBuf<float> tv_in, tv_y, tv_P, tv_R;
+  DevBuf<float2> tv_in, tv_y, tv_P, tv_R;   // used as float arrays for real input
+  DevBuf<float2> gr_u, gr_w;
+  DevBuf<double2> gr_out;
 
-  void tv_prox(const float* in, float* out, const uint32_t* dims, double lam, double vmin, double vmax, int non_neg,
-               int dirichlet, uint32_t inner, int mem) {
+  // Both element types share the buffers: a real problem of n voxels uses the first
+  // n (or 3n) floats of buffers sized for n complex values.
+  void tv_prox(const void* in, void* out, const uint32_t* dims, double lam, double vmin, double vmax,
+               double vimag_max, int non_neg, int dirichlet, uint32_t inner, int mem, bool cplx) {
     require(in && out && dims, CBS_ERR_ARG, "null TV argument");
     require(dims[0] >= 1 && dims[1] >= 1 && dims[2] >= 1, CBS_ERR_ARG, "TV dims must be >= 1");
     require(lam > 0, CBS_ERR_ARG, "TV lambda must be > 0");
     require(mem == CBS_MEM_HOST || mem == CBS_MEM_DEVICE, CBS_ERR_ARG, "mem must be CBS_MEM_HOST or CBS_MEM_DEVICE");
     const long long n = (long long)dims[0] * dims[1] * dims[2];
+    const size_t esz = cplx ? sizeof(float2) : sizeof(float);
+    const long long nf = cplx ? 2 * n : n;   // floats per volume
     tv_y.ensure((size_t)n);
     tv_P.ensure((size_t)(3 * n));
     tv_R.ensure((size_t)(3 * n));
-    const float* din = in;
+    const void* din = in;
     if (mem == CBS_MEM_HOST) {
       tv_in.ensure((size_t)n);
-      HIP_CHECK(hipMemcpyAsync(tv_in.p, in, (size_t)n * sizeof(float), hipMemcpyHostToDevice, stream));
+      HIP_CHECK(hipMemcpyAsync(tv_in.p, in, (size_t)n * esz, hipMemcpyHostToDevice, stream));
       din = tv_in.p;
     }
     TvArgs a;
     a.nx = (int)dims[0]; a.ny = (int)dims[1]; a.nz = (int)dims[2];
     a.lam = (float)lam; a.vmin = (float)vmin; a.vmax = (float)vmax;
+    a.vimag_max = (vimag_max > 0) ? (float)vimag_max : 0.f;   // NaN or <= 0: no cap (the reference tests > 0)
     a.non_neg = non_neg; a.dirichlet = dirichlet;
-    hipLaunchKernelGGL(tv_zero, dim3(grid_for(3 * n)), dim3(256), 0, stream, tv_P.p, 3 * n);
-    hipLaunchKernelGGL(tv_zero, dim3(grid_for(3 * n)), dim3(256), 0, stream, tv_R.p, 3 * n);
+    float* P = reinterpret_cast<float*>(tv_P.p);
+    float* R = reinterpret_cast<float*>(tv_R.p);
+    float* Y = reinterpret_cast<float*>(tv_y.p);
+    hipLaunchKernelGGL(tv_zero, dim3(grid_for(3 * nf)), dim3(256), 0, stream, P, 3 * nf);
+    hipLaunchKernelGGL(tv_zero, dim3(grid_for(3 * nf)), dim3(256), 0, stream, R, 3 * nf);
+    const unsigned g = grid_for(n);
+    auto div_project = [&](const float* Q, void* dst) {
+      if (cplx)
+        hipLaunchKernelGGL(tv_div_project_c, dim3(g), dim3(256), 0, stream, (const float2*)din, (const float2*)Q,
+                           (float2*)dst, a);
+      else
+        hipLaunchKernelGGL(tv_div_project, dim3(g), dim3(256), 0, stream, (const float*)din, Q, (float*)dst, a);
+    };
     double t = 1.0;
     for (uint32_t it = 0; it < inner; ++it) {
       const double tn = (1.0 + std::sqrt(1.0 + 4.0 * t * t)) / 2.0;
-      hipLaunchKernelGGL(tv_div_project, dim3(grid_for(n)), dim3(256), 0, stream, din, tv_R.p, tv_y.p, a);
-      hipLaunchKernelGGL(tv_dual_update, dim3(grid_for(n)), dim3(256), 0, stream, tv_y.p, tv_P.p, tv_R.p, a,
-                         (float)((t - 1.0) / tn));
+      const float theta = (float)((t - 1.0) / tn);
+      div_project(R, Y);
+      if (cplx)
+        hipLaunchKernelGGL(tv_dual_update_c, dim3(g), dim3(256), 0, stream, (const float2*)Y, (float2*)P,
+                           (float2*)R, a, theta);
+      else
+        hipLaunchKernelGGL(tv_dual_update, dim3(g), dim3(256), 0, stream, (const float*)Y, P, R, a, theta);
       t = tn;
     }
-    float* dout = mem == CBS_MEM_HOST ? tv_y.p : out;
-    hipLaunchKernelGGL(tv_div_project, dim3(grid_for(n)), dim3(256), 0, stream, din, tv_P.p, dout, a);
+    div_project(P, mem == CBS_MEM_HOST ? (void*)Y : out);
+    HIP_CHECK(hipGetLastError());
+    if (mem == CBS_MEM_HOST) HIP_CHECK(hipMemcpyAsync(out, Y, (size_t)n * esz, hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+  }
+
+  void field_grad(const void* u, const void* w, uint32_t nr, const uint32_t* dims, void* grad, int mem) {
+    require(u && w && grad && dims, CBS_ERR_ARG, "null gradient argument");
+    require(nr >= 1, CBS_ERR_ARG, "n_rhs must be >= 1");
+    require(dims[0] >= 1 && dims[1] >= 1 && dims[2] >= 1, CBS_ERR_ARG, "gradient dims must be >= 1");
+    require(mem == CBS_MEM_HOST || mem == CBS_MEM_DEVICE, CBS_ERR_ARG, "mem must be CBS_MEM_HOST or CBS_MEM_DEVICE");
+    const long long n = (long long)dims[0] * dims[1] * dims[2];
+    const float2* du = (const float2*)u;
+    const float2* dw = (const float2*)w;
+    double2* dg = (double2*)grad;
+    if (mem == CBS_MEM_HOST) {
+      gr_u.ensure((size_t)(n * nr));
+      gr_w.ensure((size_t)(n * nr));
+      gr_out.ensure((size_t)n);
+      HIP_CHECK(hipMemcpyAsync(gr_u.p, u, (size_t)(n * nr) * sizeof(float2), hipMemcpyHostToDevice, stream));
+      HIP_CHECK(hipMemcpyAsync(gr_w.p, w, (size_t)(n * nr) * sizeof(float2), hipMemcpyHostToDevice, stream));
+      du = gr_u.p; dw = gr_w.p; dg = gr_out.p;
+    }
+    hipLaunchKernelGGL(field_gradient, dim3(grid_for(n)), dim3(256), 0, stream, du, dw, (int)dims[0], (int)dims[1],
+                       (int)dims[2], (int)nr, dg);
     HIP_CHECK(hipGetLastError());
     if (mem == CBS_MEM_HOST)
-      HIP_CHECK(hipMemcpyAsync(out, tv_y.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, stream));
+      HIP_CHECK(hipMemcpyAsync(grad, gr_out.p, (size_t)n * sizeof(double2), hipMemcpyDeviceToHost, stream));
     HIP_CHECK(hipStreamSynchronize(stream));
   }
 
@@ -1061,7 +1106,25 @@ int cbs_tv_prox(cbs_handle h, const float* in, float* out, const uint32_t dims[3
   return guarded([&] {
     Solver* s = lookup(h);
     DeviceGuard dg(s->device);
-    s->tv_prox(in, out, dims, lambda, vmin, vmax, non_neg, dirichlet, inner_itt, mem);
+    s->tv_prox(in, out, dims, lambda, vmin, vmax, 0.0, non_neg, dirichlet, inner_itt, mem, false);
+  });
+}
+
+int cbs_tv_prox_complex(cbs_handle h, const void* in, void* out, const uint32_t dims[3], double lambda, double vmin,
+                        double vmax, double vimag_max, int non_neg, int dirichlet, uint32_t inner_itt, int mem) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    DeviceGuard dg(s->device);
+    s->tv_prox(in, out, dims, lambda, vmin, vmax, vimag_max, non_neg, dirichlet, inner_itt, mem, true);
+  });
+}
+
+int cbs_field_gradient(cbs_handle h, const void* u, const void* w, uint32_t n_rhs, const uint32_t dims[3], void* grad,
+                       int mem) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    DeviceGuard dg(s->device);
+    s->field_grad(u, w, n_rhs, dims, grad, mem);
   });
 }
 

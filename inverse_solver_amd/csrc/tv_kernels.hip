@@ -1,11 +1,16 @@
-// Isotropic 3D TV proximal step (FISTA on the dual), real FP32, on the device.
-// Codes/sub_code/TV_FISTA_inner_v2.m:1-123 for a real 3D input (kappamax = 0):
-//   y   = clamp(in - lam * div(R))                     (:38, project_non_neg + TV_L)
+// Isotropic 3D TV proximal step (FISTA on the dual) on the device, FP32, for a
+// real potential (kappamax = 0) or a complex one.
+// Codes/sub_code/TV_FISTA_inner_v2.m:1-123 for 3D input:
+//   y   = project(in - lam * div(R))                   (:38, project_non_neg + TV_L)
 //   P'  = R + grad(y) / (12 lam);  P' /= max(|P'|, 1)  (:38-42, TV_L_trans, project_TV)
 //   R   = P' + (t - 1)/t' (P' - P)                     (:44-47)
-//   out = clamp(in - lam * div(P))                     (:51)
+//   out = project(in - lam * div(P))                   (:51)
 // with periodic forward differences grad_k(y) = y - y(+e_k) and
-// div(P) = sum_k P_k - P_k(-e_k).
+// div(P) = sum_k P_k - P_k(-e_k).  |P'| is the isotropic norm over the three
+// components, each |.|^2 of a complex value for complex input (project_TV :79-91).
+// project (project_non_neg :57-75): with non_neg, real part clamped to [vmin, vmax];
+// for complex input a negative imaginary part is set to 0 and, when vimag_max > 0,
+// the imaginary part is capped at vimag_max; dirichlet zeroes the six faces.
 #include "tv_kernels.hpp"
 
 namespace cbs {
@@ -13,15 +18,29 @@ namespace cbs {
 #define GRID_STRIDE_TV(i, total) \
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (total); i += (long long)gridDim.x * blockDim.x)
 
-__device__ __forceinline__ float tv_project(float v, TvArgs a, int x, int y, int z) {
+__device__ __forceinline__ float tv_abs2(float v) { return v * v; }
+__device__ __forceinline__ float tv_abs2(float2 v) { return v.x * v.x + v.y * v.y; }
+__device__ __forceinline__ float2 operator+(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 operator-(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 operator*(float s, float2 a) { return make_float2(s * a.x, s * a.y); }
+__device__ __forceinline__ float2 operator/(float2 a, float s) { return make_float2(a.x / s, a.y / s); }
+__device__ __forceinline__ void tv_zero_val(float& v) { v = 0.f; }
+__device__ __forceinline__ void tv_zero_val(float2& v) { v = make_float2(0.f, 0.f); }
+
+__device__ __forceinline__ float tv_bounds(float v, const TvArgs& a) {
   if (a.non_neg) v = fminf(fmaxf(v, a.vmin), a.vmax);
-  if (a.dirichlet && (x == 0 || y == 0 || z == 0 || x == a.nx - 1 || y == a.ny - 1 || z == a.nz - 1)) v = 0.f;
+  return v;
+}
+__device__ __forceinline__ float2 tv_bounds(float2 v, const TvArgs& a) {
+  if (a.non_neg) v.x = fminf(fmaxf(v.x, a.vmin), a.vmax);
+  if (v.y < 0.f) v.y = 0.f;
+  if (a.vimag_max > 0.f && v.y > a.vimag_max) v.y = a.vimag_max;
   return v;
 }
 
-// y = project(in - lam * div(Q)), Q = R (iterations) or P (final)
-__global__ void tv_div_project(const float* __restrict__ in, const float* __restrict__ Q, float* __restrict__ y,
-                               TvArgs a) {
+template <class T>
+__device__ __forceinline__ void tv_div_project_body(const T* __restrict__ in, const T* __restrict__ Q,
+                                                    T* __restrict__ y, const TvArgs& a) {
   const long long n = (long long)a.nx * a.ny * a.nz;
   GRID_STRIDE_TV(i, n) {
     const int x = (int)(i % a.nx);
@@ -30,14 +49,17 @@ __global__ void tv_div_project(const float* __restrict__ in, const float* __rest
     const long long ix = x == 0 ? i + (a.nx - 1) : i - 1;
     const long long iy = yy == 0 ? i + (long long)(a.ny - 1) * a.nx : i - a.nx;
     const long long iz = z == 0 ? i + (long long)(a.nz - 1) * a.nx * a.ny : i - (long long)a.nx * a.ny;
-    const float div = Q[i] + Q[n + i] + Q[2 * n + i] - Q[ix] - Q[n + iy] - Q[2 * n + iz];
-    y[i] = tv_project(in[i] - a.lam * div, a, x, yy, z);
+    const T div = ((Q[i] + Q[n + i]) + Q[2 * n + i]) - Q[ix] - Q[n + iy] - Q[2 * n + iz];
+    T v = tv_bounds(in[i] - a.lam * div, a);
+    if (a.dirichlet && (x == 0 || yy == 0 || z == 0 || x == a.nx - 1 || yy == a.ny - 1 || z == a.nz - 1))
+      tv_zero_val(v);
+    y[i] = v;
   }
 }
 
-// P' = proj(R + grad(y)/(12 lam)); R = P' + theta (P' - P); P = P'
-__global__ void tv_dual_update(const float* __restrict__ y, float* __restrict__ P, float* __restrict__ R, TvArgs a,
-                               float theta) {
+template <class T>
+__device__ __forceinline__ void tv_dual_update_body(const T* __restrict__ y, T* __restrict__ P, T* __restrict__ R,
+                                                    const TvArgs& a, float theta) {
   const long long n = (long long)a.nx * a.ny * a.nz;
   const float c = 1.0f / (12.0f * a.lam);
   GRID_STRIDE_TV(i, n) {
@@ -47,15 +69,15 @@ __global__ void tv_dual_update(const float* __restrict__ y, float* __restrict__ 
     const long long jx = x == a.nx - 1 ? i - (a.nx - 1) : i + 1;
     const long long jy = yy == a.ny - 1 ? i - (long long)(a.ny - 1) * a.nx : i + a.nx;
     const long long jz = z == a.nz - 1 ? i - (long long)(a.nz - 1) * a.nx * a.ny : i + (long long)a.nx * a.ny;
-    const float yi = y[i];
-    float p0 = R[i] + c * (yi - y[jx]);
-    float p1 = R[n + i] + c * (yi - y[jy]);
-    float p2 = R[2 * n + i] + c * (yi - y[jz]);
-    const float nrm = sqrtf(fmaxf(p0 * p0 + p1 * p1 + p2 * p2, 1.0f));
-    p0 /= nrm;
-    p1 /= nrm;
-    p2 /= nrm;
-    const float o0 = P[i], o1 = P[n + i], o2 = P[2 * n + i];
+    const T yi = y[i];
+    T p0 = R[i] + c * (yi - y[jx]);
+    T p1 = R[n + i] + c * (yi - y[jy]);
+    T p2 = R[2 * n + i] + c * (yi - y[jz]);
+    const float nrm = sqrtf(fmaxf(tv_abs2(p0) + tv_abs2(p1) + tv_abs2(p2), 1.0f));
+    p0 = p0 / nrm;
+    p1 = p1 / nrm;
+    p2 = p2 / nrm;
+    const T o0 = P[i], o1 = P[n + i], o2 = P[2 * n + i];
     P[i] = p0;
     P[n + i] = p1;
     P[2 * n + i] = p2;
@@ -65,8 +87,47 @@ __global__ void tv_dual_update(const float* __restrict__ y, float* __restrict__ 
   }
 }
 
+__global__ void tv_div_project(const float* __restrict__ in, const float* __restrict__ Q, float* __restrict__ y,
+                               TvArgs a) {
+  tv_div_project_body(in, Q, y, a);
+}
+__global__ void tv_div_project_c(const float2* __restrict__ in, const float2* __restrict__ Q, float2* __restrict__ y,
+                                 TvArgs a) {
+  tv_div_project_body(in, Q, y, a);
+}
+__global__ void tv_dual_update(const float* __restrict__ y, float* __restrict__ P, float* __restrict__ R, TvArgs a,
+                               float theta) {
+  tv_dual_update_body(y, P, R, a, theta);
+}
+__global__ void tv_dual_update_c(const float2* __restrict__ y, float2* __restrict__ P, float2* __restrict__ R,
+                                 TvArgs a, float theta) {
+  tv_dual_update_body(y, P, R, a, theta);
+}
+
 __global__ void tv_zero(float* __restrict__ buf, long long n) {
   GRID_STRIDE_TV(i, n) buf[i] = 0.f;
+}
+
+// grad[i] = -(i / n_rhs) * sum_r conj(u_r[i]) * conj(w_r[flip_z(i)]), FP64 sums.
+// Codes/BACKWARD_SOLVER_MULTI.m:50-53 with w = flip(conj(w_adjoint), 3).
+__global__ void field_gradient(const float2* __restrict__ u, const float2* __restrict__ w, int nx, int ny, int nz,
+                               int n_rhs, double2* __restrict__ grad) {
+  const long long nxy = (long long)nx * ny;
+  const long long n = nxy * nz;
+  GRID_STRIDE_TV(i, n) {
+    const long long z = i / nxy;
+    const long long j = i + (nz - 1 - 2 * z) * nxy;   // same (x, y), z -> nz - 1 - z
+    double sr = 0.0, si = 0.0;
+    for (int r = 0; r < n_rhs; ++r) {
+      const float2 a = u[(long long)r * n + i];
+      const float2 b = w[(long long)r * n + j];
+      // conj(a) * conj(b) = (a.x b.x - a.y b.y) - i (a.x b.y + a.y b.x)
+      sr += (double)a.x * b.x - (double)a.y * b.y;
+      si -= (double)a.x * b.y + (double)a.y * b.x;
+    }
+    const double inv = 1.0 / n_rhs;
+    grad[i] = make_double2(si * inv, -sr * inv);   // -i (sr + i si) / n_rhs
+  }
 }
 
 }  // namespace cbs

@@ -214,6 +214,18 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
         input_field: (x, y, 1, A) (or (x, y, A) / (x, y)) on the field grid.
         Returns (fields_trans (x, y, 1, A) or None, fields_ref (x, y, 1, A) or None,
         fields_3D (x, y, z, 1, A) or None) as complex64, following return_* flags."""
+        trans, ref, f3 = self._solve(input_field, None)
+        return trans, ref, (None if f3 is None else f3[:, :, :, None, :])
+
+    def solve_keep_3d(self, input_field, field_3d, return_2d=True):
+        """solve() with the 3D fields left in device memory: field_3d is a
+        device.DeviceArray of n_rhs * RI-grid complex64 (x fastest, angle slowest),
+        e.g. for cbs_field_gradient.  Returns (fields_trans, fields_ref), both None
+        when return_2d is False (only the 3D fields are formed)."""
+        trans, ref, _ = self._solve(input_field, field_3d, return_2d)
+        return trans, ref
+
+    def _solve(self, input_field, field_3d_dev, return_2d=True):
         p = self.parameters
         inp = np.asarray(input_field)
         if inp.ndim == 2:
@@ -232,23 +244,37 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
         src = np.asfortranarray(centered_ifft2(U)[:, :, 0, :].astype(np.complex64))    # _CUDA.m:122
         fx, fy = int(p['size'][0]), int(p['size'][1])
         rx, ry, rz = self.expected_RI_size
-        want_t, want_r, want_3 = bool(p['return_transmission']), bool(p['return_reflection']), bool(p['return_3D'])
-        t2 = np.empty((fx, fy, A), np.complex64, order='F') if want_t else None
-        r2 = np.empty((fx, fy, A), np.complex64, order='F') if want_r else None
-        f3 = np.empty((rx, ry, rz, A), np.complex64, order='F') if want_3 else None
+        want_t = bool(p['return_transmission']) and return_2d
+        want_r = bool(p['return_reflection']) and return_2d
+        want_3 = bool(p['return_3D']) or field_3d_dev is not None
         nul = C.c_void_p(None)
-        check(lib.cbs_solve_2d(self._h, _fptr(src), A, _fptr(t2) if want_t else nul, _fptr(r2) if want_r else nul,
-                               _fptr(f3) if want_3 else nul, _lib.CBS_MEM_HOST))
-        trans = ref = field3d = None
+        if field_3d_dev is None:
+            t2 = np.empty((fx, fy, A), np.complex64, order='F') if want_t else None
+            r2 = np.empty((fx, fy, A), np.complex64, order='F') if want_r else None
+            f3 = np.empty((rx, ry, rz, A), np.complex64, order='F') if want_3 else None
+            check(lib.cbs_solve_2d(self._h, _fptr(src), A, _fptr(t2) if want_t else nul,
+                                   _fptr(r2) if want_r else nul, _fptr(f3) if want_3 else nul, _lib.CBS_MEM_HOST))
+        else:
+            from . import device as dv
+            if field_3d_dev.nbytes != 8 * rx * ry * rz * A or field_3d_dev.dtype != np.complex64:
+                raise ValueError('field_3d must hold %d x %s complex64 values' % (A, [rx, ry, rz]))
+            f3 = None
+            d_src = dv.DeviceArray.from_host(np.ascontiguousarray(src.T))
+            d_t = dv.DeviceArray((A, fy, fx)) if want_t else None
+            d_r = dv.DeviceArray((A, fy, fx)) if want_r else None
+            check(lib.cbs_solve_2d(self._h, C.c_void_p(d_src.ptr), A, C.c_void_p(d_t.ptr) if want_t else nul,
+                                   C.c_void_p(d_r.ptr) if want_r else nul, C.c_void_p(field_3d_dev.ptr),
+                                   _lib.CBS_MEM_DEVICE))
+            t2 = d_t.to_host().T if want_t else None
+            r2 = d_r.to_host().T if want_r else None
+        trans = ref = None
         if want_t:                                                                      # :481-492
             T = centered_fft2(t2.astype(np.complex128)) + U[:, :, 0, :]
             trans = centered_ifft2(self.transform_field_2D(T)).astype(np.complex64)[:, :, None, :]
         if want_r:                                                                      # :505-508
             R = centered_fft2(r2.astype(np.complex128))
             ref = centered_ifft2(self.transform_field_2D_reflection(R)).astype(np.complex64)[:, :, None, :]
-        if want_3:
-            field3d = f3[:, :, :, None, :]
-        return trans, ref, field3d
+        return trans, ref, f3
 
     # ---------------------------------------------------------------- Born loop from a 3D incident field
     def solve_incident(self, u_in):

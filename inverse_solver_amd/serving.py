@@ -240,6 +240,16 @@ class SolverCache:
             s._io_ready = True
         return k, s
 
+    def prewarm(self, params_list, log=None):
+        """Build the solver and its 2D projection kernels for each configuration ahead
+        of the request stream (one idle solver per configuration is kept)."""
+        for p in params_list:
+            t0 = time.perf_counter()
+            k, s = self._get(p)
+            self._put(k, s)
+            if log:
+                log('prewarmed %s in %.1f s' % (list(p['size']), time.perf_counter() - t0))
+
     def _put(self, k, s):
         with self._lock:
             self._free[k].append(s)

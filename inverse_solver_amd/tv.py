@@ -75,16 +75,24 @@ def tv_fista_inner(in_mat, lam, Vmin, Vmax, Vimag_max, non_neg, dirichlet, inner
     return project_bounds(in_mat - lam * _div(P, div), Vmin, Vmax, Vimag_max, non_neg, dirichlet)
 
 
-def tv_fista_inner_device(solver, in_mat, lam, Vmin, Vmax, non_neg, dirichlet, inner_itt):
-    """The same proximal step on the GPU of `solver` (a FORWARD_SOLVER_CONVERGENT_BORN),
-    real FP32 as the reference computes it (cbs_tv_prox); for real potentials only."""
+def tv_fista_inner_device(solver, in_mat, lam, Vmin, Vmax, Vimag_max, non_neg, dirichlet, inner_itt):
+    """The same proximal step on the GPU of `solver` (a FORWARD_SOLVER_CONVERGENT_BORN), in
+    single precision as the reference computes it: real when Vimag_max == 0 (cbs_tv_prox),
+    complex otherwise (cbs_tv_prox_complex; NaN, the value the defaults nmax = Inf and
+    kappamax = 0 give, keeps the imaginary part with no upper cap)."""
     import ctypes as C
     from ._lib import check, lib, CBS_MEM_HOST
-    x = np.asfortranarray(np.real(in_mat).astype(np.float32))
+    dims = (C.c_uint32 * 3)(*np.shape(in_mat))
+    if Vimag_max == 0:
+        x = np.asfortranarray(np.real(in_mat).astype(np.float32))
+        out = np.empty_like(x, order='F')
+        check(lib.cbs_tv_prox(solver._h, x.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), dims,
+                              float(lam), float(Vmin), float(Vmax), int(bool(non_neg)), int(bool(dirichlet)),
+                              int(inner_itt), CBS_MEM_HOST))
+        return out.astype(np.float64)
+    x = np.asfortranarray(np.asarray(in_mat).astype(np.complex64))
     out = np.empty_like(x, order='F')
-    dims = (C.c_uint32 * 3)(*x.shape)
-    lo = float(Vmin) if np.isfinite(Vmin) else -3.0e38
-    hi = float(Vmax) if np.isfinite(Vmax) else 3.0e38
-    check(lib.cbs_tv_prox(solver._h, x.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), dims, float(lam),
-                          lo, hi, int(bool(non_neg)), int(bool(dirichlet)), int(inner_itt), CBS_MEM_HOST))
-    return out.astype(np.float64)
+    check(lib.cbs_tv_prox_complex(solver._h, x.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), dims,
+                                  float(lam), float(Vmin), float(Vmax), float(Vimag_max), int(bool(non_neg)),
+                                  int(bool(dirichlet)), int(inner_itt), CBS_MEM_HOST))
+    return out.astype(np.complex128)

@@ -139,10 +139,48 @@ def test_gpu_tv_prox_matches_oracle():
     s = FORWARD_SOLVER_CONVERGENT_BORN(params(ri, iterations=4))
     rng = np.random.default_rng(4)
     x = rng.standard_normal((20, 18, 17)) * 0.05
-    for args in [(0.02, -np.inf, np.inf, False), (0.01, -0.03, 0.04, True)]:
-        lam, lo, hi, nn = args
-        got = tv_fista_inner_device(s, x, lam, lo, hi, nn, False, 60)
-        ref = tv_fista_inner_v2(x, lam, lo, hi, 0, nn, False, 60)
-        print('TV device vs oracle', rel(got, ref))
+    xc = x + 1j * rng.standard_normal(x.shape) * 0.02
+    nan = 2 * 0 * np.inf          # Vimag_max of the defaults nmax = Inf, kappamax = 0
+    cases = [(x, 0.02, -np.inf, np.inf, 0, False), (x, 0.01, -0.03, 0.04, 0, True),
+             (xc, 0.02, -np.inf, np.inf, nan, False), (xc, 0.01, -0.03, 0.04, 0.01, True),
+             (xc, 0.02, -np.inf, np.inf, 0, False)]
+    for (inp, lam, lo, hi, vim, nn) in cases:
+        got = tv_fista_inner_device(s, inp, lam, lo, hi, vim, nn, False, 60)
+        ref = tv_fista_inner_v2(inp, lam, lo, hi, vim, nn, False, 60)
+        print('TV device vs oracle (complex=%s, Vimag_max=%s): %.2e' % (np.iscomplexobj(ref), vim, rel(got, ref)))
+        assert np.iscomplexobj(got) == np.iscomplexobj(ref)
         assert rel(got, ref) < 2e-5
+        if np.iscomplexobj(got):
+            assert np.all(got.imag >= 0)
+            if vim > 0:
+                assert np.all(got.imag <= vim * (1 + 1e-6))
+    s.close()
+
+
+@pytest.mark.gpu
+def test_gpu_field_gradient_kernel():
+    """cbs_field_gradient against -i mean(conj(u) flip_z(conj(w))) (BACKWARD_SOLVER_MULTI.m:50-53),
+    host and device memory, odd sizes."""
+    import ctypes as C
+    from inverse_solver_amd import FORWARD_SOLVER_CONVERGENT_BORN, device as dv
+    from inverse_solver_amd._lib import check, lib, CBS_MEM_HOST, CBS_MEM_DEVICE
+    s = FORWARD_SOLVER_CONVERGENT_BORN(params((16, 16, 17), iterations=4))
+    rng = np.random.default_rng(9)
+    for dims, A in [((7, 5, 3), 1), ((20, 18, 17), 5)]:
+        u = (rng.standard_normal(dims + (A,)) + 1j * rng.standard_normal(dims + (A,))).astype(np.complex64)
+        w = (rng.standard_normal(dims + (A,)) + 1j * rng.standard_normal(dims + (A,))).astype(np.complex64)
+        ref = -(1j * np.mean(np.conj(u.astype(np.complex128)) * np.flip(np.conj(w.astype(np.complex128)), axis=2),
+                             axis=3))
+        uf, wf = np.asfortranarray(u), np.asfortranarray(w)
+        cd = (C.c_uint32 * 3)(*dims)
+        g = np.empty(dims, np.complex128, order='F')
+        check(lib.cbs_field_gradient(s._h, uf.ctypes.data_as(C.c_void_p), wf.ctypes.data_as(C.c_void_p), A, cd,
+                                     g.ctypes.data_as(C.c_void_p), CBS_MEM_HOST))
+        assert rel(g, ref) < 1e-12
+        du = dv.DeviceArray.from_host(np.ascontiguousarray(uf.T))
+        dw = dv.DeviceArray.from_host(np.ascontiguousarray(wf.T))
+        dg = dv.DeviceArray(dims[::-1], np.complex128)
+        check(lib.cbs_field_gradient(s._h, C.c_void_p(du.ptr), C.c_void_p(dw.ptr), A, cd, C.c_void_p(dg.ptr),
+                                     CBS_MEM_DEVICE))
+        assert rel(dg.to_host().T, ref) < 1e-12
     s.close()
