@@ -54,6 +54,10 @@ typedef struct {
   uint32_t conv_size[3];       /* xy convolution grid (field + RI); [2] ignored */
   int32_t device;              /* HIP device ordinal */
   int32_t RI_center[2];        /* xy offset of the RI grid in the field (RI_center, :40-41) */
+  int32_t pole_num;            /* field components: 1 scalar, 3 vector with the dyadic Green's
+                                  function (FORWARD_SOLVER_CONVERGENT_BORN.m:294-297; 0 means 1).
+                                  With 3, every per-rhs array holds the x, y, z components in turn:
+                                  [rhs][component][z][y][x]. */
 } cbs_geometry;
 
 /* Scalars set with the potential.
@@ -85,6 +89,8 @@ typedef struct {
   int32_t time_kernels; /* 1: time each pass kind with HIP events (cbs_get_kernel_times) */
   int32_t threads_real;   /* fused form: force threads per block of the real-space pass (0 = planner) */
   int32_t threads_kspace; /* fused form: force threads per block of the k-space pass (0 = planner) */
+  int32_t block_order;    /* fused form: bit 0 real-space pass, bit 1 k-space pass; 0 = rhs-fastest
+                             block numbering, 1 = rows of one rhs together (cbs_autotune picks) */
 } cbs_options;
 
 /* Creates a solver on geo->device.  Replaces CudaBorn action 0 (main.cpp:57-101). */

@@ -32,7 +32,8 @@ class CbsError(RuntimeError):
 
 class Geometry(C.Structure):
     _fields_ = [('simulation_size', C.c_uint32 * 3), ('RI_size', C.c_uint32 * 3), ('field_size', C.c_uint32 * 3),
-                ('conv_size', C.c_uint32 * 3), ('device', C.c_int32), ('RI_center', C.c_int32 * 2)]
+                ('conv_size', C.c_uint32 * 3), ('device', C.c_int32), ('RI_center', C.c_int32 * 2),
+                ('pole_num', C.c_int32)]
 
 
 class PotentialParams(C.Structure):
@@ -45,7 +46,7 @@ class PotentialParams(C.Structure):
 class Options(C.Structure):
     _fields_ = [('form', C.c_int32), ('split_a', C.c_int32), ('tile_t', C.c_int32), ('max_batch', C.c_int32),
                 ('track_norms', C.c_int32), ('use_graph', C.c_int32), ('time_kernels', C.c_int32),
-                ('threads_real', C.c_int32), ('threads_kspace', C.c_int32)]
+                ('threads_real', C.c_int32), ('threads_kspace', C.c_int32), ('block_order', C.c_int32)]
 
 
 def _load():

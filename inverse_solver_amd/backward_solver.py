@@ -59,13 +59,14 @@ class BACKWARD_SOLVER_MULTI:
             self.adjoint._io_ready = True
 
     def _fields(self, A):
-        """Device buffers for the forward and adjoint 3D fields of A angles (kept between calls)."""
+        """Device buffers for the forward and adjoint 3D fields of A angles (all field
+        components; kept between calls)."""
         from . import device as dv
-        nri = int(np.prod(self.forward.expected_RI_size))
+        n = int(np.prod(self.forward.expected_RI_size)) * self.forward.pole_num * A
         if getattr(self, '_nbuf', None) != A:
             self._bufs = None
-            self._bufs = (dv.DeviceArray((A * nri,)), dv.DeviceArray((A * nri,)),
-                          dv.DeviceArray((nri,), np.complex128))
+            self._bufs = (dv.DeviceArray((n,)), dv.DeviceArray((n,)),
+                          dv.DeviceArray((int(np.prod(self.forward.expected_RI_size)),), np.complex128))
             self._nbuf = A
         return self._bufs
 
@@ -77,8 +78,8 @@ class BACKWARD_SOLVER_MULTI:
         from ._lib import check, lib, CBS_MEM_DEVICE
         self._share_io()
         RI = np.asarray(RI)
-        inp = np.asarray(input_field)
-        A = inp.shape[-1] if inp.ndim >= 3 else 1
+        inp = self.forward._inputs(input_field)
+        A = inp.shape[3]
         du, dw, dg = self._fields(A)
         self.forward.set_RI(RI)                                                   # :32
         trans, _ = self.forward.solve_keep_3d(inp, du)                            # :34
@@ -87,8 +88,9 @@ class BACKWARD_SOLVER_MULTI:
         self.adjoint.set_RI(np.conj(np.flip(RI, axis=2)))                         # :48
         self.adjoint.solve_keep_3d(np.conj(diff), dw, return_2d=False)            # :49
         dims = (C.c_uint32 * 3)(*self.forward.expected_RI_size)
-        check(lib.cbs_field_gradient(self.adjoint._h, C.c_void_p(du.ptr), C.c_void_p(dw.ptr), A, dims,
-                                     C.c_void_p(dg.ptr), CBS_MEM_DEVICE))         # :50-53
+        # mean over polarisation components and angles (:53 averages dims 4 and 5)
+        check(lib.cbs_field_gradient(self.adjoint._h, C.c_void_p(du.ptr), C.c_void_p(dw.ptr),
+                                     A * self.forward.pole_num, dims, C.c_void_p(dg.ptr), CBS_MEM_DEVICE))   # :50-53
         rx, ry, rz = self.forward.expected_RI_size
         grad = dg.to_host().reshape(rz, ry, rx).transpose(2, 1, 0)
         return grad, err

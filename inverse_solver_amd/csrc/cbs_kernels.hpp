@@ -48,6 +48,9 @@ struct Tables {
   const float* green_x;   // [2][nx]  4 pi^2 (k + sh dk/4)^2 at x position p (index 0: sigma=+1, 1: flipped)
   const float* green_y;   // [2][b][a] at (k2, p1)
   const float* green_z;   // [2][nz]  at z position p
+  const float* rad_x;     // vector runs: (k + sh dk/4) / k0 per axis, same layout as green_* (else null)
+  const float* rad_y;
+  const float* rad_z;
 };
 
 struct Consts {
@@ -58,6 +61,7 @@ struct Consts {
   float eps;
   float km2;
   float inv_n;
+  float c_re, c_im;   // vector runs: k^2 / (k^2 + i eps), the dyadic absorption correction
 };
 
 struct StateArgs {

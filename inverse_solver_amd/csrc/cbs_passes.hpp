@@ -19,6 +19,8 @@ struct PassRArgs {
   Consts c;
   StateArgs st;
   int upd, fwd, jj_inv, jj_fwd;
+  int order;   // block order: 0 = rhs fastest (V rows reused by consecutive blocks),
+               // 1 = y2 fastest, then rhs (the b blocks of one (rhs, z) slab run together)
 };
 
 template <int NX, int A, int NT>
@@ -41,10 +43,18 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
   const Tables& tb = args.tb;
   const int tid = threadIdx.x;
   const int bid = blockIdx.x;
-  const int r = bid % c.nrhs;
-  const int tile = bid / c.nrhs;
-  const int y2 = tile % c.b;
-  const int z = tile / c.b;
+  int r, y2, z;
+  if (args.order == 0) {
+    r = bid % c.nrhs;
+    const int tile = bid / c.nrhs;
+    y2 = tile % c.b;
+    z = tile / c.b;
+  } else {
+    y2 = bid % c.b;
+    const int t = bid / c.b;
+    r = t % c.nrhs;
+    z = t / c.nrhs;
+  }
   const long long roff = (long long)r * args.st.rhs_stride;
   float4* __restrict__ spec4 = reinterpret_cast<float4*>(args.st.spec + roff + (long long)(z * c.b + y2) * (A * NX));
   const int upd = args.upd, fwd = args.fwd;
@@ -181,43 +191,63 @@ struct PassKArgs {
   const uint8_t* parity;
   long long rhs_stride;
   int jj;
+  int order;   // block order: 0 = rhs fastest, then a-index, then x tile;
+               // 1 = x tile fastest, then a-index, then rhs (whole rows read together)
 };
 
-template <int B, int NZ, int T, int NT>
+// P = 1: scalar Green's function.  P = 3: vector run; the block holds the three
+// field components of one rhs (stored as consecutive rhs slots 3r, 3r+1, 3r+2)
+// and applies the dyadic Green's function G (I - c r r^T) of
+// Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:607-615 at every frequency.
+template <int B, int NZ, int T, int NT, int P>
 __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
   static_assert(T % 2 == 0, "x tile must hold pairs");
-  constexpr int LW = B * T;                   // lines of the z-FFT
+  static_assert(P == 1 || P == 3, "one or three field components");
+  constexpr int LW = B * T;                   // lines of the z-FFT per component
   constexpr int TP = T / 2;                   // pairs per z row
   constexpr int NTASK = NZ * TP;              // (z, pair) tasks
   constexpr int L = (NTASK + NT - 1) / NT;
-  // the tile alone fills the LDS budget (two blocks per CU at 80 KiB), so the
-  // small per-axis tables are read through the L1 cache instead
-  __shared__ float2 lds[NZ * LW];
+  constexpr int PLW = P * LW;                 // LDS row width (all components)
+  // the tile alone fills the LDS budget, so the small per-axis tables are read
+  // through the L1 cache instead
+  __shared__ float2 lds[NZ * PLW];
   const Consts& c = args.c;
   const Tables& tb = args.tb;
   const int tid = threadIdx.x;
   const int bid = blockIdx.x;
-  const int r = bid % c.nrhs;
-  const int t2 = bid / c.nrhs;
-  const int p1 = t2 % c.a;
-  const int xt = t2 / c.a;
+  int r, p1, xt;
+  if (args.order == 0) {
+    r = bid % c.nrhs;
+    const int t2 = bid / c.nrhs;
+    p1 = t2 % c.a;
+    xt = t2 / c.a;
+  } else {
+    const int ntile = (c.nx + T - 1) / T;
+    xt = bid % ntile;
+    const int t2 = bid / ntile;
+    p1 = t2 % c.a;
+    r = t2 / c.a;
+  }
   const int x0 = xt * T;
   const long long rowstride = (long long)c.a * c.nx;     // between consecutive y2
-  float2* __restrict__ spec = args.spec + (long long)r * args.rhs_stride + (long long)p1 * c.nx + x0;
-  const int flip = flip_of(args.jj, args.parity, r);
+  float2* __restrict__ spec = args.spec + (long long)(P * r) * args.rhs_stride + (long long)p1 * c.nx + x0;
+  const int flip = flip_of(args.jj, args.parity, P * r);
   const int tvalid = c.nx - x0 < T ? c.nx - x0 : T;      // even: nx and x0 are even
 
-  float4 reg[L][B];
+  float4 reg[L][P][B];
 #pragma unroll
   for (int k = 0; k < L; ++k) {
     const int task = tid + k * NT;
     if (task < NTASK) {
       const int tp = task % TP, z = task / TP;
-      const float2* src = spec + (long long)z * B * rowstride + 2 * tp;
 #pragma unroll
-      for (int y2 = 0; y2 < B; ++y2)
-        reg[k][y2] = (2 * tp < tvalid) ? *reinterpret_cast<const float4*>(src + y2 * rowstride)
-                                       : make_float4(0.f, 0.f, 0.f, 0.f);
+      for (int q = 0; q < P; ++q) {
+        const float2* src = spec + (long long)q * args.rhs_stride + (long long)z * B * rowstride + 2 * tp;
+#pragma unroll
+        for (int y2 = 0; y2 < B; ++y2)
+          reg[k][q][y2] = (2 * tp < tvalid) ? *reinterpret_cast<const float4*>(src + y2 * rowstride)
+                                            : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
     }
   }
 #pragma unroll
@@ -225,23 +255,26 @@ __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
     const int task = tid + k * NT;
     if (task < NTASK) {
       const int tp = task % TP, z = task / TP;
-      float2 u0[B], u1[B];
 #pragma unroll
-      for (int y2 = 0; y2 < B; ++y2) {
-        u0[y2] = lo2(reg[k][y2]);
-        u1[y2] = hi2(reg[k][y2]);
-      }
-      Dft<B, false>::run(u0);
-      Dft<B, false>::run(u1);
+      for (int q = 0; q < P; ++q) {
+        float2 u0[B], u1[B];
 #pragma unroll
-      for (int k2 = 0; k2 < B; ++k2) {
-        lds[z * LW + k2 * T + 2 * tp] = u0[k2];
-        lds[z * LW + k2 * T + 2 * tp + 1] = u1[k2];
+        for (int y2 = 0; y2 < B; ++y2) {
+          u0[y2] = lo2(reg[k][q][y2]);
+          u1[y2] = hi2(reg[k][q][y2]);
+        }
+        Dft<B, false>::run(u0);
+        Dft<B, false>::run(u1);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) {
+          lds[z * PLW + q * LW + k2 * T + 2 * tp] = u0[k2];
+          lds[z * PLW + q * LW + k2 * T + 2 * tp + 1] = u1[k2];
+        }
       }
     }
   }
   __syncthreads();
-  auto zl = [](int l, int e) { return e * LW + l; };
+  auto zl = [](int l, int e) { return e * PLW + l; };
   {
     // (1/N) / (d - i eps) = (d + i eps) / (N (d^2 + eps^2)),  d = 4 pi^2 |k_s|^2 - k_m^2,
     // applied between the forward and inverse z transforms of each line
@@ -249,13 +282,35 @@ __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
     const float* gx = tb.green_x + flip * c.nx + x0;
     const float* gy = tb.green_y + flip * c.ny + p1;
     const float km2 = c.km2, eps2 = c.eps * c.eps, inv_n = c.inv_n, eps = c.eps;
-    auto green = [&](int l, int pz, float2 v) {
-      const int k2 = l / T, t = l - k2 * T;
-      const float d = gx[t < tvalid ? t : 0] + gy[k2 * c.a] + gz[pz] - km2;
-      const float s = inv_n / fmaf(d, d, eps2);
-      return cmul(v, make_float2(d * s, eps * s));
-    };
-    line_fft_mul_ifft<NZ, LW, NT>(lds, zl, tb.tw_z, tid, green);
+    if constexpr (P == 1) {
+      auto green = [&](int l, int pz, float2 v) {
+        const int k2 = l / T, t = l - k2 * T;
+        const float d = gx[t < tvalid ? t : 0] + gy[k2 * c.a] + gz[pz] - km2;
+        const float s = inv_n / fmaf(d, d, eps2);
+        return cmul(v, make_float2(d * s, eps * s));
+      };
+      line_fft_mul_ifft<NZ, LW, NT>(lds, zl, tb.tw_z, tid, green);
+    } else {
+      const float* rz = tb.rad_z + flip * c.nz;
+      const float* rx = tb.rad_x + flip * c.nx + x0;
+      const float* ry = tb.rad_y + flip * c.ny + p1;
+      const float2 cc = make_float2(c.c_re, c.c_im);
+      // PSI_j = G (v_j - c r_j (r . v))  ==  sum_i G (delta_ji - c r_j r_i) v_i
+      auto dyadic = [&](int l, int pz, float2& v0, float2& v1, float2& v2) {
+        const int k2 = l / T, t = l - k2 * T;
+        const int tx = t < tvalid ? t : 0;
+        const float d = gx[tx] + gy[k2 * c.a] + gz[pz] - km2;
+        const float s = inv_n / fmaf(d, d, eps2);
+        const float2 g = make_float2(d * s, eps * s);
+        const float r0 = rx[tx], r1 = ry[k2 * c.a], r2 = rz[pz];
+        const float2 dot = make_float2(r0 * v0.x + r1 * v1.x + r2 * v2.x, r0 * v0.y + r1 * v1.y + r2 * v2.y);
+        const float2 cd = cmul(cc, dot);
+        v0 = cmul(g, make_float2(v0.x - r0 * cd.x, v0.y - r0 * cd.y));
+        v1 = cmul(g, make_float2(v1.x - r1 * cd.x, v1.y - r1 * cd.y));
+        v2 = cmul(g, make_float2(v2.x - r2 * cd.x, v2.y - r2 * cd.y));
+      };
+      line_fft_mul3_ifft<NZ, LW, NT>(lds, zl, tb.tw_z, tid, dyadic);
+    }
   }
 #pragma unroll
   for (int k = 0; k < L; ++k) {
@@ -263,17 +318,20 @@ __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
     if (task < NTASK) {
       const int tp = task % TP, z = task / TP;
       if (2 * tp >= tvalid) continue;
-      float2 u0[B], u1[B];
 #pragma unroll
-      for (int k2 = 0; k2 < B; ++k2) {
-        u0[k2] = lds[z * LW + k2 * T + 2 * tp];
-        u1[k2] = lds[z * LW + k2 * T + 2 * tp + 1];
+      for (int q = 0; q < P; ++q) {
+        float2 u0[B], u1[B];
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) {
+          u0[k2] = lds[z * PLW + q * LW + k2 * T + 2 * tp];
+          u1[k2] = lds[z * PLW + q * LW + k2 * T + 2 * tp + 1];
+        }
+        Dft<B, true>::run(u0);
+        Dft<B, true>::run(u1);
+        float2* dst = spec + (long long)q * args.rhs_stride + (long long)z * B * rowstride + 2 * tp;
+#pragma unroll
+        for (int y2 = 0; y2 < B; ++y2) *reinterpret_cast<float4*>(dst + y2 * rowstride) = pack2(u0[y2], u1[y2]);
       }
-      Dft<B, true>::run(u0);
-      Dft<B, true>::run(u1);
-      float2* dst = spec + (long long)z * B * rowstride + 2 * tp;
-#pragma unroll
-      for (int y2 = 0; y2 < B; ++y2) *reinterpret_cast<float4*>(dst + y2 * rowstride) = pack2(u0[y2], u1[y2]);
     }
   }
 }

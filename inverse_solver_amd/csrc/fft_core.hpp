@@ -351,4 +351,92 @@ __device__ __forceinline__ void line_fft_mul_ifft(float2* __restrict__ lds, cons
   }
 }
 
+// Three-component variant for the dyadic Green's multiply: NL lines per
+// component, component c's line l at index c * NL + l of addr().  The forward
+// step A and inverse step A run over all 3 NL lines; step B, the multiply and
+// the inverse step B are fused per (l, k1) so that mul3(l, p, v0, v1, v2) sees
+// the three components of one frequency together (in place on v0..v2).
+template <int N, int NL, int NT, class Addr, class Mul3>
+__device__ __forceinline__ void line_fft_mul3_ifft(float2* __restrict__ lds, const Addr& addr,
+                                                   const float2* __restrict__ tw, int tid, const Mul3& mul3) {
+  using P = LineFft<N>;
+  constexpr int R1 = P::R1, R2 = P::R2;
+  if constexpr (R2 == 1) {
+    for (int l = tid; l < NL; l += NT) {
+      float2 v0[N], v1[N], v2[N];
+#pragma unroll
+      for (int e = 0; e < N; ++e) {
+        v0[e] = lds[addr(l, e)];
+        v1[e] = lds[addr(NL + l, e)];
+        v2[e] = lds[addr(2 * NL + l, e)];
+      }
+      Dft<N, false>::run(v0);
+      Dft<N, false>::run(v1);
+      Dft<N, false>::run(v2);
+#pragma unroll
+      for (int e = 0; e < N; ++e) mul3(l, e, v0[e], v1[e], v2[e]);
+      Dft<N, true>::run(v0);
+      Dft<N, true>::run(v1);
+      Dft<N, true>::run(v2);
+#pragma unroll
+      for (int e = 0; e < N; ++e) {
+        lds[addr(l, e)] = v0[e];
+        lds[addr(NL + l, e)] = v1[e];
+        lds[addr(2 * NL + l, e)] = v2[e];
+      }
+    }
+    __syncthreads();
+  } else {
+    for (int t = tid; t < 3 * NL * R2; t += NT) {       // forward step A, all components
+      const int l = t % (3 * NL), n2 = t / (3 * NL);
+      float2 v[R1];
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) v[n1] = lds[addr(l, R2 * n1 + n2)];
+      Dft<R1, false>::run(v);
+#pragma unroll
+      for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], tw[n2 * k1]);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) lds[addr(l, R2 * k1 + n2)] = v[k1];
+    }
+    __syncthreads();
+    for (int t = tid; t < NL * R1; t += NT) {           // step B, 3x3 multiply, inverse step B
+      const int l = t % NL, k1 = t / NL;
+      float2 v0[R2], v1[R2], v2[R2];
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) {
+        v0[n2] = lds[addr(l, R2 * k1 + n2)];
+        v1[n2] = lds[addr(NL + l, R2 * k1 + n2)];
+        v2[n2] = lds[addr(2 * NL + l, R2 * k1 + n2)];
+      }
+      Dft<R2, false>::run(v0);
+      Dft<R2, false>::run(v1);
+      Dft<R2, false>::run(v2);
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) mul3(l, R2 * k1 + k2, v0[k2], v1[k2], v2[k2]);
+      Dft<R2, true>::run(v0);
+      Dft<R2, true>::run(v1);
+      Dft<R2, true>::run(v2);
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) {
+        lds[addr(l, R2 * k1 + n2)] = v0[n2];
+        lds[addr(NL + l, R2 * k1 + n2)] = v1[n2];
+        lds[addr(2 * NL + l, R2 * k1 + n2)] = v2[n2];
+      }
+    }
+    __syncthreads();
+    for (int t = tid; t < 3 * NL * R2; t += NT) {       // inverse step A, all components
+      const int l = t % (3 * NL), n2 = t / (3 * NL);
+      float2 v[R1];
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) v[k1] = lds[addr(l, R2 * k1 + n2)];
+#pragma unroll
+      for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmulc(v[k1], tw[n2 * k1]);
+      Dft<R1, true>::run(v);
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) lds[addr(l, R2 * n1 + n2)] = v[n1];
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace cbs

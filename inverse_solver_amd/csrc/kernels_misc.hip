@@ -23,6 +23,28 @@ __global__ void green_plain(float2* __restrict__ spec, Tables tb, Consts c, int 
   }
 }
 
+// Plain form, vector runs: the dyadic multiply G (I - c r r^T) on the three
+// component spectra of one rhs (comp_stride elements apart); :607-615.
+__global__ void green_plain_vec(float2* __restrict__ spec, long long comp_stride, Tables tb, Consts c, int flip) {
+  const float2 cc = make_float2(c.c_re, c.c_im);
+  GRID_STRIDE(i, c.nvox) {
+    const int x = (int)(i % c.nx);
+    const long long yz = i / c.nx;
+    const int y = (int)(yz % c.ny);
+    const int z = (int)(yz / c.ny);
+    const float d = tb.green_x[flip * c.nx + x] + tb.green_y[flip * c.ny + y] + tb.green_z[flip * c.nz + z] - c.km2;
+    const float s = c.inv_n / fmaf(d, d, c.eps * c.eps);
+    const float2 g = make_float2(d * s, c.eps * s);
+    const float r0 = tb.rad_x[flip * c.nx + x], r1 = tb.rad_y[flip * c.ny + y], r2 = tb.rad_z[flip * c.nz + z];
+    const float2 v0 = spec[i], v1 = spec[comp_stride + i], v2 = spec[2 * comp_stride + i];
+    const float2 dot = make_float2(r0 * v0.x + r1 * v1.x + r2 * v2.x, r0 * v0.y + r1 * v1.y + r2 * v2.y);
+    const float2 cd = cmul(cc, dot);
+    spec[i] = cmul(g, make_float2(v0.x - r0 * cd.x, v0.y - r0 * cd.y));
+    spec[comp_stride + i] = cmul(g, make_float2(v1.x - r1 * cd.x, v1.y - r1 * cd.y));
+    spec[2 * comp_stride + i] = cmul(g, make_float2(v2.x - r2 * cd.x, v2.y - r2 * cd.y));
+  }
+}
+
 // Plain form: the pointwise part of one iteration on natural-layout arrays
 // (Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:595-642).
 __global__ void update_plain(StateArgs st, Tables tb, Consts c, int upd, int fwd, int jj_inv, int jj_fwd, int rhs) {

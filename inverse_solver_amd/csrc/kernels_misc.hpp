@@ -10,6 +10,7 @@ struct RoiGeom {
 };
 
 __global__ void green_plain(float2* __restrict__ spec, Tables tb, Consts c, int flip);
+__global__ void green_plain_vec(float2* __restrict__ spec, long long comp_stride, Tables tb, Consts c, int flip);
 __global__ void update_plain(StateArgs st, Tables tb, Consts c, int upd, int fwd, int jj_inv, int jj_fwd, int rhs);
 __global__ void seed_source(const float2* __restrict__ u_in, const float2* __restrict__ V, float2* __restrict__ fstate,
                             Consts c, RoiGeom rg, long long rhs_stride, int nrhs);
@@ -23,12 +24,12 @@ using RowsFn = void (*)(float2*, long long, const float2*, int);
 using StridedFn = void (*)(float2*, int, long long, long long, int, const float2*, int);
 
 struct PassRInfo { PassRFn fn; int nx, a, nt, lds; };
-struct PassKInfo { PassKFn fn; int b, nz, t, nt, lds; };
+struct PassKInfo { PassKFn fn; int b, nz, t, nt, lds, pole; };
 struct RowsInfo { RowsFn fn; int n, rows, nt; };
 struct StridedInfo { StridedFn fn; int n, t, nt; };
 
 const PassRInfo* find_pass_real(int nx, int a);
-const PassKInfo* find_pass_kspace(int b, int nz, int t);
+const PassKInfo* find_pass_kspace(int b, int nz, int t, int pole);
 const RowsInfo* find_fft_rows(int n);
 const StridedInfo* find_fft_strided(int n, int t);
 // enumerate for planning

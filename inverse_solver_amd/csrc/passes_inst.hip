@@ -30,6 +30,15 @@ constexpr int kNT = 256;
   X(2, 160, 32, 512) X(2, 160, 32, 256) \
   X(9, 160, 4, 256)
 
+// vector runs (three field components per block): (B, NZ, T, NT)
+#define PKV_LIST(X) \
+  X(1, 18, 8, 256) X(1, 20, 8, 256) X(1, 24, 8, 256) X(2, 24, 8, 256) X(4, 24, 8, 256) X(1, 30, 8, 256) \
+  X(1, 32, 16, 256) X(2, 32, 16, 256) X(1, 36, 8, 256) X(1, 40, 16, 256) X(2, 40, 16, 256) \
+  X(1, 96, 16, 256) X(2, 96, 16, 512) X(3, 96, 8, 256) \
+  X(2, 128, 16, 512) X(4, 128, 8, 512) X(2, 128, 8, 256) \
+  X(2, 135, 16, 512) X(2, 135, 8, 256) X(4, 135, 8, 512) \
+  X(2, 160, 8, 512) X(4, 160, 8, 512) X(4, 160, 8, 1024) X(2, 160, 16, 1024)
+
 #define RW_LIST(X) \
   X(16) X(18) X(20) X(24) X(25) X(27) X(30) X(32) X(36) X(40) X(48) X(64) X(96) X(128) X(135) X(160) X(192) \
   X(256) X(288) X(320) X(512) X(576) X(1024)
@@ -42,12 +51,14 @@ constexpr int rows_for(int n) { return n <= 64 ? 16 : (n <= 256 ? 8 : (n <= 576 
 constexpr int strided_t(int n) { return n <= 512 ? 16 : 8; }
 
 #define PR_ENTRY(NX, A, NT) {&pass_real<NX, A, NT>, NX, A, NT, (int)(A * row_pitch(NX) * sizeof(float2))},
-#define PK_ENTRY(B, NZ, T, NT) {&pass_kspace<B, NZ, T, NT>, B, NZ, T, NT, (int)(NZ * B * T * sizeof(float2))},
+#define PK_ENTRY(B, NZ, T, NT) {&pass_kspace<B, NZ, T, NT, 1>, B, NZ, T, NT, (int)(NZ * B * T * sizeof(float2)), 1},
+#define PKV_ENTRY(B, NZ, T, NT) \
+  {&pass_kspace<B, NZ, T, NT, 3>, B, NZ, T, NT, (int)(3 * NZ * B * T * sizeof(float2)), 3},
 #define RW_ENTRY(N) {&fft_rows<N, rows_for(N), kNT>, N, rows_for(N), kNT},
 #define ST_ENTRY(N) {&fft_strided<N, strided_t(N), kNT>, N, strided_t(N), kNT},
 
 static const PassRInfo kPR[] = {PR_LIST(PR_ENTRY)};
-static const PassKInfo kPK[] = {PK_LIST(PK_ENTRY)};
+static const PassKInfo kPK[] = {PK_LIST(PK_ENTRY) PKV_LIST(PKV_ENTRY)};
 static const RowsInfo kRW[] = {RW_LIST(RW_ENTRY)};
 static const StridedInfo kST[] = {ST_LIST(ST_ENTRY)};
 
@@ -56,9 +67,9 @@ const PassRInfo* find_pass_real(int nx, int a) {
     if (e.nx == nx && e.a == a) return &e;
   return nullptr;
 }
-const PassKInfo* find_pass_kspace(int b, int nz, int t) {
+const PassKInfo* find_pass_kspace(int b, int nz, int t, int pole) {
   for (const auto& e : kPK)
-    if (e.b == b && e.nz == nz && e.t == t) return &e;
+    if (e.b == b && e.nz == nz && e.t == t && e.pole == pole) return &e;
   return nullptr;
 }
 const RowsInfo* find_fft_rows(int n) {

@@ -82,13 +82,14 @@ static int pos2freq(int n, int p) {
 struct TableSet {
   int a = 0, b = 0;
   DevBuf<float2> tw_x, tw_a, tw_z, cross, ramp_x, ramp_y, ramp_z;
-  DevBuf<float> mask_x, mask_y, mask_z, green_x, green_y, green_z;
+  DevBuf<float> mask_x, mask_y, mask_z, green_x, green_y, green_z, rad_x, rad_y, rad_z;
   Tables view() const {
     Tables t;
     t.tw_x = tw_x.p; t.tw_a = tw_a.p; t.tw_z = tw_z.p; t.cross = cross.p;
     t.ramp_x = ramp_x.p; t.ramp_y = ramp_y.p; t.ramp_z = ramp_z.p;
     t.mask_x = mask_x.p; t.mask_y = mask_y.p; t.mask_z = mask_z.p;
     t.green_x = green_x.p; t.green_y = green_y.p; t.green_z = green_z.p;
+    t.rad_x = rad_x.p; t.rad_y = rad_y.p; t.rad_z = rad_z.p;
     return t;
   }
 };
@@ -115,6 +116,7 @@ struct Plan {
 // ------------------------------------------------------------------ solver
 struct Solver {
   int device = 0;
+  int P = 1;                      // field components: 1 scalar, 3 vector (dyadic Green's function)
   int nx = 0, ny = 0, nz = 0;
   int rx = 0, ry = 0, rz = 0;     // RI grid
   int fx = 0, fy = 0;             // field grid
@@ -180,6 +182,10 @@ struct Solver {
     const double km = 2.0 * M_PI * prm.k0_nm;
     c.km2 = (float)(km * km);
     c.inv_n = (float)(1.0 / (double)nvox);
+    // k^2 / (k^2 + i eps) = k^2 (k^2 - i eps) / (k^4 + eps^2)
+    const double k2 = km * km, e = prm.eps_imag, den = k2 * k2 + e * e;
+    c.c_re = (float)(k2 * k2 / den);
+    c.c_im = (float)(-k2 * e / den);
     return c;
   }
   RoiGeom roi() const {
@@ -258,7 +264,7 @@ struct Solver {
     return true;
   }
   bool fused_ok_k(const PassKInfo& q, int b) const {
-    if (q.b != b || q.nz != nz) return false;
+    if (q.b != b || q.nz != nz || q.pole != P) return false;
     if (opt.tile_t > 0 && q.t != opt.tile_t) return false;
     if (opt.threads_kspace > 0 && q.nt != opt.threads_kspace) return false;
     return true;
@@ -329,6 +335,13 @@ struct Solver {
       return 4.0 * M_PI * M_PI * k * k;
     };
     auto gfl = [&](int j, int n, int flip, int d) { return flip ? g0(j, n, (n - d) % n) : g0(j, n, d); };
+    // vector runs: rads = (k + sh dk/4) / k0_nm (init :318-321), flipped like Greenp (:582)
+    auto r0 = [&](int j, int n, int d) {
+      const int m = ((d + n / 2) % n) - n / 2;
+      const double sh = (prm.acyclic && (prm.shift_axes >> j & 1u)) ? 0.25 : 0.0;
+      return ((double)m + sh) * prm.fourier_res[j] / prm.k0_nm;
+    };
+    auto rfl = [&](int j, int n, int flip, int d) { return flip ? r0(j, n, (n - d) % n) : r0(j, n, d); };
     {
       std::vector<float> gx(2 * (size_t)nx + 64, 0.f);
       for (int f = 0; f < 2; ++f)
@@ -346,6 +359,21 @@ struct Solver {
       for (int f = 0; f < 2; ++f)
         for (int p = 0; p < nz; ++p) gz[(size_t)f * nz + p] = (float)gfl(2, nz, f, pos2freq(nz, p));
       T.green_z.upload(gz);
+    }
+    if (P == 3) {
+      std::vector<float> qx(2 * (size_t)nx + 64, 0.f), qy(2 * (size_t)ny, 0.f), qz(2 * (size_t)nz, 0.f);
+      for (int f = 0; f < 2; ++f) {
+        for (int p = 0; p < nx; ++p) qx[(size_t)f * nx + p] = (float)rfl(0, nx, f, pos2freq(nx, p));
+        for (int k2 = 0; k2 < b; ++k2)
+          for (int p1 = 0; p1 < a; ++p1) {
+            const int ky = (plan.form == CBS_FORM_FUSED) ? pos2freq(a, p1) + a * k2 : pos2freq(ny, p1);
+            qy[(size_t)f * ny + (size_t)k2 * a + p1] = (float)rfl(1, ny, f, ky);
+          }
+        for (int p = 0; p < nz; ++p) qz[(size_t)f * nz + p] = (float)rfl(2, nz, f, pos2freq(nz, p));
+      }
+      T.rad_x.upload(qx);
+      T.rad_y.upload(qy);
+      T.rad_z.upload(qz);
     }
   }
 
@@ -399,21 +427,25 @@ struct Solver {
   void run_fused(int nb, int M) {
     const PassRInfo* pr = plan.pr;
     const PassKInfo* pk = plan.pk;
-    const Consts c = consts(nb);
+    const int nv = nb * P;                 // component slots: the real-space pass treats each as an rhs
+    const Consts c = consts(nv);
+    const Consts ck = consts(nb);
     const Tables tb = tabs.view();
     auto pass_r = [&](int upd, int fwd, int jj_inv, int jj_fwd) {
       PassRArgs args;
       args.tb = tb; args.c = c; args.st = state_args(nb);
       if (opt.track_norms && jj_inv > 0) {
-        args.st.norms = norms.p + (size_t)(jj_inv - 1) * nb;
+        args.st.norms = norms.p + (size_t)(jj_inv - 1) * nv;
       }
       args.upd = upd; args.fwd = fwd; args.jj_inv = jj_inv; args.jj_fwd = jj_fwd;
-      const dim3 grid((unsigned)((long long)nb * plan.b * nz));
+      args.order = opt.block_order & 1;
+      const dim3 grid((unsigned)((long long)nv * plan.b * nz));
       timed(0, [&] { hipLaunchKernelGGL(pr->fn, grid, dim3(pr->nt), 0, stream, args); });
     };
     auto pass_k = [&](int jj) {
       PassKArgs args;
-      args.tb = tb; args.c = c; args.spec = spec.p; args.parity = parity.p; args.rhs_stride = nvox; args.jj = jj;
+      args.tb = tb; args.c = ck; args.spec = spec.p; args.parity = parity.p; args.rhs_stride = nvox; args.jj = jj;
+      args.order = (opt.block_order >> 1) & 1;
       const int ntile = (nx + pk->t - 1) / pk->t;
       const dim3 grid((unsigned)((long long)nb * plan.a * ntile));
       timed(1, [&] { hipLaunchKernelGGL(pk->fn, grid, dim3(pk->nt), 0, stream, args); });
@@ -427,20 +459,20 @@ struct Solver {
     HIP_CHECK(hipGetLastError());
   }
 
-  // plain schedule for rhs r of a batch of nb (state arrays indexed by r)
+  // plain schedule for rhs r of a batch of nb (state arrays indexed by component slot P r + q)
   void run_plain(int r, int nb, int M) {
     const Consts c = consts(1);
     const Tables tb = tabs.view();
-    const long long ro = (long long)r * nvox;
-    float2* sp = spec.p + ro;
+    const int nv = nb * P;
     const unsigned pw_grid = grid_for(nvox);
     auto update = [&](int upd, int fwd, int jj_inv, int jj_fwd) {
       StateArgs s = state_args(1);
-      s.norms = (opt.track_norms && jj_inv > 0) ? norms.p + (size_t)(jj_inv - 1) * nb : nullptr;
-      timed(2, [&] { hipLaunchKernelGGL(update_plain, dim3(pw_grid), dim3(256), 0, stream, s, tb, c, upd, fwd, jj_inv,
-                                        jj_fwd, r); });
+      s.norms = (opt.track_norms && jj_inv > 0) ? norms.p + (size_t)(jj_inv - 1) * nv : nullptr;
+      for (int q = 0; q < P; ++q)
+        timed(2, [&] { hipLaunchKernelGGL(update_plain, dim3(pw_grid), dim3(256), 0, stream, s, tb, c, upd, fwd,
+                                          jj_inv, jj_fwd, P * r + q); });
     };
-    auto fft3 = [&](int inverse) {
+    auto fft3 = [&](float2* sp, int inverse) {
       const RowsInfo* rw = plan.rw;
       const StridedInfo* sy = plan.sy;
       const StridedInfo* sz = plan.sz;
@@ -464,12 +496,17 @@ struct Solver {
       if (!inverse) { fx(); fy(); fz(); }
       else { fz(); fy(); fx(); }
     };
+    float2* sp0 = spec.p + (long long)(P * r) * nvox;
     update(UPD_NONE, FWD_P0, 0, 1);
     for (int jj = 1; jj <= M; ++jj) {
-      fft3(0);
-      const int flip = (jj + (int)host_parity[r]) & 1;
-      timed(2, [&] { hipLaunchKernelGGL(green_plain, dim3(pw_grid), dim3(256), 0, stream, sp, tb, c, flip); });
-      fft3(1);
+      for (int q = 0; q < P; ++q) fft3(sp0 + (long long)q * nvox, 0);
+      const int flip = (jj + (int)host_parity[P * r]) & 1;
+      if (P == 1)
+        timed(2, [&] { hipLaunchKernelGGL(green_plain, dim3(pw_grid), dim3(256), 0, stream, sp0, tb, c, flip); });
+      else
+        timed(2, [&] { hipLaunchKernelGGL(green_plain_vec, dim3(pw_grid), dim3(256), 0, stream, sp0, nvox, tb, c,
+                                          flip); });
+      for (int q = 0; q < P; ++q) fft3(sp0 + (long long)q * nvox, 1);
       const int fwd = fwd_of(jj, M);
       update(upd_of(jj), fwd, jj, fwd == FWD_NONE ? 0 : jj + 1);
     }
@@ -478,23 +515,24 @@ struct Solver {
   std::vector<uint8_t> host_parity;
 
   void ensure_workspace(int nb, bool host_io) {
-    const size_t n = (size_t)nb * nvox;
+    const size_t nv = (size_t)nb * P;
+    const size_t n = nv * nvox;
     field.ensure(n);
     fstate.ensure(n);
     spec.ensure(n);
-    parity.ensure((size_t)nb);
+    parity.ensure(nv);
     if (host_io) {
-      uin.ensure((size_t)nb * nri);
-      uout.ensure((size_t)nb * nri);
+      uin.ensure(nv * nri);
+      uout.ensure(nv * nri);
     }
   }
 
   uint64_t footprint(uint32_t n_rhs) const {
-    const int nb = batch_width(n_rhs);
-    uint64_t b = 3ull * nb * nvox * sizeof(float2);   // field, fstate, spec
-    b += 2ull * nb * nri * sizeof(float2);             // staging for host I/O
+    const int nv = batch_width(n_rhs) * P;
+    uint64_t b = 3ull * nv * nvox * sizeof(float2);   // field, fstate, spec
+    b += 2ull * nv * nri * sizeof(float2);             // staging for host I/O
     b += (uint64_t)nvox * sizeof(float2);              // V
-    b += io.footprint(nb, cx, cy, rz) + (io.ready ? (unsigned long long)nb * nri * 16ull : 0ull);
+    b += io.footprint(nv, cx, cy, rz) + (io.ready ? (unsigned long long)nv * nri * 16ull : 0ull);
     return b;
   }
   int batch_width(uint32_t n_rhs) const {
@@ -509,23 +547,24 @@ struct Solver {
     const int M = (int)prm.Bornmax;
     // the previous batch's kernels may still read the device parity array
     HIP_CHECK(hipStreamSynchronize(stream));
-    host_parity.assign((size_t)nb, 0);
-    for (int r = 0; r < nb; ++r)
-      host_parity[r] = (uint8_t)((flip_state + (long long)(rhs_index0 + r) * M) & 1);
-    HIP_CHECK(hipMemcpy(parity.p, host_parity.data(), (size_t)nb, hipMemcpyHostToDevice));
-    const Consts c = consts(nb);
+    const int nv = nb * P;
+    host_parity.assign((size_t)nv, 0);
+    for (int v = 0; v < nv; ++v)
+      host_parity[v] = (uint8_t)((flip_state + (long long)(rhs_index0 + v / P) * M) & 1);
+    HIP_CHECK(hipMemcpy(parity.p, host_parity.data(), (size_t)nv, hipMemcpyHostToDevice));
+    const Consts c = consts(nv);
     const RoiGeom g = roi();
     if (opt.track_norms && M > 0) {
-      norms.ensure((size_t)M * nb);
-      HIP_CHECK(hipMemsetAsync(norms.p, 0, (size_t)M * nb * sizeof(double), stream));
+      norms.ensure((size_t)M * nv);
+      HIP_CHECK(hipMemsetAsync(norms.p, 0, (size_t)M * nv * sizeof(double), stream));
     }
     {
-      const long long tot = (long long)nb * nvox;
+      const long long tot = (long long)nv * nvox;
       timed(2, [&] { hipLaunchKernelGGL(seed_source, dim3(grid_for(tot)), dim3(256), 0, stream, d_uin,
-                                        V.p, fstate.p, c, g, nvox, nb); });
+                                        V.p, fstate.p, c, g, nvox, nv); });
     }
     if (M == 0) {
-      HIP_CHECK(hipMemsetAsync(field.p, 0, (size_t)nb * nvox * sizeof(float2), stream));
+      HIP_CHECK(hipMemsetAsync(field.p, 0, (size_t)nv * nvox * sizeof(float2), stream));
     } else if (plan.form == CBS_FORM_FUSED) {
       if (opt.use_graph && !opt.time_kernels) {
         // replay a captured iteration loop (the kernel arguments, including the
@@ -533,7 +572,7 @@ struct Solver {
         // potential, the options or the plan change)
         const std::vector<long long> key = {nb, M, (long long)field.p, (long long)fstate.p, (long long)spec.p,
                                             (long long)parity.p, (long long)(opt.track_norms ? norms.p : nullptr),
-                                            (long long)plan.pr, (long long)plan.pk};
+                                            (long long)plan.pr, (long long)plan.pk, (long long)opt.block_order};
         if (!graph_exec || key != graph_key) {
           drop_graph();
           HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -554,18 +593,21 @@ struct Solver {
       for (int r = 0; r < nb; ++r) run_plain(r, nb, M);
     }
     {
-      const long long tot = (long long)nb * nri;
+      const long long tot = (long long)nv * nri;
       timed(2, [&] { hipLaunchKernelGGL(finalize_field, dim3(grid_for(tot)), dim3(256), 0, stream,
-                                        field.p, d_uin, d_uout, c, g, nvox, nb); });
+                                        field.p, d_uin, d_uout, c, g, nvox, nv); });
     }
     HIP_CHECK(hipGetLastError());
     if (opt.track_norms && M > 0) {
-      std::vector<double> tmp((size_t)M * nb);
+      std::vector<double> tmp((size_t)M * nv);
       HIP_CHECK(hipMemcpyAsync(tmp.data(), norms.p, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
       HIP_CHECK(hipStreamSynchronize(stream));
       for (int r = 0; r < nb; ++r)
-        for (int j = 0; j < M; ++j)
-          last_norms[(size_t)(rhs_index0 + r) * M + j] = tmp[(size_t)j * nb + r];
+        for (int j = 0; j < M; ++j) {
+          double acc = 0.0;   // |f_n|^2 summed over the field components
+          for (int q = 0; q < P; ++q) acc += tmp[(size_t)j * nv + (size_t)P * r + q];
+          last_norms[(size_t)(rhs_index0 + r) * M + j] = acc;
+        }
     }
   }
 
@@ -587,19 +629,18 @@ struct Solver {
       const int nb = (int)std::min<uint32_t>(nbw, n_rhs - r0);
       const float2* din;
       float2* dout;
+      const size_t off = (size_t)r0 * P * nri, cnt = (size_t)nb * P * nri;   // [rhs][component][z][y][x]
       if (mem == CBS_MEM_HOST) {
-        HIP_CHECK(hipMemcpyAsync(uin.p, u_in + (size_t)r0 * nri, (size_t)nb * nri * sizeof(float2),
-                                 hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(uin.p, u_in + off, cnt * sizeof(float2), hipMemcpyHostToDevice, stream));
         din = uin.p;
         dout = uout.p;
       } else {
-        din = u_in + (size_t)r0 * nri;
-        dout = u_out + (size_t)r0 * nri;
+        din = u_in + off;
+        dout = u_out + off;
       }
       born_batch(din, dout, nb, (int)r0);
       if (mem == CBS_MEM_HOST) {
-        HIP_CHECK(hipMemcpyAsync(u_out + (size_t)r0 * nri, uout.p, (size_t)nb * nri * sizeof(float2),
-                                 hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipMemcpyAsync(u_out + off, uout.p, cnt * sizeof(float2), hipMemcpyDeviceToHost, stream));
       }
     }
     HIP_CHECK(hipStreamSynchronize(stream));
@@ -680,16 +721,17 @@ struct Solver {
     const int nbw = batch_width(n_rhs);
     ensure_workspace(nbw, false);
     const long long n2 = (long long)cx * cy, nf = (long long)fx * fy;
-    io_buf3.ensure((size_t)nbw * n2 * rz);
-    io_spec2.ensure((size_t)nbw * n2);
-    if (trans) io_out_t.ensure((size_t)nbw * n2);
-    if (refl) io_out_r.ensure((size_t)nbw * n2);
-    io_uin.ensure((size_t)nbw * nri);
-    if (mem == CBS_MEM_HOST || !f3d) io_u3d.ensure((size_t)nbw * nri);
+    const size_t nvw = (size_t)nbw * P;   // component slots per batch; every 2D step is per component
+    io_buf3.ensure(nvw * n2 * rz);
+    io_spec2.ensure(nvw * n2);
+    if (trans) io_out_t.ensure(nvw * n2);
+    if (refl) io_out_r.ensure(nvw * n2);
+    io_uin.ensure(nvw * nri);
+    if (mem == CBS_MEM_HOST || !f3d) io_u3d.ensure(nvw * nri);
     if (mem == CBS_MEM_HOST) {
-      io_src.ensure((size_t)nbw * nf);
-      if (trans) io_tr.ensure((size_t)nbw * nf);
-      if (refl) io_rf.ensure((size_t)nbw * nf);
+      io_src.ensure(nvw * nf);
+      if (trans) io_tr.ensure(nvw * nf);
+      if (refl) io_rf.ensure(nvw * nf);
     }
     const int M = (int)prm.Bornmax;
     if (opt.track_norms) {
@@ -703,53 +745,55 @@ struct Solver {
     const RoiGeom rg = roi();
     for (uint32_t r0 = 0; r0 < n_rhs; r0 += nbw) {
       const int nb = (int)std::min<uint32_t>(nbw, n_rhs - r0);
-      const float2* s_d = src + (size_t)r0 * nf;
+      const int nv = nb * P;
+      const size_t v0 = (size_t)r0 * P;
+      const float2* s_d = src + v0 * nf;
       if (mem == CBS_MEM_HOST) {
-        HIP_CHECK(hipMemcpyAsync(io_src.p, s_d, (size_t)nb * nf * sizeof(float2), hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(io_src.p, s_d, (size_t)nv * nf * sizeof(float2), hipMemcpyHostToDevice, stream));
         s_d = io_src.p;
       }
       // incident field: pad -> fft2 -> x refocus(z) -> ifft2 -> crop  (:543-554)
-      timed(2, [&] { hipLaunchKernelGGL(io_pad_in, dim3(grid_for(nb * n2)), dim3(256), 0, stream, s_d, io_spec2.p, iog, nb); });
-      fft2d(io_spec2.p, nb, 0);
-      timed(2, [&] { hipLaunchKernelGGL(io_refocus, dim3(grid_for(nb * n2 * rz)), dim3(256), 0, stream, io_spec2.p,
-                                        io_refocus_k.p, io_buf3.p, io_kcx.p, io_kcy.p, iog, nb, sc); });
-      fft2d(io_buf3.p, (long long)nb * rz, 1);
-      timed(2, [&] { hipLaunchKernelGGL(io_crop_in, dim3(grid_for(nb * nri)), dim3(256), 0, stream, io_buf3.p,
-                                        io_uin.p, iog, nb); });
-      float2* u3 = (mem == CBS_MEM_DEVICE && f3d) ? f3d + (size_t)r0 * nri : io_u3d.p;
+      timed(2, [&] { hipLaunchKernelGGL(io_pad_in, dim3(grid_for(nv * n2)), dim3(256), 0, stream, s_d, io_spec2.p, iog, nv); });
+      fft2d(io_spec2.p, nv, 0);
+      timed(2, [&] { hipLaunchKernelGGL(io_refocus, dim3(grid_for(nv * n2 * rz)), dim3(256), 0, stream, io_spec2.p,
+                                        io_refocus_k.p, io_buf3.p, io_kcx.p, io_kcy.p, iog, nv, sc); });
+      fft2d(io_buf3.p, (long long)nv * rz, 1);
+      timed(2, [&] { hipLaunchKernelGGL(io_crop_in, dim3(grid_for(nv * nri)), dim3(256), 0, stream, io_buf3.p,
+                                        io_uin.p, iog, nv); });
+      float2* u3 = (mem == CBS_MEM_DEVICE && f3d) ? f3d + v0 * nri : io_u3d.p;
       born_batch(io_uin.p, u3, nb, (int)r0);
       if (trans || refl) {
         // emitter -> fft2 per z -> sum_z E K (FP64) -> ifft2 -> crop  (:444-511)
-        timed(2, [&] { hipLaunchKernelGGL(io_emitter, dim3(grid_for(nb * n2 * rz)), dim3(256), 0, stream, field.p,
-                                          io_uin.p, V.p, io_buf3.p, iog, c, rg.ox, rg.oy, rg.oz, nvox, nb,
+        timed(2, [&] { hipLaunchKernelGGL(io_emitter, dim3(grid_for(nv * n2 * rz)), dim3(256), 0, stream, field.p,
+                                          io_uin.p, V.p, io_buf3.p, iog, c, rg.ox, rg.oy, rg.oz, nvox, nv,
                                           (float)prm.dV); });
-        fft2d(io_buf3.p, (long long)nb * rz, 0);
-        timed(2, [&] { hipLaunchKernelGGL(io_zsum, dim3(grid_for(nb * n2)), dim3(256), 0, stream, io_buf3.p,
+        fft2d(io_buf3.p, (long long)nv * rz, 0);
+        timed(2, [&] { hipLaunchKernelGGL(io_zsum, dim3(grid_for(nv * n2)), dim3(256), 0, stream, io_buf3.p,
                                           io_ktrans.p, io.have_ref ? io_kref.p : nullptr, trans ? io_out_t.p : nullptr,
-                                          refl ? io_out_r.p : nullptr, io_kcx.p, io_kcy.p, iog, nb); });
+                                          refl ? io_out_r.p : nullptr, io_kcx.p, io_kcy.p, iog, nv); });
         if (trans) {
-          fft2d(io_out_t.p, nb, 1);
-          float2* dst = mem == CBS_MEM_HOST ? io_tr.p : trans + (size_t)r0 * nf;
-          timed(2, [&] { hipLaunchKernelGGL(io_crop_out, dim3(grid_for(nb * nf)), dim3(256), 0, stream, io_out_t.p,
-                                            dst, iog, nb, sc); });
+          fft2d(io_out_t.p, nv, 1);
+          float2* dst = mem == CBS_MEM_HOST ? io_tr.p : trans + v0 * nf;
+          timed(2, [&] { hipLaunchKernelGGL(io_crop_out, dim3(grid_for(nv * nf)), dim3(256), 0, stream, io_out_t.p,
+                                            dst, iog, nv, sc); });
         }
         if (refl) {
-          fft2d(io_out_r.p, nb, 1);
-          float2* dst = mem == CBS_MEM_HOST ? io_rf.p : refl + (size_t)r0 * nf;
-          timed(2, [&] { hipLaunchKernelGGL(io_crop_out, dim3(grid_for(nb * nf)), dim3(256), 0, stream, io_out_r.p,
-                                            dst, iog, nb, sc); });
+          fft2d(io_out_r.p, nv, 1);
+          float2* dst = mem == CBS_MEM_HOST ? io_rf.p : refl + v0 * nf;
+          timed(2, [&] { hipLaunchKernelGGL(io_crop_out, dim3(grid_for(nv * nf)), dim3(256), 0, stream, io_out_r.p,
+                                            dst, iog, nv, sc); });
         }
       }
       HIP_CHECK(hipGetLastError());
       if (mem == CBS_MEM_HOST) {
         if (f3d)
-          HIP_CHECK(hipMemcpyAsync(f3d + (size_t)r0 * nri, io_u3d.p, (size_t)nb * nri * sizeof(float2),
+          HIP_CHECK(hipMemcpyAsync(f3d + v0 * nri, io_u3d.p, (size_t)nv * nri * sizeof(float2),
                                    hipMemcpyDeviceToHost, stream));
         if (trans)
-          HIP_CHECK(hipMemcpyAsync(trans + (size_t)r0 * nf, io_tr.p, (size_t)nb * nf * sizeof(float2),
+          HIP_CHECK(hipMemcpyAsync(trans + v0 * nf, io_tr.p, (size_t)nv * nf * sizeof(float2),
                                    hipMemcpyDeviceToHost, stream));
         if (refl)
-          HIP_CHECK(hipMemcpyAsync(refl + (size_t)r0 * nf, io_rf.p, (size_t)nb * nf * sizeof(float2),
+          HIP_CHECK(hipMemcpyAsync(refl + v0 * nf, io_rf.p, (size_t)nv * nf * sizeof(float2),
                                    hipMemcpyDeviceToHost, stream));
       }
     }
@@ -783,10 +827,10 @@ struct Solver {
     const int nb = batch_width(n_rhs);
     ensure_workspace(nb, false);
     DevBuf<float2> src, dst;
-    src.ensure((size_t)nb * nri);
-    dst.ensure((size_t)nb * nri);
+    src.ensure((size_t)nb * P * nri);
+    dst.ensure((size_t)nb * P * nri);
     {
-      std::vector<float2> h((size_t)nb * nri);
+      std::vector<float2> h((size_t)nb * P * nri);
       for (size_t i = 0; i < h.size(); ++i) h[i] = make_float2(std::cos(0.37 * (double)i), std::sin(0.21 * (double)i));
       HIP_CHECK(hipMemcpy(src.p, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice));
     }
@@ -801,6 +845,8 @@ struct Solver {
     HIP_CHECK(hipEventCreate(&e1));
     double best = 1e30;
     Plan best_plan;
+    int best_order = 0;
+    opt.block_order = 0;
     try {
       for (auto& cd : cands) {
         Plan p;
@@ -824,6 +870,25 @@ struct Solver {
           best_plan = p;
         }
       }
+      // then the block orders of the two passes for the chosen kernels
+      plan = best_plan;
+      build_tables();
+      double best_o = 1e30;
+      for (int ord = 0; ord < 4; ++ord) {
+        opt.block_order = ord;
+        born_batch(src.p, dst.p, nb, 0);      // warm-up
+        HIP_CHECK(hipEventRecord(e0, stream));
+        born_batch(src.p, dst.p, nb, 0);
+        HIP_CHECK(hipEventRecord(e1, stream));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_o) {
+          best_o = ms;
+          best_order = ord;
+        }
+      }
+      best = best_o;
     } catch (...) {
       prm.Bornmax = saved_iters;
       flip_state = saved_flip;
@@ -844,6 +909,7 @@ struct Solver {
     opt.tile_t = best_plan.t;
     opt.threads_real = best_plan.pr->nt;
     opt.threads_kspace = best_plan.pk->nt;
+    opt.block_order = best_order;
     opt.time_kernels = saved_time;
     opt.track_norms = saved_norm;
     plan = best_plan;
@@ -1033,6 +1099,9 @@ int cbs_create(const cbs_geometry* geo, cbs_handle* out) {
     s->fx = (int)geo->field_size[0]; s->fy = (int)geo->field_size[1];
     s->cx = (int)geo->conv_size[0]; s->cy = (int)geo->conv_size[1];
     s->rc_x = geo->RI_center[0]; s->rc_y = geo->RI_center[1];
+    require(geo->pole_num == 0 || geo->pole_num == 1 || geo->pole_num == 3, CBS_ERR_ARG,
+            "pole_num must be 1 (scalar) or 3 (vector)");
+    s->P = geo->pole_num == 3 ? 3 : 1;
     s->nvox = (long long)s->nx * s->ny * s->nz;
     s->nri = (long long)s->rx * s->ry * s->rz;
     s->opt.form = CBS_FORM_FUSED;
@@ -1067,6 +1136,7 @@ int cbs_set_options(cbs_handle h, const cbs_options* opt) {
     require(opt != nullptr, CBS_ERR_ARG, "null options");
     require(opt->form == CBS_FORM_FUSED || opt->form == CBS_FORM_PLAIN, CBS_ERR_ARG, "unknown form");
     require(opt->max_batch >= 0 && opt->split_a >= 0 && opt->tile_t >= 0, CBS_ERR_ARG, "negative option");
+    require(opt->block_order >= 0 && opt->block_order <= 3, CBS_ERR_ARG, "block_order must be 0..3");
     DeviceGuard dg(s->device);
     s->opt = *opt;
     s->drop_graph();

@@ -8,8 +8,10 @@ transmitted, reflected and 3D fields.  The host side here only prepares the
 potential and the geometry in FP64, as the MATLAB class does before calling its
 MEX; every solve runs in libcbs_hip.so on the GPU.
 
-Arrays use MATLAB index order: RI is (x, y, z); 2D fields (x, y, 1, A); 3D fields
-(x, y, z, 1, A).
+Arrays use MATLAB index order: RI is (x, y, z); 2D fields (x, y, P, A) and 3D
+fields (x, y, z, P, A), with P = 1 for scalar runs and, for vector runs
+(vector_simulation = true, the dyadic Green's function), P = 2 for the 2D input and
+output fields and P = 3 (x, y, z components) for the 3D fields.
 """
 from __future__ import annotations
 
@@ -20,7 +22,7 @@ import numpy as np
 from . import _lib
 from ._lib import check, lib
 from .params import (basic_optical_parameter, update_struct, round_half_away, ri_to_potential, pupil,
-                     centered_fft2, centered_ifft2)
+                     centered_fft2, centered_ifft2, polarisation_basis)
 from .io_kernels import build_io_kernels
 
 
@@ -39,19 +41,17 @@ def _fptr(a):
 
 
 class FORWARD_SOLVER_CONVERGENT_BORN:
-    """Scalar CBS forward solver on one MI355X.
+    """CBS forward solver (scalar or vector) on one MI355X.
 
     ``options`` maps to cbs_options (form, split_a, tile_t, max_batch,
-    track_norms, use_graph, time_kernels, threads_real, threads_kspace).
+    track_norms, use_graph, time_kernels, threads_real, threads_kspace, block_order).
     """
 
     get_default_parameters = staticmethod(get_default_parameters)
 
     def __init__(self, params, device=0, options=None):
         p = dict(params)
-        if p.get('vector_simulation', False):
-            raise NotImplementedError('the dyadic (vector) Green\'s function is not implemented; '
-                                      'set vector_simulation=false')
+        self.pole_num = 3 if p.get('vector_simulation', False) else 1        # :294-297
         rxy = list(p.get('RI_xy_size', [0, 0]))
         rxy[0] = rxy[0] or p['size'][0]                                     # :54-59
         rxy[1] = rxy[1] or p['size'][1]
@@ -83,6 +83,7 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
         geo.device = int(device)
         rc = list(p.get('RI_center', [0, 0]))
         geo.RI_center[0], geo.RI_center[1] = int(rc[0]), int(rc[1])
+        geo.pole_num = self.pole_num
         h = C.c_void_p()
         check(lib.cbs_create(C.byref(geo), C.byref(h)))
         self._h = h
@@ -126,7 +127,8 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
         o = _lib.Options()
         check(lib.cbs_get_options(self._h, C.byref(o)))
         return {'form': 'plain' if f.value == _lib.CBS_FORM_PLAIN else 'fused', 'a': a.value, 'b': b.value,
-                'tile_t': t.value, 'threads_real': o.threads_real, 'threads_kspace': o.threads_kspace}
+                'tile_t': t.value, 'threads_real': o.threads_real, 'threads_kspace': o.threads_kspace,
+                'block_order': o.block_order}
 
     # ---------------------------------------------------------------- potential (:180-396)
     def set_RI(self, RI):
@@ -185,63 +187,113 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
         check(lib.cbs_set_io_kernels(self._h, *[_fptr(a) for a in self._io_arrays], _lib.CBS_MEM_HOST))
         self._io_ready = True
 
-    def transform_field_3D(self, F):
-        """FORWARD_SOLVER.m:38-48 (scalar): NA circle, then 1/sqrt(cos theta) (Abbe sine)."""
+    def _pupil_filter(self, F, inverse):
         p = self.parameters
         na, cos_t, _ = pupil(p['size'][:2], p['resolution'], p['wavelength'], p['NA'], p['RI_bg'])
         shp = na.shape + (1,) * (F.ndim - 2)
         F = F * na.reshape(shp)
         if p['use_abbe_sine']:
-            filt = np.where(na, 1.0 / np.sqrt(np.where(na, cos_t, 1.0)), 0.0)
-            F = F * filt.reshape(shp)
+            if inverse:
+                F = F * np.where(na, 1.0 / np.sqrt(np.where(na, cos_t, 1.0)), 0.0).reshape(shp)
+            else:
+                F = F * (np.sqrt(cos_t) * na).reshape(shp)
         return F
+
+    def transform_field_3D(self, F):
+        """FORWARD_SOLVER.m:38-68: NA circle, 1/sqrt(cos theta) (Abbe sine) and, for two
+        polarisations (x, y, 2, ...), the change to the 3D field (x, y, 3, ...) through
+        the radial / perpendicular basis and the Ewald tangent."""
+        F = self._pupil_filter(F, True)
+        if F.shape[2] == 1:
+            return F
+        if F.shape[2] != 2:
+            raise ValueError('Far field has two polarisation')
+        p = self.parameters
+        rad, perp, tan, inside = polarisation_basis(F.shape[:2], p['resolution'], p['wavelength'], p['RI_bg'])
+        ex = (slice(None), slice(None), None) + (None,) * (F.ndim - 3)
+        F = F * inside[ex]
+        a_rad = F[:, :, 0] * rad[:, :, 0][ex[:2] + ex[3:]] + F[:, :, 1] * rad[:, :, 1][ex[:2] + ex[3:]]
+        a_perp = F[:, :, 0] * perp[:, :, 0][ex[:2] + ex[3:]] + F[:, :, 1] * perp[:, :, 1][ex[:2] + ex[3:]]
+        out = np.empty(F.shape[:2] + (3,) + F.shape[3:], dtype=np.result_type(F, np.complex64))
+        for j in range(3):
+            out[:, :, j] = a_rad * tan[:, :, j][ex[:2] + ex[3:]]
+        for j in range(2):
+            out[:, :, j] += a_perp * perp[:, :, j][ex[:2] + ex[3:]]
+        return out
+
+    def _to_2d(self, F, reflection):
+        F = self._pupil_filter(F, False)
+        if F.shape[2] == 1:
+            return F
+        if F.shape[2] != 3:
+            raise ValueError('Near field has three polarisation')
+        p = self.parameters
+        rad, perp, tan, inside = polarisation_basis(F.shape[:2], p['resolution'], p['wavelength'], p['RI_bg'])
+        if reflection:
+            tan = tan * np.array([1.0, 1.0, -1.0])                         # :114 (k3 reversed)
+        ex = (slice(None), slice(None)) + (None,) * (F.ndim - 3)
+        F = F * inside[ex][:, :, None]
+        a_tan = sum(F[:, :, j] * tan[:, :, j][ex] for j in range(3))
+        a_perp = sum(F[:, :, j] * perp[:, :, j][ex] for j in range(2))
+        out = np.empty(F.shape[:2] + (2,) + F.shape[3:], dtype=np.result_type(F, np.complex64))
+        for j in range(2):
+            out[:, :, j] = a_tan * rad[:, :, j][ex] + a_perp * perp[:, :, j][ex]
+        return out
 
     def transform_field_2D(self, F):
-        """FORWARD_SOLVER.m:69-77 (scalar; the reflection variant :98-106 is identical for scalars)."""
-        p = self.parameters
-        na, cos_t, _ = pupil(p['size'][:2], p['resolution'], p['wavelength'], p['NA'], p['RI_bg'])
-        shp = na.shape + (1,) * (F.ndim - 2)
-        F = F * na.reshape(shp)
-        if p['use_abbe_sine']:
-            F = F * (np.sqrt(cos_t) * na).reshape(shp)
-        return F
+        """FORWARD_SOLVER.m:69-97 (3 -> 2 polarisations for vector fields)."""
+        return self._to_2d(F, False)
 
-    transform_field_2D_reflection = transform_field_2D
+    def transform_field_2D_reflection(self, F):
+        """FORWARD_SOLVER.m:98-129: as transform_field_2D with the tangent's z part
+        negated for the reversed propagation direction."""
+        return self._to_2d(F, True)
 
     def solve(self, input_field):
-        """Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:397-520 / _CUDA.m:69-148 (scalar).
+        """Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:397-520 / _CUDA.m:69-148.
 
-        input_field: (x, y, 1, A) (or (x, y, A) / (x, y)) on the field grid.
-        Returns (fields_trans (x, y, 1, A) or None, fields_ref (x, y, 1, A) or None,
-        fields_3D (x, y, z, 1, A) or None) as complex64, following return_* flags."""
+        input_field: (x, y, P, A) on the field grid (P = 1 scalar, 2 vector; (x, y, A) and
+        (x, y) are accepted for scalar runs, (x, y, 2) for one vector angle).
+        Returns (fields_trans (x, y, P, A) or None, fields_ref (x, y, P, A) or None,
+        fields_3D (x, y, z, 1 or 3, A) or None) as complex64, following return_* flags."""
         trans, ref, f3 = self._solve(input_field, None)
-        return trans, ref, (None if f3 is None else f3[:, :, :, None, :])
+        return trans, ref, f3
 
     def solve_keep_3d(self, input_field, field_3d, return_2d=True):
         """solve() with the 3D fields left in device memory: field_3d is a
-        device.DeviceArray of n_rhs * RI-grid complex64 (x fastest, angle slowest),
-        e.g. for cbs_field_gradient.  Returns (fields_trans, fields_ref), both None
-        when return_2d is False (only the 3D fields are formed)."""
+        device.DeviceArray of n_rhs * components * RI-grid complex64 (x fastest, then
+        y, z, component, angle), e.g. for cbs_field_gradient.  Returns (fields_trans,
+        fields_ref), both None when return_2d is False (only the 3D fields are formed)."""
         trans, ref, _ = self._solve(input_field, field_3d, return_2d)
         return trans, ref
 
-    def _solve(self, input_field, field_3d_dev, return_2d=True):
-        p = self.parameters
+    def _inputs(self, input_field):
         inp = np.asarray(input_field)
+        vec = self.pole_num == 3
         if inp.ndim == 2:
             inp = inp[:, :, None, None]
         elif inp.ndim == 3:
-            inp = inp[:, :, None, :]
-        if inp.shape[2] > 1:
+            inp = inp[:, :, :, None] if vec else inp[:, :, None, :]
+        if inp.shape[2] > 1 and not vec:                                    # :403-407
             raise ValueError('the source is 2D but parameter indicate a vectorial simulation')
+        if inp.shape[2] == 1 and vec:
+            raise ValueError('the source is 3D but parameter indicate a non-vectorial simulation')
+        if inp.shape[2] > 2:                                                # :411-413
+            raise ValueError('Input field must be either a scalar or a 2D vector')
+        return inp
+
+    def _solve(self, input_field, field_3d_dev, return_2d=True):
+        p = self.parameters
+        inp = self._inputs(input_field)
         if list(inp.shape[:2]) != [int(p['size'][0]), int(p['size'][1])]:
             raise ValueError('input field must have the field size %s' % (p['size'][:2],))
         if not hasattr(self, 'Bornmax'):
             raise RuntimeError('set_RI must be called before solve')
         self._ensure_io()
         A = inp.shape[3]
-        U = self.transform_field_3D(centered_fft2(inp.astype(np.complex128)))          # :416-418
-        src = np.asfortranarray(centered_ifft2(U)[:, :, 0, :].astype(np.complex64))    # _CUDA.m:122
+        P = self.pole_num
+        U = self.transform_field_3D(centered_fft2(inp.astype(np.complex128)))          # :415-417
+        src = np.asfortranarray(centered_ifft2(U).astype(np.complex64))                # _CUDA.m:122
         fx, fy = int(p['size'][0]), int(p['size'][1])
         rx, ry, rz = self.expected_RI_size
         want_t = bool(p['return_transmission']) and return_2d
@@ -249,19 +301,19 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
         want_3 = bool(p['return_3D']) or field_3d_dev is not None
         nul = C.c_void_p(None)
         if field_3d_dev is None:
-            t2 = np.empty((fx, fy, A), np.complex64, order='F') if want_t else None
-            r2 = np.empty((fx, fy, A), np.complex64, order='F') if want_r else None
-            f3 = np.empty((rx, ry, rz, A), np.complex64, order='F') if want_3 else None
+            t2 = np.empty((fx, fy, P, A), np.complex64, order='F') if want_t else None
+            r2 = np.empty((fx, fy, P, A), np.complex64, order='F') if want_r else None
+            f3 = np.empty((rx, ry, rz, P, A), np.complex64, order='F') if want_3 else None
             check(lib.cbs_solve_2d(self._h, _fptr(src), A, _fptr(t2) if want_t else nul,
                                    _fptr(r2) if want_r else nul, _fptr(f3) if want_3 else nul, _lib.CBS_MEM_HOST))
         else:
             from . import device as dv
-            if field_3d_dev.nbytes != 8 * rx * ry * rz * A or field_3d_dev.dtype != np.complex64:
-                raise ValueError('field_3d must hold %d x %s complex64 values' % (A, [rx, ry, rz]))
+            if field_3d_dev.nbytes != 8 * rx * ry * rz * P * A or field_3d_dev.dtype != np.complex64:
+                raise ValueError('field_3d must hold %d x %d x %s complex64 values' % (A, P, [rx, ry, rz]))
             f3 = None
             d_src = dv.DeviceArray.from_host(np.ascontiguousarray(src.T))
-            d_t = dv.DeviceArray((A, fy, fx)) if want_t else None
-            d_r = dv.DeviceArray((A, fy, fx)) if want_r else None
+            d_t = dv.DeviceArray((A, P, fy, fx)) if want_t else None
+            d_r = dv.DeviceArray((A, P, fy, fx)) if want_r else None
             check(lib.cbs_solve_2d(self._h, C.c_void_p(d_src.ptr), A, C.c_void_p(d_t.ptr) if want_t else nul,
                                    C.c_void_p(d_r.ptr) if want_r else nul, C.c_void_p(field_3d_dev.ptr),
                                    _lib.CBS_MEM_DEVICE))
@@ -269,27 +321,34 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
             r2 = d_r.to_host().T if want_r else None
         trans = ref = None
         if want_t:                                                                      # :481-492
-            T = centered_fft2(t2.astype(np.complex128)) + U[:, :, 0, :]
-            trans = centered_ifft2(self.transform_field_2D(T)).astype(np.complex64)[:, :, None, :]
+            T = centered_fft2(t2.astype(np.complex128)) + U
+            trans = centered_ifft2(self.transform_field_2D(T)).astype(np.complex64)
         if want_r:                                                                      # :505-508
             R = centered_fft2(r2.astype(np.complex128))
-            ref = centered_ifft2(self.transform_field_2D_reflection(R)).astype(np.complex64)[:, :, None, :]
+            ref = centered_ifft2(self.transform_field_2D_reflection(R)).astype(np.complex64)
         return trans, ref, f3
 
     # ---------------------------------------------------------------- Born loop from a 3D incident field
     def solve_incident(self, u_in):
-        """Born loop of :564-647 for incident fields u_in (x, y, z[, A]) on the RI grid;
-        returns Field[ROI] + u_in with the same shape (complex64)."""
+        """Born loop of :564-647 for incident fields on the RI grid: (x, y, z[, A]) for
+        scalar runs, (x, y, z, 3[, A]) for vector runs; returns Field[ROI] + u_in with
+        the same shape (complex64)."""
         u = np.asarray(u_in)
-        squeeze = u.ndim == 3
+        P = self.pole_num
+        squeeze = u.ndim == (3 if P == 1 else 4)
         if squeeze:
             u = u[..., None]
-        if list(u.shape[:3]) != self.expected_RI_size:
-            raise ValueError('incident field must have the RI grid size %s' % (self.expected_RI_size,))
-        A = u.shape[3]
+        if P == 1:
+            u = u[:, :, :, None, :]
+        if list(u.shape[:3]) != self.expected_RI_size or u.shape[3] != P:
+            raise ValueError('incident field must have the RI grid size %s%s' % (
+                self.expected_RI_size, '' if P == 1 else ' x 3 components'))
+        A = u.shape[4]
         uf = np.asfortranarray(u.astype(np.complex64))
         out = np.empty_like(uf, order='F')
         check(lib.cbs_solve_3d(self._h, _fptr(uf), A, _fptr(out), _lib.CBS_MEM_HOST))
+        if P == 1:
+            out = out[:, :, :, 0, :]
         return out[..., 0] if squeeze else out
 
     def solve_incident_device(self, u_in_ptr, n_rhs, u_out_ptr):

@@ -69,3 +69,33 @@ def centered_ifft2(a):
     """fftshift(ifft2(ifftshift(a))) over the first two axes."""
     ax = (0, 1)
     return np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(a, axes=ax), axes=ax), axes=ax)
+
+
+def polarisation_basis(size_xy, resolution, wavelength, n_m):
+    """Per transverse frequency of a centred 2D grid: the radial and perpendicular
+    unit vectors in the pupil, the unit tangent of the Ewald sphere in the plane of
+    incidence, and the mask of propagating frequencies (Codes/FORWARD_SOLVER.m:131-180).
+
+    With K = k_xy / k_m, rho = |K|, K3 = sqrt(1 - rho^2) and phi the azimuth:
+    radial = (cos phi, sin phi), perpendicular = (sin phi, -cos phi) and tangent =
+    (K3 cos phi, K3 sin phi, -rho); at rho = 0 the azimuth is taken as 0.  The
+    reference builds its K_2 axis with size(1) samples (:141), so only square field
+    grids work there; other grids are refused here."""
+    n0, n1 = int(size_xy[0]), int(size_xy[1])
+    if n0 != n1:
+        raise ValueError('vector fields need a square field grid (FORWARD_SOLVER.m:141 builds K_2 with size(1) '
+                         'samples)')
+    kx, ky = fourier_grid((n0, n1), resolution[:2])
+    k_m = n_m / wavelength
+    K1 = kx[:, None] / k_m
+    K2 = ky[None, :] / k_m
+    rho = np.sqrt(K1 ** 2 + K2 ** 2)
+    K3 = np.sqrt(np.maximum(1.0 - rho ** 2, 0.0))
+    inside = K3 > 0
+    safe = np.where(rho > 0, rho, 1.0)
+    cphi = np.where(rho > 0, K1 / safe, 1.0)
+    sphi = np.where(rho > 0, K2 / safe, 0.0)
+    radial = np.stack([cphi, sphi], axis=2)
+    perp = np.stack([sphi, -cphi], axis=2)
+    tangent = np.stack([K3 * cphi, K3 * sphi, -rho], axis=2) * inside[:, :, None]
+    return radial, perp, tangent, inside

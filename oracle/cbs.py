@@ -3,7 +3,8 @@
 Test infrastructure only (see oracle/__init__.py).
 
 Follows Codes/FORWARD_SOLVER_CONVERGENT_BORN.m and Codes/FORWARD_SOLVER.m line by
-line for the scalar (vector_simulation=false) branches.  ``dtype`` selects the
+line for the scalar (vector_simulation=false) and the vector (dyadic Green's
+function, pole_num = 3, scalar RI) branches.  ``dtype`` selects the
 complex type of the solve: complex128 is the oracle; complex64 is the op-for-op
 CPU port of the reference loop that ``bench.py`` times as the CPU baseline.
 """
@@ -15,7 +16,7 @@ import numpy as np
 import scipy.fft as sfft
 
 from .optics import (basic_optical_parameter, update_struct, derive_optical_tool, ri2potential,
-                     potential2ri, fft_flip, cfft2, cifft2, cifftn3)
+                     potential2ri, fft_flip, cfft2, cifft2, cifftn3, polarisation_utility)
 from .hankel import truncated_green_plus
 
 
@@ -37,17 +38,16 @@ def forward_solver_default_parameters(init=None):
 
 
 class ForwardSolverConvergentBorn:
-    """FORWARD_SOLVER_CONVERGENT_BORN (scalar) restated in NumPy.
+    """FORWARD_SOLVER_CONVERGENT_BORN restated in NumPy.
 
-    Arrays use MATLAB index order: RI is (x, y, z); input fields (x, y, 1, A);
-    3D fields (x, y, z, 1, A).
+    Arrays use MATLAB index order: RI is (x, y, z); input fields (x, y, P, A) with
+    P = 1 (scalar) or 2 (vector); 3D fields (x, y, z, 1 or 3, A).
     """
 
     def __init__(self, params, dtype=np.complex128, workers=None, io_kernels=True):
         self.parameters = dict(params)
         p = self.parameters
-        if p.get('vector_simulation', False):
-            raise NotImplementedError('oracle restates the scalar branch only')
+        self.pole_num = 3 if p.get('vector_simulation', False) else 1          # :294-297
         self.dtype = np.dtype(dtype)
         self.rdtype = np.float64 if self.dtype == np.complex128 else np.float32
         self.workers = workers if workers is not None else os.cpu_count()
@@ -265,79 +265,134 @@ class ForwardSolverConvergentBorn:
             ramp = np.ones((1, 1, 1), dtype=np.complex128)
         self.phase_ramp = ramp
         self.Greenp = sfft.ifftshift(g, axes=(0, 1, 2))                         # :384
+        # dyadic terms (:318-324): rads = k_shifted / k0_nm per axis, and the
+        # absorption correction k^2 / (k^2 + i eps) of the longitudinal part
+        full = self.Greenp.shape
+        rads = np.stack([np.broadcast_to(sc[j] / ub['k0_nm'], full) for j in range(3)], axis=3)
+        self.rads = sfft.ifftshift(rads, axes=(0, 1, 2))                         # :385
+        k2 = (2 * np.pi * ub['k0_nm']) ** 2
+        self.green_absorbtion_correction = k2 / (k2 + 1j * self.eps_imag)       # :324
         self.flip_count = 0
 
-    # ---- 2D <-> 3D field transforms (FORWARD_SOLVER.m:38-97, scalar branches)
-    def transform_field_3D(self, F):
-        """FORWARD_SOLVER.m:38-48,65-67."""
-        u = self.utility
-        na = u['NA_circle'][:, :, 0]
-        F = F * na.reshape(na.shape + (1,) * (F.ndim - 2))
-        if self.parameters['use_abbe_sine']:
-            filt = na.astype(np.float64)
-            filt[na] = filt[na] / np.sqrt(u['cos_theta'][:, :, 0][na])
-            F = F * filt.reshape(filt.shape + (1,) * (F.ndim - 2))
-        return F
-
-    def transform_field_2D(self, F):
-        """FORWARD_SOLVER.m:69-77,94-96 (also :98-106 reflection, identical for scalar)."""
+    # ---- 2D <-> 3D field transforms (FORWARD_SOLVER.m:38-129)
+    def _na_abbe(self, F, inverse):
         u = self.utility
         na = u['NA_circle'][:, :, 0]
         shp = na.shape + (1,) * (F.ndim - 2)
         F = F * na.reshape(shp)
         if self.parameters['use_abbe_sine']:
-            F = F * (np.sqrt(u['cos_theta'][:, :, 0]) * na).reshape(shp)
+            if inverse:                                                        # :43-47
+                filt = na.astype(np.float64)
+                filt[na] = filt[na] / np.sqrt(u['cos_theta'][:, :, 0][na])
+            else:                                                              # :72-75
+                filt = np.sqrt(u['cos_theta'][:, :, 0]) * na
+            F = F * filt.reshape(shp)
         return F
+
+    def transform_field_3D(self, F):
+        """FORWARD_SOLVER.m:38-68: NA circle, 1/sqrt(cos theta) (Abbe sine) and, for two
+        polarisations, the (radial, perpendicular) -> 3D Ewald-tangent basis change."""
+        F = self._na_abbe(F, True)
+        if F.shape[2] == 1:
+            return F
+        if F.shape[2] != 2:
+            raise ValueError('Far field has two polarisation')                 # :49-51
+        p = self.parameters
+        R2, P2, T3, Km = polarisation_utility(F.shape, p['RI_bg'], p['wavelength'], p['resolution'])   # :53
+        F = F * Km[:, :, None, None]                                           # :55
+        b1 = np.sum(F * R2[:, :, :, None], axis=2)                             # :58
+        b2 = np.sum(F * P2[:, :, :, None], axis=2)                             # :59
+        out = b1[:, :, None, :] * T3[:, :, :, None]                            # :62
+        out[:, :, :2, :] += b2[:, :, None, :] * P2[:, :, :, None]              # :63
+        return out
+
+    def _to_2d(self, F, reflection):
+        F = self._na_abbe(F, False)
+        if F.shape[2] == 1:
+            return F
+        if F.shape[2] != 3:
+            raise ValueError('Near field has three polarisation')              # :78-80
+        p = self.parameters
+        R2, P2, T3, Km = polarisation_utility(F.shape, p['RI_bg'], p['wavelength'], p['resolution'])
+        if reflection:
+            T3 = T3.copy()
+            T3[:, :, 2] = -T3[:, :, 2]                                         # :114
+        F = F * Km[:, :, None, None]
+        b1 = np.sum(F * T3[:, :, :, None], axis=2)                             # :87
+        b2 = np.sum(F[:, :, :2, :] * P2[:, :, :, None], axis=2)                # :88
+        return b1[:, :, None, :] * R2[:, :, :, None] + b2[:, :, None, :] * P2[:, :, :, None]   # :90-92
+
+    def transform_field_2D(self, F):
+        """FORWARD_SOLVER.m:69-97 (3D -> 2 polarisations for vector fields)."""
+        return self._to_2d(F, False)
+
+    def transform_field_2D_reflection(self, F):
+        """FORWARD_SOLVER.m:98-129 (the Ewald tangent's z part is negated, :114)."""
+        return self._to_2d(F, True)
 
     # ---- solve (:397-520)
     def solve(self, input_field):
+        """:397-520.  input_field (x, y, P, A); P = 1 for scalar, 2 for vector runs."""
         p = self.parameters
         inp = np.asarray(input_field)
         if inp.ndim == 2:
             inp = inp[:, :, None, None]
         elif inp.ndim == 3:
-            inp = inp[:, :, None, :]
-        if inp.shape[2] != 1:
-            raise ValueError('scalar solver expects one polarisation')
-        inp = cfft2(inp.astype(self.dtype), workers=self.workers)              # :416
-        inp = self.transform_field_3D(inp)                                     # :418
+            inp = inp[:, :, None, :] if self.pole_num == 1 else inp[:, :, :, None]
+        if inp.shape[2] > 1 and self.pole_num == 1:                            # :403-407
+            raise ValueError('the source is 2D but parameter indicate a vectorial simulation')
+        if inp.shape[2] == 1 and self.pole_num == 3:
+            raise ValueError('the source is 3D but parameter indicate a non-vectorial simulation')
+        if inp.shape[2] > 2:                                                   # :411-413
+            raise ValueError('Input field must be either a scalar or a 2D vector')
+        inp = cfft2(inp.astype(self.dtype), workers=self.workers)              # :415
+        inp = self.transform_field_3D(inp)                                     # :417
         A = inp.shape[3]
+        npol = inp.shape[2]
+        out_pol = 2 if self.pole_num == 3 else 1                               # :419-422
         R = self.ROI
         nx, ny, nz = R[1] - R[0] + 1, R[3] - R[2] + 1, R[5] - R[4] + 1
-        trans = np.ones((nx, ny, 1, A), dtype=self.dtype) if p['return_transmission'] else None
-        refl = np.ones((nx, ny, 1, A), dtype=self.dtype) if p['return_reflection'] else None
-        f3d = np.ones((nx, ny, nz, 1, A), dtype=self.dtype) if p['return_3D'] else None
+        trans = np.ones((nx, ny, out_pol, A), dtype=self.dtype) if p['return_transmission'] else None
+        refl = np.ones((nx, ny, out_pol, A), dtype=self.dtype) if p['return_reflection'] else None
+        f3d = np.ones((nx, ny, nz, npol, A), dtype=self.dtype) if p['return_3D'] else None
         roi = (slice(R[0] - 1, R[1]), slice(R[2] - 1, R[3]), slice(R[4] - 1, R[5]))
         for n in range(A):                                                     # :437
-            Field = self.solve_raw(inp[:, :, 0, n])                            # :439
+            Field = self.solve_raw(inp[:, :, 0, n] if npol == 1 else inp[:, :, :, n])   # :439
+            Field = Field.reshape(Field.shape[:3] + (npol,))
             if p['return_3D']:
-                f3d[:, :, :, 0, n] = Field
+                f3d[:, :, :, :, n] = Field
             if p['return_transmission'] or p['return_reflection']:
                 pot = ri2potential(self.RI[roi], p['wavelength'], p['RI_bg'])  # :445
-                em = Field * pot * self.utility_border['dV']                    # :448
+                em = Field * pot[:, :, :, None] * self.utility_border['dV']     # :448 (scalar RI)
                 if not self.cyclic_boundary_xy:
                     em = self.padd_RI2conv(em)                                  # :458
                 em = cfft2(em, workers=self.workers)                           # :460
             if p['return_transmission']:
-                ft = self.crop_conv2field(cifft2(np.sum(em * self.kernel_trans, axis=2)))  # :472
-                ft = cfft2(ft) + inp[:, :, 0, n]                               # :481-484
-                ft = self.transform_field_2D(ft)                               # :490
-                trans[:, :, 0, n] = cifft2(ft)                                 # :491-492
+                ft = self.crop_conv2field(cifft2(np.sum(em * self.kernel_trans[:, :, :, None], axis=2)))  # :472
+                ft = cfft2(ft) + inp[:, :, :, n]                               # :481-484
+                ft = self.transform_field_2D(ft[:, :, :, None])[:, :, :, 0]    # :490
+                trans[:, :, :, n] = cifft2(ft)                                 # :491-492
             if p['return_reflection']:
-                fr = self.crop_conv2field(cifft2(np.sum(em * self.kernel_ref, axis=2)))  # :500
+                fr = self.crop_conv2field(cifft2(np.sum(em * self.kernel_ref[:, :, :, None], axis=2)))  # :500
                 fr = cfft2(fr)                                                 # :505
-                fr = self.transform_field_2D(fr)                               # :506
-                refl[:, :, 0, n] = cifft2(fr)                                  # :507-508
+                fr = self.transform_field_2D_reflection(fr[:, :, :, None])[:, :, :, 0]   # :506
+                refl[:, :, :, n] = cifft2(fr)                                  # :507-508
         return trans, refl, f3d
 
     def incident_field(self, source):
-        """:543-554: refocus the filtered 2D spectrum into the 3D incident field u_in."""
-        s = cifft2(source.astype(self.dtype), workers=self.workers)            # :543
+        """:543-554: refocus the filtered 2D spectrum (x, y[, P]) into the 3D incident
+        field u_in (x, y, z[, P]) on the RI grid, each polarisation separately."""
+        src = np.asarray(source)
+        vec = src.ndim == 3
+        if not vec:
+            src = src[:, :, None]
+        s = cifft2(src.astype(self.dtype), workers=self.workers)               # :543
         s = self.padd_field2conv(s)                                            # :544
         s = cfft2(s, workers=self.workers)                                     # :548
-        s = s[:, :, None] * self.refocusing_util                               # :550
+        s = s[:, :, None, :] * self.refocusing_util[:, :, :, None]             # :550
         s = cifft2(s, workers=self.workers)                                    # :553
-        return self.crop_conv2RI(s)                                            # :554
+        s = self.crop_conv2RI(s)                                               # :554
+        return s if vec else s[:, :, :, 0]
 
     def solve_raw(self, source):
         """:521-654 from the 2D spectrum of one angle."""
@@ -345,28 +400,38 @@ class ForwardSolverConvergentBorn:
         return self.born_loop(u_in)
 
     def born_loop(self, incident_field):
-        """:564-647 (scalar): the two-chain, alternating-shift Born series.
+        """:564-647: the two-chain, alternating-shift Born series.
 
-        ``self.Greenp`` and ``self.phase_ramp`` are flipped/conjugated at the top
-        of every iteration and the flipped state persists across calls, exactly
-        as the reference's handle properties do (:579-585); it is reset by
-        set_RI.  For odd Bornmax this makes consecutive angles start on
-        opposite shifts, as in the reference.
+        Scalar: incident_field (x, y, z).  Vector (pole_num = 3): (x, y, z, 3), with
+        the dyadic Green's function G (I - c rads rads^T) of :608-615, c the
+        absorption correction of :324.
+
+        ``self.Greenp``, ``self.rads`` and ``self.phase_ramp`` are flipped/conjugated
+        at the top of every iteration and the flipped state persists across calls,
+        exactly as the reference's handle properties do (:579-585); it is reset by
+        set_RI.  For odd Bornmax this makes consecutive angles start on opposite
+        shifts, as in the reference.
         """
         p = self.parameters
         dt = self.dtype
         R = self.ROI
         roi = (slice(R[0] - 1, R[1]), slice(R[2] - 1, R[3]), slice(R[4] - 1, R[5]))
+        vec = self.pole_num == 3
         V = self.V.astype(dt)
         mask = self.attenuation_mask.astype(self.rdtype)
         eps = self.eps_imag
-        shape = V.shape
+        inc = np.asarray(incident_field).astype(dt)
+        if vec:
+            V = V[:, :, :, None]
+            mask = mask[:, :, :, None] if mask.ndim == 3 else mask.reshape(mask.shape + (1,))
+        shape = self.V.shape + ((3,) if vec else ())
         psi = np.zeros(shape, dt)
         Field = np.zeros(shape, dt)
         Field_n = np.zeros(shape, dt)
-        inc = np.asarray(incident_field).astype(dt)
-        source = (V[roi] + 1j * eps) * inc                                     # :566
+        source = (V[roi] + 1j * eps) * inc                                     # :565-575 (scalar RI)
         Greenp = self.Greenp.astype(dt)
+        rads = self.rads.astype(dt) if vec else None
+        cabs = self.green_absorbtion_correction
         ramp = self.phase_ramp.astype(dt)
         ax = (0, 1, 2)
         w = self.workers
@@ -374,6 +439,8 @@ class ForwardSolverConvergentBorn:
         for jj in range(1, self.Bornmax + 1):                                  # :577
             if p['acyclic']:                                                   # :579-585
                 Greenp = fft_flip(Greenp, [1, 1, 1], False)
+                if vec:
+                    rads = fft_flip(rads, [1, 1, 1], False)
                 ramp = np.conj(ramp)
             psi[...] = 0                                                       # :588-589
             if jj in (1, 2):
@@ -381,8 +448,18 @@ class ForwardSolverConvergentBorn:
                 psi[roi] = (1j / eps) * source / 2                             # :596
             else:
                 psi = (1j / eps) * Field_n * V                                 # :599
-            PSI = Greenp * sfft.fftn(psi * ramp, axes=ax, workers=w)           # :609,613
-            PSI = sfft.ifftn(PSI, axes=ax, workers=w) * np.conj(ramp)          # :617
+            if vec:
+                PSI = np.zeros(shape, dt)
+                for j2 in range(3):                                            # :607-611
+                    coeff = sfft.fftn(psi[..., j2] * ramp, axes=ax, workers=w)
+                    e = np.zeros(3)
+                    e[j2] = 1
+                    PSI = PSI + (Greenp[..., None] * (e - cabs * rads * rads[..., j2:j2 + 1])) * coeff[..., None]
+                for j1 in range(3):                                            # :616-618
+                    PSI[..., j1] = sfft.ifftn(PSI[..., j1], axes=ax, workers=w) * np.conj(ramp)
+            else:
+                PSI = Greenp * sfft.fftn(psi * ramp, axes=ax, workers=w)       # :609,613
+                PSI = sfft.ifftn(PSI, axes=ax, workers=w) * np.conj(ramp)      # :617
             if jj not in (1, 2):
                 Field_n = Field_n - psi                                        # :619-621
             Field_n = Field_n + V * PSI                                        # :623
@@ -392,8 +469,10 @@ class ForwardSolverConvergentBorn:
             Field = Field + Field_n                                            # :638
             if jj == 3:
                 Field_n = Field_n + temp                                       # :639-642
-        if p['acyclic']:
+        if p['acyclic'] and self.Bornmax % 2:
             # persist the flipped state like the reference's handle properties
-            self.Greenp = Greenp.astype(np.complex128) if self.Bornmax % 2 else self.Greenp
-            self.phase_ramp = ramp.astype(np.complex128) if self.Bornmax % 2 else self.phase_ramp
+            self.Greenp = Greenp.astype(np.complex128)
+            self.phase_ramp = ramp.astype(np.complex128)
+            if vec:
+                self.rads = rads.astype(np.complex128)
         return Field[roi] + inc                                                # :646-647

@@ -121,3 +121,41 @@ def cfft1(a, axis, inverse=False, workers=None):
     """fftshift(fft(ifftshift(a), [], axis)) (or ifft) along one axis."""
     f = sfft.ifft if inverse else sfft.fft
     return sfft.fftshift(f(sfft.ifftshift(a, axes=axis), axis=axis, workers=workers), axes=axis)
+
+
+def polarisation_utility(ZP, n_m, lam, dx):
+    """Codes/FORWARD_SOLVER.m:131-180 (FP64).  Returns Radial_2D (x, y, 2), Perp_2D
+    (x, y, 2), ewald_TanVec (x, y, 3) and K_mask (x, y).
+
+    The reference builds K_2 over -floor(ZP(2)/2):ZP(1)-floor(ZP(2)/2)-1 (:141), i.e.
+    with ZP(1) samples, so a non-square field grid fails inside MATLAB; the same
+    grids are refused here."""
+    ZP = [int(ZP[0]), int(ZP[1])]
+    if ZP[0] != ZP[1]:
+        raise ValueError('polarisation_utility: the reference supports square field grids only (FORWARD_SOLVER.m:141)')
+    k0 = 1.0 / lam                                                          # :135
+    k = 2 * np.pi * n_m * k0                                                # :136
+    kres = [1.0 / dx[0] / ZP[0], 1.0 / dx[1] / ZP[1]]                       # :138
+    K1 = (2 * np.pi * kres[0] / k * (np.arange(ZP[0]) - ZP[0] // 2)).reshape(-1, 1)   # :140,143
+    K2 = (2 * np.pi * kres[1] / k * (np.arange(ZP[0]) - ZP[1] // 2)).reshape(1, -1)   # :141,144
+    K3 = np.real(np.sqrt((1 - (K1 ** 2 + K2 ** 2)).astype(np.complex128)))  # :146
+    K_mask = ~(K3 == 0)                                                     # :147
+    norm_rad = np.sqrt(K1 ** 2 + K2 ** 2)                                   # :151
+    with np.errstate(divide='ignore', invalid='ignore'):
+        t1 = K1 / norm_rad
+        t2 = K2 / norm_rad
+    t1 = np.where(norm_rad == 0, 1.0, t1)                                   # :155-156
+    t2 = np.where(norm_rad == 0, 0.0, t2)
+    Radial_2D = np.stack([t1, t2], axis=2)                                  # :158-159
+    Perp_2D = np.stack([Radial_2D[:, :, 1], -Radial_2D[:, :, 0]], axis=2)   # :163-164
+    norm_sph = np.sqrt(K1 ** 2 + K2 ** 2 + K3 ** 2)                         # :167
+    Radial_3D = np.stack([np.broadcast_to(K1 / norm_sph, K3.shape), np.broadcast_to(K2 / norm_sph, K3.shape),
+                          K3 / norm_sph], axis=2)                           # :168-170
+    proj = np.sum(Radial_3D[:, :, :2] * Radial_2D, axis=2)                  # :172
+    T = np.zeros(K3.shape + (3,))
+    T[:, :, :2] = Radial_2D                                                 # :175
+    T = T - proj[:, :, None] * Radial_3D                                    # :176
+    nrm = np.sqrt(np.sum(T ** 2, axis=2))                                   # :177
+    nrm[~K_mask] = 1                                                        # :178
+    T = T / nrm[:, :, None]                                                 # :179
+    return Radial_2D, Perp_2D, T, K_mask
