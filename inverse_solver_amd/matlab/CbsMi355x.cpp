@@ -103,6 +103,8 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
       const mxArray* rc = mxGetField(s, 0, "RI_center");
       e->geo.RI_center[0] = rc ? (int32_t)num_at(rc, 0) : 0;
       e->geo.RI_center[1] = rc ? (int32_t)num_at(rc, 1) : 0;
+      const mxArray* pn = mxGetField(s, 0, "pole_num");   // 3: vector_simulation (:294-297)
+      e->geo.pole_num = (pn && mxGetNumberOfElements(pn)) ? (int32_t)num_at(pn, 0) : 1;
       check(cbs_create(&e->geo, &e->h));
       const size_t nk = (size_t)e->geo.conv_size[0] * e->geo.conv_size[1] * e->geo.RI_size[2];
       const void* kref = nrhs > 5 ? (const void*)complex_single(prhs[5], nk, "kernel_ref") : nullptr;
@@ -143,9 +145,9 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     case 2: {  // Born loop from 3D incident fields
       if (nrhs < 3) mexErrMsgIdAndTxt("cbs:arg", "Need to give the field to simulate");
       Entry* e = entry(prhs[1]);
-      const size_t nri = (size_t)e->geo.RI_size[0] * e->geo.RI_size[1] * e->geo.RI_size[2];
+      const size_t nri = (size_t)e->geo.RI_size[0] * e->geo.RI_size[1] * e->geo.RI_size[2] * e->geo.pole_num;
       const size_t n = mxGetNumberOfElements(prhs[2]);
-      if (n % nri) mexErrMsgIdAndTxt("cbs:arg", "field size is not a multiple of the RI grid");
+      if (n % nri) mexErrMsgIdAndTxt("cbs:arg", "field size is not a multiple of the RI grid (times the components)");
       const mxComplexSingle* u = complex_single(prhs[2], 0, "u_in");
       plhs[0] = mxCreateUninitNumericArray(mxGetNumberOfDimensions(prhs[2]), mxGetDimensions(prhs[2]),
                                            mxSINGLE_CLASS, mxCOMPLEX);
@@ -163,26 +165,25 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     case 4: {  // 2D in / 2D out
       if (nrhs < 3) mexErrMsgIdAndTxt("cbs:arg", "Need to give the field to simulate");
       Entry* e = entry(prhs[1]);
-      const size_t nf = (size_t)e->geo.field_size[0] * e->geo.field_size[1];
-      const size_t nri = (size_t)e->geo.RI_size[0] * e->geo.RI_size[1] * e->geo.RI_size[2];
+      const mwSize P = (mwSize)e->geo.pole_num;   // components per angle: [x y (P) A]
+      const size_t nf = (size_t)e->geo.field_size[0] * e->geo.field_size[1] * P;
       const size_t n = mxGetNumberOfElements(prhs[2]);
-      if (n % nf) mexErrMsgIdAndTxt("cbs:arg", "source size is not a multiple of the field grid");
+      if (n % nf) mexErrMsgIdAndTxt("cbs:arg", "source size is not a multiple of the field grid (times the components)");
       const mwSize A = (mwSize)(n / nf);
-      const mwSize d2[3] = {e->geo.field_size[0], e->geo.field_size[1], A};
-      const mwSize d3[4] = {e->geo.RI_size[0], e->geo.RI_size[1], e->geo.RI_size[2], A};
-      plhs[0] = mxCreateUninitNumericArray(3, d2, mxSINGLE_CLASS, mxCOMPLEX);
+      const mwSize d2[4] = {e->geo.field_size[0], e->geo.field_size[1], P, A};
+      const mwSize d3[5] = {e->geo.RI_size[0], e->geo.RI_size[1], e->geo.RI_size[2], P, A};
+      plhs[0] = mxCreateUninitNumericArray(4, d2, mxSINGLE_CLASS, mxCOMPLEX);
       void* tr = mxGetComplexSingles(plhs[0]);
       void* rf = nullptr;
       void* f3 = nullptr;
       if (nlhs > 1) {
-        plhs[1] = mxCreateUninitNumericArray(3, d2, mxSINGLE_CLASS, mxCOMPLEX);
+        plhs[1] = mxCreateUninitNumericArray(4, d2, mxSINGLE_CLASS, mxCOMPLEX);
         rf = mxGetComplexSingles(plhs[1]);
       }
       if (nlhs > 2) {
-        plhs[2] = mxCreateUninitNumericArray(4, d3, mxSINGLE_CLASS, mxCOMPLEX);
+        plhs[2] = mxCreateUninitNumericArray(5, d3, mxSINGLE_CLASS, mxCOMPLEX);
         f3 = mxGetComplexSingles(plhs[2]);
       }
-      (void)nri;
       check(cbs_solve_2d(e->h, complex_single(prhs[2], 0, "source_2d"), (uint32_t)A, tr, rf, f3, CBS_MEM_HOST));
       break;
     }

@@ -31,6 +31,7 @@ classdef FORWARD_SOLVER_CONVERGENT_BORN_MI355X < FORWARD_SOLVER_CONVERGENT_BORN
             end
             g.used_gpu=uint32(params.used_gpu);
             g.RI_center=h.parameters.RI_center;
+            g.pole_num=int32(h.pole_num);
             h.cbs_id=CbsMi355x(uint32(0),uint32(0),g,single(h.refocusing_util),single(h.kernel_trans),single(h.kernel_ref));
         end
         function set_RI(h,RI)
@@ -59,26 +60,29 @@ classdef FORWARD_SOLVER_CONVERGENT_BORN_MI355X < FORWARD_SOLVER_CONVERGENT_BORN
         end
         function [fields_trans,fields_ref,fields_3D]=solve(h,input_field)
             input_field=single(input_field);
-            if size(input_field,3)>1
-                error('the scalar MI355X backend expects one polarisation');
+            if size(input_field,3)>1 && ~h.parameters.vector_simulation
+                error('the source is 2D but parameter indicate a vectorial simulation');
+            elseif size(input_field,3)==1 && h.parameters.vector_simulation
+                error('the source is 3D but parameter indicate a non-vectorial simulation');
             end
             input_field=fftshift(fft2(ifftshift(input_field)));
             input_field=h.transform_field_3D(input_field);
             src=complex(fftshift(ifft2(ifftshift(input_field))));
             [t,r,u]=CbsMi355x(uint32(4),uint32(h.cbs_id),reshape(src,size(src,1),size(src,2),[]));
             A=size(input_field,4);
+            P=size(input_field,3); % 1, or 3 field components for vector runs
             fields_trans=[]; fields_ref=[]; fields_3D=[];
             if h.parameters.return_3D
-                fields_3D=reshape(u,size(u,1),size(u,2),size(u,3),1,A);
+                fields_3D=reshape(u,size(u,1),size(u,2),size(u,3),P,A);
             end
             if h.parameters.return_transmission
-                t=fftshift(fft2(ifftshift(reshape(t,size(t,1),size(t,2),1,A))));
+                t=fftshift(fft2(ifftshift(reshape(t,size(t,1),size(t,2),P,A))));
                 t=t+input_field;
                 t=h.transform_field_2D(t);
                 fields_trans=fftshift(ifft2(ifftshift(t)));
             end
             if h.parameters.return_reflection
-                r=fftshift(fft2(ifftshift(reshape(r,size(r,1),size(r,2),1,A))));
+                r=fftshift(fft2(ifftshift(reshape(r,size(r,1),size(r,2),P,A))));
                 r=h.transform_field_2D_reflection(r);
                 fields_ref=fftshift(ifft2(ifftshift(r)));
             end
