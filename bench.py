@@ -282,6 +282,87 @@ def workload_large(args):
                                                                    'iterations': 6}})
 
 
+def workload_vector(args):
+    """Vector (dyadic Green's function) Born loop: 128^3 RI + 16 px = 160^3 sim, 8 spheres
+    (seed 2), p-polarised plane waves for 16 circle angles, 200 iterations; inputs
+    resident in HBM; the same unit as the scalar line (voxel-iterations, each voxel
+    carrying the three field components)."""
+    from inverse_solver_amd import FORWARD_SOLVER_CONVERGENT_BORN
+    from inverse_solver_amd import device as dv
+    from inverse_solver_amd.phantoms import spheres, incident_plane_waves, circle_delta_positions
+    from oracle.cbs import ForwardSolverConvergentBorn as Oracle
+    ri = list(args.ri)
+    A = 16 if args.angles == 49 else args.angles
+    M = args.iters
+    RI, _ = spheres(tuple(ri), 8, (0.5, 2.0), (1.36, 1.48), seed=args.seed)
+    u = incident_plane_waves(ri, A, 1.2, 0.532, 1.336, [0.1, 0.1, 0.1])
+    # p polarisation: unit vector in the plane of incidence, perpendicular to k
+    k0 = 1.336 / 0.532
+    pol = np.zeros((3, A))
+    for m, (p1, p2) in enumerate(circle_delta_positions(ri[:2], A, 1.2, 0.532, [0.1] * 3, 0.95)):
+        kx = -(p1 - ri[0] // 2) / (0.1 * ri[0])
+        ky = -(p2 - ri[1] // 2) / (0.1 * ri[1])
+        kz = np.sqrt(max(k0 * k0 - kx * kx - ky * ky, 0.0))
+        rho = np.hypot(kx, ky)
+        c, sn = (kx / rho, ky / rho) if rho > 0 else (1.0, 0.0)
+        pol[:, m] = [kz / k0 * c, kz / k0 * sn, -rho / k0]
+    u3 = (u[:, :, :, None, :] * pol[None, None, None, :, :]).astype(np.complex64)   # (x, y, z, 3, A)
+    del u
+    p = workload_params(ri, M, A)
+    p['vector_simulation'] = True
+    s = FORWARD_SOLVER_CONVERGENT_BORN(p, options=dict(time_kernels=1))
+    s.set_RI(RI)
+    if not args.no_tune:
+        s.autotune(A, 6)
+    nvox = int(np.prod(s.simulation_size))
+    u_host = np.ascontiguousarray(np.transpose(u3, (4, 3, 2, 1, 0)))
+    d_in = dv.DeviceArray.from_host(u_host)
+    d_out = dv.DeviceArray(u_host.shape, np.complex64)
+    stream = s.stream()
+    for _ in range(args.warmup):
+        s.solve_incident_device(d_in.ptr, A, d_out.ptr)
+    dv.synchronize()
+    e0, e1 = dv.Event(), dv.Event()
+    step_ms, pk = [], [0.0, 0.0, 0.0, 0.0]
+    for k in range(args.steps):
+        e0.record(stream)
+        s.solve_incident_device(d_in.ptr, A, d_out.ptr)
+        e1.record(stream)
+        e1.synchronize()
+        step_ms.append(e0.elapsed_ms(e1))
+        kt = s.kernel_times()
+        pk[0] += kt['pass_real'][0]; pk[1] += kt['pass_real'][1]
+        pk[2] += kt['pass_kspace'][0]; pk[3] += kt['pass_kspace'][1]
+        log('vector step', k + 1, '%.1f ms' % step_ms[-1])
+    ms = float(np.mean(step_ms))
+    pr_ms, pk_ms = pk[0] / max(pk[1], 1), pk[2] / max(pk[3], 1)
+    bytes_pr = float(pass_real_bytes(nvox, 3 * A, M).sum()) / (pk[1] / args.steps)   # 3A component slots share V
+    bytes_pk = 16.0 * 3 * A * nvox
+    out = d_out.to_host()
+    # parity sample: angle 0, 8 iterations, against the FP64 oracle
+    pc = workload_params(ri, 8, 1)
+    pc['vector_simulation'] = True
+    sg = FORWARD_SOLVER_CONVERGENT_BORN(pc)
+    sg.set_RI(RI)
+    u0 = u_host[0].transpose(3, 2, 1, 0)
+    got = sg.solve_incident(u0)
+    sg.close()
+    o = Oracle(pc, io_kernels=False, workers=os.cpu_count())
+    o.set_RI(RI)
+    err = _rel(got, o.born_loop(u0))
+    plan = s.plan()
+    s.close()
+    return _line('Gvoxel-iter/s (vector CBS, 3 field components per voxel)', round(nvox * M * A / (ms / 1e3) / 1e9, 3),
+                 'Gvoxel-iter/s', 'vector Born loop: %dx%dx%d RI + 16-px boundary = %s sim, %d p-polarised plane '
+                 'waves, %d iterations' % (ri[0], ri[1], ri[2], s.simulation_size, A, M), ms, args,
+                 {'plan': plan, 'finite': bool(np.isfinite(out).all()),
+                  'kernels': {'pass_real': {'avg_launch_ms': round(pr_ms, 4),
+                                            'GB_s': round(bytes_pr / (pr_ms / 1e3) / 1e9, 1)},
+                              'pass_kspace_vector': {'avg_launch_ms': round(pk_ms, 4),
+                                                     'GB_s': round(bytes_pk / (pk_ms / 1e3) / 1e9, 1)}},
+                  'accuracy': {'rel_l2_err_vs_fp64_oracle': err, 'angles': 1, 'iterations': 8}})
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     ap.add_argument('--gpus', type=int, default=1)
@@ -300,7 +381,8 @@ def main():
     ap.add_argument('--no-tune', action='store_true', help='use the planner default instead of the autotuner')
     ap.add_argument('--copy-probe-mb', type=int, default=0,
                     help='run one device-to-device copy of this many MiB first (calibrates HBM byte counters)')
-    ap.add_argument('--workload', default='batched', choices=['batched', 'c1', 'inverse', 'serving', 'large'],
+    ap.add_argument('--workload', default='batched',
+                    choices=['batched', 'c1', 'inverse', 'serving', 'large', 'vector'],
                     help='batched: the headline 49-angle solve (default); the others are the secondary '
                          'configurations, each printing its own JSON line')
     ap.add_argument('--requests', type=int, default=200, help='serving: number of requests')
@@ -316,7 +398,7 @@ def main():
 
     if args.workload != 'batched':
         out = {'c1': workload_c1, 'inverse': workload_inverse, 'serving': workload_serving,
-               'large': workload_large}[args.workload](args)
+               'large': workload_large, 'vector': workload_vector}[args.workload](args)
         print(json.dumps(out), flush=True)
         return
 
