@@ -282,6 +282,95 @@ def workload_large(args):
                                                                    'iterations': 6}})
 
 
+def reference_form_gpu(s, u0, M, log_fn):
+    """The reference's own GPU loop, op for op: FORWARD_SOLVER_CONVERGENT_BORN.solve_raw
+    (:577-647) as MATLAB runs it on gpuArrays for one angle -- one device kernel per
+    array expression, 3D FFTs from the vendor library (here torch.fft on rocFFT),
+    fft_flip of Greenp and conj of the ramp every iteration, complex64.  A
+    measurement baseline only (the product never runs it).  Tables follow
+    :251-271 (mask), :347-351,384 (Greenp), :355-377 (ramp)."""
+    import torch
+    dev = torch.device('cuda:0')
+    sim = list(s.simulation_size)
+    b = [int(x) for x in s.boundary_thickness_pixel]
+    p = s.parameters
+    eps = float(s.eps_imag)
+    # arrays held as (z, y, x) so x is the fastest axis, as in MATLAB's column-major order
+    V = torch.from_numpy(np.ascontiguousarray(s._Vf.T)).to(dev)
+    def axis_shape(j):
+        shp = [1, 1, 1]
+        shp[2 - j] = -1
+        return shp
+    mask = torch.ones((1, 1, 1), dtype=torch.float32, device=dev)
+    ramp = torch.ones((1, 1, 1), dtype=torch.complex64, device=dev)
+    g = None
+    fres = [1.0 / (p['resolution'][j] * sim[j]) for j in range(3)]
+    k0 = p['RI_bg'] / p['wavelength']
+    ks = []
+    for j in range(3):
+        n = sim[j]
+        x = np.abs(np.arange(1, n + 1) - (np.floor(n / 2 + 1)) + 0.5) - 0.5
+        x = np.roll(x, -(n // 2))
+        if b[j] > 0:
+            v0 = np.minimum(np.abs(x / (b[j] - 0.5)), 1.0)
+            v0 = np.minimum(1 - v0, p['boundary_sharpness']) / p['boundary_sharpness']
+        else:
+            v0 = np.zeros(n)
+        mask = mask * torch.from_numpy((1 - v0).astype(np.float32)).to(dev).reshape(axis_shape(j))
+        r = np.exp(-1j * np.pi * np.arange(n) / n / 2)
+        r = r / r[n // 2]
+        ramp = ramp * torch.from_numpy(r.astype(np.complex64)).to(dev).reshape(axis_shape(j))
+        ks.append(((np.arange(n) - n // 2) * fres[j] + fres[j] / 4).reshape(axis_shape(j)))
+    g = 1.0 / (4 * np.pi ** 2 * (ks[0] ** 2 + ks[1] ** 2 + ks[2] ** 2) - (2 * np.pi * k0) ** 2 - 1j * eps)
+    Greenp = torch.from_numpy(np.fft.ifftshift(g).astype(np.complex64)).to(dev)
+    R = [int(v) for v in s.ROI]
+    roi = (slice(R[4] - 1, R[5]), slice(R[2] - 1, R[3]), slice(R[0] - 1, R[1]))
+    inc = torch.from_numpy(np.ascontiguousarray(np.asarray(u0).T).astype(np.complex64)).to(dev)
+    dims = (0, 1, 2)
+
+    def run():
+        nonlocal Greenp, ramp
+        psi = torch.zeros_like(V)
+        Field = torch.zeros_like(V)
+        Field_n = torch.zeros_like(V)
+        source = (V[roi] + 1j * eps) * inc
+        temp = None
+        for jj in range(1, M + 1):
+            Greenp = torch.roll(torch.flip(Greenp, dims), (1, 1, 1), dims)     # fft_flip(Greenp,[1 1 1],false)
+            ramp = torch.conj(ramp)
+            psi.zero_()
+            if jj in (1, 2):
+                Field_n.zero_()
+                psi[roi] = (1j / eps) * source / 2
+            else:
+                psi = (1j / eps) * Field_n * V
+            PSI = Greenp * torch.fft.fftn(psi * ramp, dim=dims)
+            PSI = torch.fft.ifftn(PSI, dim=dims) * torch.conj(ramp)
+            if jj not in (1, 2):
+                Field_n = Field_n - psi
+            Field_n = Field_n + V * PSI
+            Field_n = Field_n * mask
+            if jj == 2:
+                temp = Field.clone()
+            Field = Field + Field_n
+            if jj == 3:
+                Field_n = Field_n + temp
+        return Field[roi] + inc
+
+    G0, r0 = Greenp.clone(), ramp.clone()
+    run()                                   # warm-up: FFT plans
+    torch.cuda.synchronize()
+    Greenp, ramp = G0, r0
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    out = run()
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1)
+    log_fn('reference-form GPU loop (torch/rocFFT, 1 angle): %.1f ms' % ms)
+    return ms, out.cpu().numpy()
+
+
 def workload_vector(args):
     """Vector (dyadic Green's function) Born loop: 128^3 RI + 16 px = 160^3 sim, 8 spheres
     (seed 2), p-polarised plane waves for 16 circle angles, 200 iterations; inputs
@@ -378,6 +467,8 @@ def main():
                     help='iterations of the complex64 CPU-port sample (1 angle, same grid)')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline and the oracle check')
     ap.add_argument('--no-plain', action='store_true', help='skip the plain single-angle comparison')
+    ap.add_argument('--no-refform', action='store_true',
+                    help='skip the reference-form GPU loop (torch/rocFFT restatement of solve_raw)')
     ap.add_argument('--no-tune', action='store_true', help='use the planner default instead of the autotuner')
     ap.add_argument('--copy-probe-mb', type=int, default=0,
                     help='run one device-to-device copy of this many MiB first (calibrates HBM byte counters)')
@@ -517,6 +608,20 @@ def main():
         out['plain_single_angle']['rel_diff_vs_batched'] = float(diff)
         sp_.close()
         log('plain single angle %.1f ms -> %.2f Gvoxel-iter/s' % (pms, plain_val))
+
+    if not args.no_refform:
+        # the reference's MATLAB gpuArray loop restated op for op on the GPU, one angle
+        sr = FORWARD_SOLVER_CONVERGENT_BORN(p)
+        sr.set_RI(RI)
+        rms, rout = reference_form_gpu(sr, u[..., 0], M, log)
+        sr.close()
+        rval = float(nvox) * M / (rms / 1e3) / 1e9
+        rdiff = float(np.linalg.norm(rout - result_gpu[0]) / np.linalg.norm(result_gpu[0]))
+        out['reference_form_single_angle'] = {
+            'value': round(rval, 3), 'unit': 'Gvoxel-iter/s', 'ms_per_angle': round(rms, 3),
+            'what': 'solve_raw :577-647 op for op on the GPU (one kernel per MATLAB array expression, '
+                    'torch.fft/rocFFT 3D FFTs, fft_flip + conj every iteration), complex64, one angle',
+            'speedup_of_batched_fused': round(value / rval, 2), 'rel_diff_vs_batched': rdiff}
 
     if not args.no_cpu:
         from oracle.cbs import ForwardSolverConvergentBorn as Oracle
