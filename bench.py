@@ -265,6 +265,29 @@ def workload_large(args):
     sec = time.perf_counter() - t0
     nvox = int(np.prod(s.simulation_size))
     val = nvox * M * A / sec / 1e9
+    kt = s.kernel_times()
+    kinds = {}
+    for k, (ms_, n_) in kt.items():
+        if n_:
+            kinds[k] = {'avg_launch_ms': round(ms_ / n_, 4), 'launches': int(n_)}
+    # per-launch algorithmic bytes of the four-pass form: real-space pass 48 B/voxel/rhs (+V),
+    # each y transform 16 B, the k-space pass 16 B
+    if 'pass_real' in kinds:
+        kinds['pass_real']['GB_s'] = round(float(pass_real_bytes(nvox, A, M).sum()) / kt['pass_real'][0] / 1e6, 1)
+    if 'pass_kspace' in kinds:
+        kinds['pass_kspace']['GB_s'] = round(16.0 * A * nvox * kt['pass_kspace'][1] / kt['pass_kspace'][0] / 1e6, 1)
+    # the per-axis plain form on the same grid, 1 angle x 10 iterations, for comparison
+    pp = workload_params(ri, 10, 1)
+    pp.update(boundary_thickness=[16, 16, 16])
+    spn = FORWARD_SOLVER_CONVERGENT_BORN(pp, options=dict(form=1))
+    spn.set_RI(RI)
+    spn.solve_incident_device(d_in.ptr, 1, d_out.ptr)
+    dv.synchronize()
+    t1 = time.perf_counter()
+    spn.solve_incident_device(d_in.ptr, 1, d_out.ptr)
+    dv.synchronize()
+    plain_val = nvox * 10 / (time.perf_counter() - t1) / 1e9
+    spn.close()
     # parity sample: 1 angle x 6 iterations against the FP64 oracle
     pc = workload_params(ri, 6, 1)
     pc.update(boundary_thickness=[16, 16, 16])
@@ -278,8 +301,9 @@ def workload_large(args):
     err = _rel(got, o.born_loop(u0))
     return _line('Gvoxel-iter/s (large volume)', round(val, 3), 'Gvoxel-iter/s',
                  'C5: 512x512x256 RI + 32-px boundary = 576x576x320 sim, 30 angles, %d iterations' % M,
-                 sec * 1e3, args, {'plan': s.plan(), 'accuracy': {'rel_l2_err_vs_fp64_oracle': err, 'angles': 1,
-                                                                   'iterations': 6}})
+                 sec * 1e3, args, {'plan': s.plan(), 'kernels': kinds,
+                                   'plain_form_single_angle': {'value': round(plain_val, 3), 'unit': 'Gvoxel-iter/s'},
+                                   'accuracy': {'rel_l2_err_vs_fp64_oracle': err, 'angles': 1, 'iterations': 6}})
 
 
 def reference_form_gpu(s, u0, M, log_fn):

@@ -41,7 +41,7 @@ enum {
 };
 
 enum { CBS_MEM_HOST = 0, CBS_MEM_DEVICE = 1 };
-enum { CBS_FORM_FUSED = 0, CBS_FORM_PLAIN = 1 };
+enum { CBS_FORM_FUSED = 0, CBS_FORM_PLAIN = 1, CBS_FORM_ROWS = 2 };
 
 /* Geometry of a solver instance.
  * Replaces the struct of CudaBorn action 0 (main.cpp:57-101: simulation_size,
@@ -80,7 +80,10 @@ typedef struct {
 } cbs_potential_params;
 
 typedef struct {
-  int32_t form;         /* CBS_FORM_FUSED (two passes per iteration) or CBS_FORM_PLAIN */
+  int32_t form;         /* CBS_FORM_FUSED (two passes per iteration; grids without such a plan fall
+                           back to CBS_FORM_ROWS, then CBS_FORM_PLAIN), CBS_FORM_ROWS (four passes:
+                           row blocks in real space, y transforms, z transforms with the Green's
+                           multiply) or CBS_FORM_PLAIN (per-axis transforms, one rhs at a time) */
   int32_t split_a;      /* fused form: force ny = a*b split (0 = planner's choice) */
   int32_t tile_t;       /* fused form: force the k-space x tile (0 = planner's choice) */
   int32_t max_batch;    /* right-hand sides per launch (0 = all of the call) */

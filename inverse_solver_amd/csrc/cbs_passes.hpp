@@ -23,7 +23,12 @@ struct PassRArgs {
                // 1 = y2 fastest, then rhs (the b blocks of one (rhs, z) slab run together)
 };
 
-template <int NX, int A, int NT>
+// ROWS = 0: the tile is the y coset y = y1 * b + y2 and the pass does the a-point
+// part of the y transform (with the cross twiddles).  ROWS = 1 (four-pass form
+// for grids without a two-pass plan): the tile is A contiguous rows
+// y = y2 * A + y1, only the x transforms are done here, and the spectrum stays
+// in the natural [z][y][x] layout for the separate y and z passes.
+template <int NX, int A, int NT, int ROWS>
 __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
   static_assert(NX % 2 == 0, "pass_real processes x in pairs");
   constexpr int PR = row_pitch(NX);
@@ -77,10 +82,10 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
   }
   for (int i = tid; i < A; i += NT) {
     s_twa[i] = tb.tw_a[i];
-    const int y = i * c.b + y2;
+    const int y = ROWS ? y2 * A + i : i * c.b + y2;
     s_ryz[i] = cmul(tb.ramp_y[y], tb.ramp_z[z]);
     s_myz[i] = tb.mask_y[y] * tb.mask_z[z];
-    s_cr[i] = tb.cross[y2 * A + i];
+    s_cr[i] = ROWS ? make_float2(1.f, 0.f) : tb.cross[y2 * A + i];
   }
   __syncthreads();
 
@@ -95,7 +100,7 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
         const int e = 2 * i;
         const int p1 = e / NX, px = e - p1 * NX;
         float2 v0 = lo2(reg[k]), v1 = hi2(reg[k]);
-        if (c.b > 1) {
+        if (!ROWS && c.b > 1) {
           const float2 w = s_cr[p1];
           v0 = cmulc(v0, w);
           v1 = cmulc(v1, w);
@@ -105,7 +110,7 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
       }
     }
     __syncthreads();
-    if constexpr (A > 1) line_fft<A, true, NX, NT>(lds, col, s_twa, tid);
+    if constexpr (A > 1 && !ROWS) line_fft<A, true, NX, NT>(lds, col, s_twa, tid);
     line_fft<NX, true, A, NT>(lds, row, s_twx, tid);
   }
 
@@ -127,7 +132,7 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
       if (k0 + k < L && i < E2) {
         const int e = 2 * i;
         const int y1 = e / NX, x = e - y1 * NX;
-        const long long g = ((long long)z * c.ny + (y1 * c.b + y2)) * NX + x;
+        const long long g = ((long long)z * c.ny + (ROWS ? y2 * A + y1 : y1 * c.b + y2)) * NX + x;
         gi[k] = g;
         vv[k] = *reinterpret_cast<const float4*>(V + g);
         if (rf) xx[k] = *reinterpret_cast<const float4*>(f + g);
@@ -163,7 +168,7 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
   if (fwd == FWD_NONE) return;
   __syncthreads();
   line_fft<NX, false, A, NT>(lds, row, s_twx, tid);
-  if constexpr (A > 1) line_fft<A, false, NX, NT>(lds, col, s_twa, tid);
+  if constexpr (A > 1 && !ROWS) line_fft<A, false, NX, NT>(lds, col, s_twa, tid);
 #pragma unroll
   for (int k = 0; k < L; ++k) {
     const int i = tid + k * NT;
@@ -171,7 +176,7 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
       const int e = 2 * i;
       const int p1 = e / NX, px = e - p1 * NX;
       float2 v0 = lds[p1 * PR + px], v1 = lds[p1 * PR + px + 1];
-      if (c.b > 1) {
+      if (!ROWS && c.b > 1) {
         const float2 w = s_cr[p1];
         v0 = cmul(v0, w);
         v1 = cmul(v1, w);
@@ -286,7 +291,7 @@ __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
       auto green = [&](int l, int pz, float2 v) {
         const int k2 = l / T, t = l - k2 * T;
         const float d = gx[t < tvalid ? t : 0] + gy[k2 * c.a] + gz[pz] - km2;
-        const float s = inv_n / fmaf(d, d, eps2);
+        const float s = inv_n * __builtin_amdgcn_rcpf(fmaf(d, d, eps2));   // hardware reciprocal (1 ulp)
         return cmul(v, make_float2(d * s, eps * s));
       };
       line_fft_mul_ifft<NZ, LW, NT>(lds, zl, tb.tw_z, tid, green);
@@ -300,7 +305,7 @@ __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
         const int k2 = l / T, t = l - k2 * T;
         const int tx = t < tvalid ? t : 0;
         const float d = gx[tx] + gy[k2 * c.a] + gz[pz] - km2;
-        const float s = inv_n / fmaf(d, d, eps2);
+        const float s = inv_n * __builtin_amdgcn_rcpf(fmaf(d, d, eps2));   // hardware reciprocal (1 ulp)
         const float2 g = make_float2(d * s, eps * s);
         const float r0 = rx[tx], r1 = ry[k2 * c.a], r2 = rz[pz];
         const float2 dot = make_float2(r0 * v0.x + r1 * v1.x + r2 * v2.x, r0 * v0.y + r1 * v1.y + r2 * v2.y);

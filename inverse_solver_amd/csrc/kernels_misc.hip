@@ -18,7 +18,7 @@ __global__ void green_plain(float2* __restrict__ spec, Tables tb, Consts c, int 
     const int y = (int)(yz % c.ny);
     const int z = (int)(yz / c.ny);
     const float d = tb.green_x[flip * c.nx + x] + tb.green_y[flip * c.ny + y] + tb.green_z[flip * c.nz + z] - c.km2;
-    const float s = c.inv_n / fmaf(d, d, c.eps * c.eps);
+    const float s = c.inv_n * __builtin_amdgcn_rcpf(fmaf(d, d, c.eps * c.eps));   // hardware reciprocal (1 ulp)
     spec[i] = cmul(spec[i], make_float2(d * s, c.eps * s));
   }
 }
@@ -33,7 +33,7 @@ __global__ void green_plain_vec(float2* __restrict__ spec, long long comp_stride
     const int y = (int)(yz % c.ny);
     const int z = (int)(yz / c.ny);
     const float d = tb.green_x[flip * c.nx + x] + tb.green_y[flip * c.ny + y] + tb.green_z[flip * c.nz + z] - c.km2;
-    const float s = c.inv_n / fmaf(d, d, c.eps * c.eps);
+    const float s = c.inv_n * __builtin_amdgcn_rcpf(fmaf(d, d, c.eps * c.eps));   // hardware reciprocal (1 ulp)
     const float2 g = make_float2(d * s, c.eps * s);
     const float r0 = tb.rad_x[flip * c.nx + x], r1 = tb.rad_y[flip * c.ny + y], r2 = tb.rad_z[flip * c.nz + z];
     const float2 v0 = spec[i], v1 = spec[comp_stride + i], v2 = spec[2 * comp_stride + i];

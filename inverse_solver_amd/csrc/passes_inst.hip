@@ -19,6 +19,9 @@ constexpr int kNT = 256;
   X(192, 48, 256) X(192, 48, 512) X(192, 32, 256) \
   X(288, 32, 256) X(288, 32, 512)
 
+// four-pass form (contiguous row blocks, y and z transforms in their own passes): (NX, rows, NT)
+#define PRR_LIST(X) X(24, 8, 256) X(24, 6, 256) X(576, 16, 512) X(576, 8, 256) X(512, 16, 512) X(512, 8, 256)
+
 // (B, NZ, T, NT): b-part of the y split, z length, x tile, threads per block
 #define PK_LIST(X) \
   X(1, 18, 8, 256) X(1, 24, 8, 256) X(1, 24, 16, 256) X(2, 24, 8, 256) X(4, 24, 8, 256) X(1, 20, 8, 256) \
@@ -28,7 +31,8 @@ constexpr int kNT = 256;
   X(2, 128, 32, 256) X(4, 128, 16, 256) X(6, 128, 8, 256) X(4, 128, 8, 256) \
   X(4, 160, 16, 256) X(4, 160, 16, 512) X(5, 160, 8, 256) X(8, 160, 8, 256) X(4, 160, 8, 256) \
   X(2, 160, 32, 512) X(2, 160, 32, 256) \
-  X(9, 160, 4, 256)
+  X(9, 160, 4, 256) \
+  X(1, 320, 16, 256) X(1, 320, 32, 512) X(1, 256, 16, 256) X(1, 256, 32, 512)
 
 // vector runs (three field components per block): (B, NZ, T, NT)
 #define PKV_LIST(X) \
@@ -37,7 +41,8 @@ constexpr int kNT = 256;
   X(1, 96, 16, 256) X(2, 96, 16, 512) X(3, 96, 8, 256) \
   X(2, 128, 16, 512) X(4, 128, 8, 512) X(2, 128, 8, 256) \
   X(2, 135, 16, 512) X(2, 135, 8, 256) X(4, 135, 8, 512) \
-  X(2, 160, 8, 512) X(4, 160, 8, 512) X(4, 160, 8, 1024) X(2, 160, 16, 1024)
+  X(2, 160, 8, 512) X(4, 160, 8, 512) X(4, 160, 8, 1024) X(2, 160, 16, 1024) \
+  X(1, 320, 8, 256) X(1, 320, 16, 512) X(1, 256, 16, 512)
 
 #define RW_LIST(X) \
   X(16) X(18) X(20) X(24) X(25) X(27) X(30) X(32) X(36) X(40) X(48) X(64) X(96) X(128) X(135) X(160) X(192) \
@@ -48,23 +53,27 @@ constexpr int kNT = 256;
   X(256) X(288) X(320) X(512) X(576) X(1024)
 
 constexpr int rows_for(int n) { return n <= 64 ? 16 : (n <= 256 ? 8 : (n <= 576 ? 4 : 2)); }
-constexpr int strided_t(int n) { return n <= 512 ? 16 : 8; }
+constexpr int strided_t(int n) { return n <= 640 ? 16 : 8; }   // tile fits 2 blocks per CU
 
-#define PR_ENTRY(NX, A, NT) {&pass_real<NX, A, NT>, NX, A, NT, (int)(A * row_pitch(NX) * sizeof(float2))},
+// LDS of pass_real: the tile plus its per-axis tables (x: twiddle, ramp, mask; rows: twiddle,
+// ramp, mask, cross twiddle)
+constexpr int pr_lds(int nx, int a) { return a * row_pitch(nx) * 8 + nx * 20 + a * 28; }
+#define PR_ENTRY(NX, A, NT) {&pass_real<NX, A, NT, 0>, NX, A, NT, pr_lds(NX, A), 0},
+#define PRR_ENTRY(NX, A, NT) {&pass_real<NX, A, NT, 1>, NX, A, NT, pr_lds(NX, A), 1},
 #define PK_ENTRY(B, NZ, T, NT) {&pass_kspace<B, NZ, T, NT, 1>, B, NZ, T, NT, (int)(NZ * B * T * sizeof(float2)), 1},
 #define PKV_ENTRY(B, NZ, T, NT) \
   {&pass_kspace<B, NZ, T, NT, 3>, B, NZ, T, NT, (int)(3 * NZ * B * T * sizeof(float2)), 3},
 #define RW_ENTRY(N) {&fft_rows<N, rows_for(N), kNT>, N, rows_for(N), kNT},
 #define ST_ENTRY(N) {&fft_strided<N, strided_t(N), kNT>, N, strided_t(N), kNT},
 
-static const PassRInfo kPR[] = {PR_LIST(PR_ENTRY)};
+static const PassRInfo kPR[] = {PR_LIST(PR_ENTRY) PRR_LIST(PRR_ENTRY)};
 static const PassKInfo kPK[] = {PK_LIST(PK_ENTRY) PKV_LIST(PKV_ENTRY)};
 static const RowsInfo kRW[] = {RW_LIST(RW_ENTRY)};
 static const StridedInfo kST[] = {ST_LIST(ST_ENTRY)};
 
 const PassRInfo* find_pass_real(int nx, int a) {
   for (const auto& e : kPR)
-    if (e.nx == nx && e.a == a) return &e;
+    if (e.nx == nx && e.a == a && !e.rows) return &e;
   return nullptr;
 }
 const PassKInfo* find_pass_kspace(int b, int nz, int t, int pole) {

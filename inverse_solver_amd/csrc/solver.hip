@@ -81,7 +81,7 @@ static int pos2freq(int n, int p) {
 // ------------------------------------------------------------------ table set
 struct TableSet {
   int a = 0, b = 0;
-  DevBuf<float2> tw_x, tw_a, tw_z, cross, ramp_x, ramp_y, ramp_z;
+  DevBuf<float2> tw_x, tw_a, tw_y, tw_z, cross, ramp_x, ramp_y, ramp_z;
   DevBuf<float> mask_x, mask_y, mask_z, green_x, green_y, green_z, rad_x, rad_y, rad_z;
   Tables view() const {
     Tables t;
@@ -199,6 +199,10 @@ struct Solver {
   void make_plan() {
     Plan p;
     p.form = opt.form;
+    if (p.form == CBS_FORM_ROWS) {
+      if (make_rows_plan(p)) return;
+      throw CbsError(CBS_ERR_UNSUPPORTED, "no four-pass (row block) kernels for grid " + shape_str());
+    }
     if (p.form == CBS_FORM_FUSED) {
       const PassRInfo* prs = nullptr;
       const int npr = list_pass_real(&prs);
@@ -244,7 +248,8 @@ struct Solver {
       if (opt.split_a > 0 || opt.tile_t > 0 || opt.threads_real > 0 || opt.threads_kspace > 0)
         throw CbsError(CBS_ERR_UNSUPPORTED,
                        "no fused kernel plan for the forced split/tile/threads on grid " + shape_str());
-      p.form = CBS_FORM_PLAIN;   // grids without a fused plan run the per-axis form
+      if (make_rows_plan(p)) return;   // then the four-pass form
+      p.form = CBS_FORM_PLAIN;         // grids without either run the per-axis form
     }
     p.rw = find_fft_rows(nx);
     p.sy = find_fft_strided(ny, 0);
@@ -257,7 +262,43 @@ struct Solver {
     p.t = p.sy->t;
     plan = p;
   }
+  // four-pass form: the real-space pass on blocks of contiguous rows, strided y
+  // transforms, the k-space pass with b = 1 on the natural layout
+  bool make_rows_plan(Plan& p) {
+    const PassRInfo* prs = nullptr;
+    const int npr = list_pass_real(&prs);
+    const PassKInfo* pks = nullptr;
+    const int npk = list_pass_kspace(&pks);
+    const StridedInfo* sy = find_fft_strided(ny, 0);
+    if (!sy || nx % 2 != 0) return false;
+    const PassRInfo* br = nullptr;
+    const PassKInfo* bk = nullptr;
+    for (int i = 0; i < npr; ++i) {
+      const PassRInfo& r = prs[i];
+      if (!r.rows || r.nx != nx || ny % r.a != 0) continue;
+      if (opt.threads_real > 0 && r.nt != opt.threads_real) continue;
+      if (!br || (r.lds <= 81920 && (br->lds > 81920 || r.a > br->a))) br = &r;
+    }
+    for (int j = 0; j < npk; ++j) {
+      const PassKInfo& q = pks[j];
+      if (q.b != 1 || q.nz != nz || q.pole != P) continue;
+      if (opt.tile_t > 0 && q.t != opt.tile_t) continue;
+      if (opt.threads_kspace > 0 && q.nt != opt.threads_kspace) continue;
+      if (!bk || (q.lds <= 81920 && (bk->lds > 81920 || q.t > bk->t))) bk = &q;
+    }
+    if (!br || !bk) return false;
+    p.form = CBS_FORM_ROWS;
+    p.pr = br;
+    p.pk = bk;
+    p.sy = sy;
+    p.a = br->a;          // rows per real-space block
+    p.b = ny / br->a;     // row blocks per z plane
+    p.t = bk->t;
+    plan = p;
+    return true;
+  }
   bool fused_ok_r(const PassRInfo& r) const {
+    if (r.rows) return false;
     if (r.nx != nx || ny % r.a != 0 || nx % 2 != 0) return false;   // pairs of x per 16-byte access
     if (opt.split_a > 0 && r.a != opt.split_a) return false;
     if (opt.threads_real > 0 && r.nt != opt.threads_real) return false;
@@ -280,6 +321,7 @@ struct Solver {
     T.a = a; T.b = b;
     T.tw_x.upload(twiddles(nx));
     T.tw_a.upload(twiddles(a));
+    if (plan.form == CBS_FORM_ROWS) T.tw_y.upload(twiddles(ny));
     T.tw_z.upload(twiddles(nz));
     {
       std::vector<float2> cr((size_t)a * b);
@@ -351,8 +393,9 @@ struct Solver {
       for (int f = 0; f < 2; ++f)
         for (int k2 = 0; k2 < b; ++k2)
           for (int p1 = 0; p1 < a; ++p1) {
-            const int ky = (plan.form == CBS_FORM_FUSED) ? pos2freq(a, p1) + a * k2 : pos2freq(ny, p1);
-            gy[(size_t)f * ny + (size_t)k2 * a + p1] = (float)gfl(1, ny, f, ky);
+            const int idx = k2 * a + p1;   // plain and four-pass forms: position of the strided y transform
+            const int ky = (plan.form == CBS_FORM_FUSED) ? pos2freq(a, p1) + a * k2 : pos2freq(ny, idx);
+            gy[(size_t)f * ny + idx] = (float)gfl(1, ny, f, ky);
           }
       T.green_y.upload(gy);
       std::vector<float> gz(2 * (size_t)nz, 0.f);
@@ -366,8 +409,9 @@ struct Solver {
         for (int p = 0; p < nx; ++p) qx[(size_t)f * nx + p] = (float)rfl(0, nx, f, pos2freq(nx, p));
         for (int k2 = 0; k2 < b; ++k2)
           for (int p1 = 0; p1 < a; ++p1) {
-            const int ky = (plan.form == CBS_FORM_FUSED) ? pos2freq(a, p1) + a * k2 : pos2freq(ny, p1);
-            qy[(size_t)f * ny + (size_t)k2 * a + p1] = (float)rfl(1, ny, f, ky);
+            const int idx = k2 * a + p1;
+            const int ky = (plan.form == CBS_FORM_FUSED) ? pos2freq(a, p1) + a * k2 : pos2freq(ny, idx);
+            qy[(size_t)f * ny + idx] = (float)rfl(1, ny, f, ky);
           }
         for (int p = 0; p < nz; ++p) qz[(size_t)f * nz + p] = (float)rfl(2, nz, f, pos2freq(nz, p));
       }
@@ -453,6 +497,55 @@ struct Solver {
     pass_r(UPD_NONE, FWD_P0, 0, 1);
     for (int jj = 1; jj <= M; ++jj) {
       pass_k(jj);
+      const int fwd = fwd_of(jj, M);
+      pass_r(upd_of(jj), fwd, jj, fwd == FWD_NONE ? 0 : jj + 1);
+    }
+    HIP_CHECK(hipGetLastError());
+  }
+
+  // four-pass schedule for one batch already seeded in fstate: the real-space pass on
+  // row blocks (x transforms + pointwise), the forward y transform, the k-space pass
+  // (z transforms around the Green's multiply, b = 1), the inverse y transform
+  void run_rows(int nb, int M) {
+    const PassRInfo* pr = plan.pr;
+    const PassKInfo* pk = plan.pk;
+    const StridedInfo* sy = plan.sy;
+    const int nv = nb * P;
+    const Consts c = consts(nv);
+    Consts ck = consts(nb);
+    ck.a = ny;   // k-space pass: every y is its own row, natural layout
+    ck.b = 1;
+    const Tables tb = tabs.view();
+    auto pass_r = [&](int upd, int fwd, int jj_inv, int jj_fwd) {
+      PassRArgs args;
+      args.tb = tb; args.c = c; args.st = state_args(nb);
+      if (opt.track_norms && jj_inv > 0) args.st.norms = norms.p + (size_t)(jj_inv - 1) * nv;
+      args.upd = upd; args.fwd = fwd; args.jj_inv = jj_inv; args.jj_fwd = jj_fwd;
+      args.order = 0;
+      const dim3 grid((unsigned)((long long)nv * plan.b * nz));
+      timed(0, [&] { hipLaunchKernelGGL(pr->fn, grid, dim3(pr->nt), 0, stream, args); });
+    };
+    auto pass_k = [&](int jj) {
+      PassKArgs args;
+      args.tb = tb; args.c = ck; args.spec = spec.p; args.parity = parity.p; args.rhs_stride = nvox; args.jj = jj;
+      args.order = 0;
+      const int ntile = (nx + pk->t - 1) / pk->t;
+      const dim3 grid((unsigned)((long long)nb * ny * ntile));
+      timed(1, [&] { hipLaunchKernelGGL(pk->fn, grid, dim3(pk->nt), 0, stream, args); });
+    };
+    auto fft_y = [&](int inverse) {
+      // all z planes of all component slots: consecutive planes are nx * ny apart
+      const int nt = (nx + sy->t - 1) / sy->t;
+      const long long nouter = (long long)nz * nv;
+      const dim3 g((unsigned)(nt * nouter));
+      timed(2, [&] { hipLaunchKernelGGL(sy->fn, g, dim3(sy->nt), 0, stream, spec.p, nx, (long long)nx,
+                                        (long long)nx * ny, (int)nouter, tabs.tw_y.p, inverse); });
+    };
+    pass_r(UPD_NONE, FWD_P0, 0, 1);
+    for (int jj = 1; jj <= M; ++jj) {
+      fft_y(0);
+      pass_k(jj);
+      fft_y(1);
       const int fwd = fwd_of(jj, M);
       pass_r(upd_of(jj), fwd, jj, fwd == FWD_NONE ? 0 : jj + 1);
     }
@@ -565,7 +658,7 @@ struct Solver {
     }
     if (M == 0) {
       HIP_CHECK(hipMemsetAsync(field.p, 0, (size_t)nv * nvox * sizeof(float2), stream));
-    } else if (plan.form == CBS_FORM_FUSED) {
+    } else if (plan.form == CBS_FORM_FUSED || plan.form == CBS_FORM_ROWS) {
       if (opt.use_graph && !opt.time_kernels) {
         // replay a captured iteration loop (the kernel arguments, including the
         // potential's scalars, are baked in; the cache is dropped whenever the
@@ -576,7 +669,8 @@ struct Solver {
         if (!graph_exec || key != graph_key) {
           drop_graph();
           HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-          run_fused(nb, M);
+          if (plan.form == CBS_FORM_ROWS) run_rows(nb, M);
+          else run_fused(nb, M);
           hipError_t e = hipStreamEndCapture(stream, &graph);
           if (e != hipSuccess) {
             graph = nullptr;
@@ -586,6 +680,8 @@ struct Solver {
           graph_key = key;
         }
         HIP_CHECK(hipGraphLaunch(graph_exec, stream));
+      } else if (plan.form == CBS_FORM_ROWS) {
+        run_rows(nb, M);
       } else {
         run_fused(nb, M);
       }
@@ -1134,7 +1230,8 @@ int cbs_set_options(cbs_handle h, const cbs_options* opt) {
   return guarded([&] {
     Solver* s = lookup(h);
     require(opt != nullptr, CBS_ERR_ARG, "null options");
-    require(opt->form == CBS_FORM_FUSED || opt->form == CBS_FORM_PLAIN, CBS_ERR_ARG, "unknown form");
+    require(opt->form == CBS_FORM_FUSED || opt->form == CBS_FORM_PLAIN || opt->form == CBS_FORM_ROWS, CBS_ERR_ARG,
+            "unknown form");
     require(opt->max_batch >= 0 && opt->split_a >= 0 && opt->tile_t >= 0, CBS_ERR_ARG, "negative option");
     require(opt->block_order >= 0 && opt->block_order <= 3, CBS_ERR_ARG, "block_order must be 0..3");
     DeviceGuard dg(s->device);

@@ -33,6 +33,8 @@ CASES = [
     ((16, 12, 10), 12, dict(form=1)),
     ((17, 17, 17), 10, dict(form=0)),
     ((17, 17, 19), 10, dict(form=0)),
+    ((16, 16, 16), 20, dict(form=2)),            # four-pass form (row blocks, y and z passes)
+    ((16, 16, 16), 9, dict(form=2, threads_real=256)),
 ]
 
 

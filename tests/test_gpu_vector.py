@@ -27,6 +27,7 @@ CASES = [
     ((16, 16, 16), 12, dict(form=0, split_a=6)),
     ((16, 12, 10), 10, dict(form=0)),
     ((17, 17, 19), 8, dict(form=0)),
+    ((16, 16, 16), 9, dict(form=2)),
 ]
 
 
