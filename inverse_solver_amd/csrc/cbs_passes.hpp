@@ -380,17 +380,56 @@ __global__ __launch_bounds__(NT) void fft_strided(float2* data, int nx, long lon
   const int o = blockIdx.x / ntile;
   const int tvalid = nx - xt * T < T ? nx - xt * T : T;
   float2* base = data + (long long)o * os + (long long)xt * T;
-  for (int i = threadIdx.x; i < N * T; i += NT) {
-    const int e = i / T, t = i - e * T;
-    lds[i] = t < tvalid ? base[e * es + t] : make_float2(0.f, 0.f);
+  auto ad = [](int l, int e) { return e * T + l; };
+  // even x extent (and so even strides and tile origins): 16-byte pair accesses,
+  // all of a thread's loads issued before the LDS writes
+  const bool pairs = (T % 2 == 0) && (nx % 2 == 0) && (es % 2 == 0) && (os % 2 == 0);
+  constexpr int TP = T / 2;
+  constexpr int LP = (N * TP + NT - 1) / NT;
+  if (pairs) {
+    float4 reg[LP];
+#pragma unroll
+    for (int k = 0; k < LP; ++k) {
+      const int i = threadIdx.x + k * NT;
+      if (i < N * TP) {
+        const int e = i / TP, tp = i - e * TP;
+        reg[k] = 2 * tp < tvalid ? *reinterpret_cast<const float4*>(base + e * es + 2 * tp)
+                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < LP; ++k) {
+      const int i = threadIdx.x + k * NT;
+      if (i < N * TP) {
+        const int e = i / TP, tp = i - e * TP;
+        lds[e * T + 2 * tp] = lo2(reg[k]);
+        lds[e * T + 2 * tp + 1] = hi2(reg[k]);
+      }
+    }
+  } else {
+    for (int i = threadIdx.x; i < N * T; i += NT) {
+      const int e = i / T, t = i - e * T;
+      lds[i] = t < tvalid ? base[e * es + t] : make_float2(0.f, 0.f);
+    }
   }
   __syncthreads();
-  auto ad = [](int l, int e) { return e * T + l; };
   if (inverse) line_fft<N, true, T, NT>(lds, ad, tw, threadIdx.x);
   else line_fft<N, false, T, NT>(lds, ad, tw, threadIdx.x);
-  for (int i = threadIdx.x; i < N * T; i += NT) {
-    const int e = i / T, t = i - e * T;
-    if (t < tvalid) base[e * es + t] = lds[i];
+  if (pairs) {
+#pragma unroll
+    for (int k = 0; k < LP; ++k) {
+      const int i = threadIdx.x + k * NT;
+      if (i < N * TP) {
+        const int e = i / TP, tp = i - e * TP;
+        if (2 * tp < tvalid)
+          *reinterpret_cast<float4*>(base + e * es + 2 * tp) = pack2(lds[e * T + 2 * tp], lds[e * T + 2 * tp + 1]);
+      }
+    }
+  } else {
+    for (int i = threadIdx.x; i < N * T; i += NT) {
+      const int e = i / T, t = i - e * T;
+      if (t < tvalid) base[e * es + t] = lds[i];
+    }
   }
 }
 

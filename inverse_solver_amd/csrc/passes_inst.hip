@@ -53,7 +53,7 @@ constexpr int kNT = 256;
   X(256) X(288) X(320) X(512) X(576) X(1024)
 
 constexpr int rows_for(int n) { return n <= 64 ? 16 : (n <= 256 ? 8 : (n <= 576 ? 4 : 2)); }
-constexpr int strided_t(int n) { return n <= 640 ? 16 : 8; }   // tile fits 2 blocks per CU
+constexpr int strided_t(int n) { return n <= 512 ? 16 : 8; }
 
 // LDS of pass_real: the tile plus its per-axis tables (x: twiddle, ramp, mask; rows: twiddle,
 // ramp, mask, cross twiddle)
