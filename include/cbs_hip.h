@@ -161,6 +161,23 @@ int cbs_tv_prox_complex(cbs_handle h, const void* in, void* out, const uint32_t 
 int cbs_field_gradient(cbs_handle h, const void* u, const void* w, uint32_t n_rhs, const uint32_t dims[3], void* grad,
                        int mem);
 
+/* Phase unwrapping for the Rytov initialiser of BACKWARD_SOLVER_MULTI (handle-free;
+ * synchronous on the default stream of `device`).  Arrays are column-major n1 x n2
+ * phase maps, n3 of them (one per angle), single precision.
+ *
+ * Residue map of every 2x2 loop, (n1-1) x (n2-1) x n3 values in {-1, 0, 1}.
+ * Replaces get_residue_gpu (Codes/sub_code/goldstein_GPU/get_residue_mex.cpp,
+ * get_residue_cuda.cu:35-66). */
+int cbs_phase_residues(int device, const float* wrapped, float* residues, const uint32_t dims[3], int mem);
+
+/* Raster unwrap: the wrapped differences are integrated down the first column
+ * (minus 2 pi step2[n2 x n3]) and then along every row (minus 2 pi
+ * step1[n1 x n2 x n3]); zero steps unwrap a residue-free map.  Replaces
+ * unwrapp_phase_goldstein_gpu (goldstein_GPU/unwrapp_phase_goldstein_mex.cpp,
+ * unwrapp_phase_goldstein_cuda.cu:60-113).  unwrapped may alias wrapped. */
+int cbs_phase_integrate(int device, const float* wrapped, const int32_t* step1, const int32_t* step2, float* unwrapped,
+                        const uint32_t dims[3], int mem);
+
 /* FP64 per-iteration |f_n|^2 of the last solve (track_norms=1): out[r * n_iter + j]. */
 int cbs_get_norms(cbs_handle h, double* out, uint32_t n_rhs, uint32_t n_iter);
 

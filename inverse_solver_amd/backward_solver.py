@@ -108,13 +108,15 @@ class BACKWARD_SOLVER_MULTI:
                                         np.asarray(output_field)[:, :, :, scan])
 
     def solve(self, input_field, output_field, init_RI=None, init_solver=None):
-        """:92-196.  The reference starts from init_solver.solve(input, output) (a Rytov
-        reconstruction); pass init_RI (or an object with .solve) instead."""
+        """:92-196.  The reference starts from init_solver.solve(input, output), a Rytov
+        reconstruction (inverse_solver_amd.rytov.BACKWARD_SOLVER_RYTOV); init_RI skips it."""
         p = self.parameters
         if init_RI is None:
             if init_solver is None:
                 raise ValueError('give init_RI or an init_solver with .solve(input_field, output_field)')
             init_RI = init_solver.solve(input_field, output_field)
+            if isinstance(init_RI, tuple):          # BACKWARD_SOLVER_RYTOV returns (RI, ORytov)
+                init_RI = init_RI[0]
         RI = np.asarray(init_RI).astype(np.complex128)
         n_m = p['RI_bg']
         alpha = 1.0 / p['step']

@@ -18,6 +18,7 @@
 #include "io_kernels.hpp"
 #include "kernels_misc.hpp"
 #include "tv_kernels.hpp"
+#include "unwrap_kernels.hpp"
 
 namespace cbs {
 
@@ -1171,6 +1172,81 @@ static int guarded(F&& f) {
 using namespace cbs;
 
 extern "C" {
+
+// ------------------------------------------------------------------ phase unwrapping (handle-free)
+namespace {
+struct DevSel {
+  int prev = 0;
+  explicit DevSel(int d) {
+    HIP_CHECK(hipGetDevice(&prev));
+    HIP_CHECK(hipSetDevice(d));
+  }
+  ~DevSel() { (void)hipSetDevice(prev); }
+};
+}  // namespace
+
+int cbs_phase_residues(int device, const float* wrapped, float* residues, const uint32_t dims[3], int mem) {
+  return guarded([&] {
+    require(wrapped && residues && dims, CBS_ERR_ARG, "null argument");
+    require(dims[0] >= 2 && dims[1] >= 2 && dims[2] >= 1, CBS_ERR_ARG, "phase maps need at least 2 x 2 pixels");
+    require(mem == CBS_MEM_HOST || mem == CBS_MEM_DEVICE, CBS_ERR_ARG, "mem must be CBS_MEM_HOST or CBS_MEM_DEVICE");
+    DevSel ds(device);
+    const long long n = (long long)dims[0] * dims[1] * dims[2];
+    const long long m = (long long)(dims[0] - 1) * (dims[1] - 1) * dims[2];
+    DevBuf<float> bp, br;
+    const float* dp = wrapped;
+    float* dr = residues;
+    if (mem == CBS_MEM_HOST) {
+      bp.ensure((size_t)n);
+      br.ensure((size_t)m);
+      HIP_CHECK(hipMemcpy(bp.p, wrapped, n * sizeof(float), hipMemcpyHostToDevice));
+      dp = bp.p;
+      dr = br.p;
+    }
+    const unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>((m + 255) / 256, 1 << 20));
+    hipLaunchKernelGGL(phase_residues, dim3(grid), dim3(256), 0, 0, dp, dr, (int)dims[0], (int)dims[1], (int)dims[2]);
+    HIP_CHECK(hipGetLastError());
+    if (mem == CBS_MEM_HOST) HIP_CHECK(hipMemcpy(residues, br.p, m * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipDeviceSynchronize());
+  });
+}
+
+int cbs_phase_integrate(int device, const float* wrapped, const int32_t* step1, const int32_t* step2, float* unwrapped,
+                        const uint32_t dims[3], int mem) {
+  return guarded([&] {
+    require(wrapped && step1 && step2 && unwrapped && dims, CBS_ERR_ARG, "null argument");
+    require(dims[0] >= 1 && dims[1] >= 1 && dims[2] >= 1, CBS_ERR_ARG, "empty phase map");
+    require(mem == CBS_MEM_HOST || mem == CBS_MEM_DEVICE, CBS_ERR_ARG, "mem must be CBS_MEM_HOST or CBS_MEM_DEVICE");
+    DevSel ds(device);
+    const long long n = (long long)dims[0] * dims[1] * dims[2];
+    const long long n23 = (long long)dims[1] * dims[2];
+    DevBuf<float> bp;
+    DevBuf<int32_t> b1, b2;
+    float* dp = unwrapped;
+    const int32_t* d1 = step1;
+    const int32_t* d2 = step2;
+    if (mem == CBS_MEM_HOST) {
+      bp.ensure((size_t)n);
+      b1.ensure((size_t)n);
+      b2.ensure((size_t)n23);
+      HIP_CHECK(hipMemcpy(bp.p, wrapped, n * sizeof(float), hipMemcpyHostToDevice));
+      HIP_CHECK(hipMemcpy(b1.p, step1, n * sizeof(int32_t), hipMemcpyHostToDevice));
+      HIP_CHECK(hipMemcpy(b2.p, step2, n23 * sizeof(int32_t), hipMemcpyHostToDevice));
+      dp = bp.p;
+      d1 = b1.p;
+      d2 = b2.p;
+    } else if (unwrapped != wrapped) {
+      HIP_CHECK(hipMemcpy(unwrapped, wrapped, n * sizeof(float), hipMemcpyDeviceToDevice));
+    }
+    hipLaunchKernelGGL(phase_integrate_column, dim3((dims[2] + 63) / 64), dim3(64), 0, 0, dp, d2, (int)dims[0],
+                       (int)dims[1], (int)dims[2]);
+    hipLaunchKernelGGL(phase_integrate_rows, dim3((unsigned)((n23 + 127) / 128)), dim3(128), 0, 0, dp, d1,
+                       (int)dims[0], (int)dims[1], (int)dims[2]);
+    HIP_CHECK(hipGetLastError());
+    if (mem == CBS_MEM_HOST) HIP_CHECK(hipMemcpy(unwrapped, bp.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipDeviceSynchronize());
+  });
+}
 
 int cbs_abi_version(void) { return CBS_ABI_VERSION; }
 
