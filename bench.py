@@ -138,7 +138,8 @@ def workload_c1(args):
 def workload_inverse(args):
     """C3: RI 96x96x97 (+16/16/19 px -> 128x128x135 sim), 5 seeded spheres,
     24 circle angles at NA 1.2, measured fields from the GPU forward solve plus
-    10 % complex Gaussian noise; BACKWARD_SOLVER_MULTI with the device TV prox."""
+    10 % complex Gaussian noise; BACKWARD_SOLVER_MULTI from the Rytov reconstruction, with
+    the device TV prox."""
     from inverse_solver_amd import FORWARD_SOLVER_CONVERGENT_BORN
     from inverse_solver_amd.backward_solver import BACKWARD_SOLVER_MULTI, get_default_parameters
     from inverse_solver_amd.phantoms import spheres, circle_illumination
@@ -162,10 +163,17 @@ def workload_inverse(args):
     b = BACKWARD_SOLVER_MULTI(bp)
     b._share_io()          # the 2D projection kernels are built once, as the reference's constructor does
     t0 = time.perf_counter()
-    # Initial guess: the phantom low-passed (sigma 2 px) at 70 % contrast, a stand-in for
-    # the Rytov reconstruction the reference starts from (a homogeneous start has eps = 0).
-    from scipy.ndimage import gaussian_filter
-    init = 1.336 + 0.7 * gaussian_filter(np.real(RI_true) - 1.336, 2.0)
+    # Initial guess: the Rytov reconstruction from the same fields, as BACKWARD_SOLVER_MULTI
+    # starts (:94); --init lowpass uses the phantom low-passed at 70 % contrast instead.
+    t_r = time.perf_counter()
+    if args.init == 'rytov':
+        from inverse_solver_amd.rytov import BACKWARD_SOLVER_RYTOV, get_default_parameters as rytov_defaults
+        rp = rytov_defaults(dict(size=list(ri), wavelength=0.532, NA=1.2, RI_bg=1.336, resolution=[0.1] * 3))
+        init = np.real(BACKWARD_SOLVER_RYTOV(rp).solve(u, meas)[0])
+    else:
+        from scipy.ndimage import gaussian_filter
+        init = 1.336 + 0.7 * gaussian_filter(np.real(RI_true) - 1.336, 2.0)
+    rytov_s = time.perf_counter() - t_r
     RI = b.solve(u, meas, init_RI=init)
     wall = time.perf_counter() - t0
     steady = b.timings[1:] if len(b.timings) > 1 else b.timings    # the first iteration allocates buffers
@@ -182,6 +190,7 @@ def workload_inverse(args):
                   'tv_prox_seconds': round(tv_t, 3), 'wall_seconds': round(wall, 2),
                   'misfit_first_last': [b.err_list[0], b.err_list[-1]],
                   'ri_rel_error_vs_phantom': err,
+                  'initial_guess': args.init, 'initial_guess_seconds': round(rytov_s, 2),
                   'ri_rel_error_of_initial_guess': float(np.sum(np.abs(init - RI_true) ** 2)
                                                          / np.sum(np.abs(1.336 - RI_true) ** 2))},
                  hib=False)
@@ -504,6 +513,8 @@ def main():
     ap.add_argument('--rate', type=float, default=4.0, help='serving: Poisson arrival rate (requests/s)')
     ap.add_argument('--outer', type=int, default=20, help='inverse: outer iterations')
     ap.add_argument('--inner', type=int, default=100, help='inverse: TV inner iterations')
+    ap.add_argument('--init', default='rytov', choices=['rytov', 'lowpass'],
+                    help='inverse: initial guess (Rytov reconstruction, or the low-passed phantom)')
     args = ap.parse_args()
 
     if args.gpus != 1:

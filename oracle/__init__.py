@@ -37,6 +37,14 @@ the checks listed below instead of by reference-produced vectors:
               is pinned to the formula only); the TV prox by its special cases
               (lambda -> 0, constants are fixed points, objective decrease).
 
+  rytov.py    BACKWARD_SOLVER_RYTOV, vector2scalarfield, the goldstein_GPU residue and
+              raster-unwrap kernels
+              pinned by a phase vortex (one residue, winding sign), smooth-map
+              unwrapping to the true phase, step-map closed forms, a weak bead
+              reconstructed from CBS-oracle fields, and the no-scatterer case.
+              The branch-cut placement is not restated (maps with residues are
+              refused): unwrapping with residues is parity unpinned.
+
 The seeded input builders (phantoms, illumination) live in the product package
 (inverse_solver_amd/phantoms.py) and are pinned by closed forms in
 tests/test_phantoms.py.
