@@ -98,8 +98,10 @@ def workload_c1(args):
     p.update(return_reflection=True)
     RI = bead_ri(ri, 30, 1.46, 1.336)
     u = circle_illumination(ri[:2], 1, 1.2, 0.532, [0.1] * 3)
-    s = FORWARD_SOLVER_CONVERGENT_BORN(p, options=dict(time_kernels=1))
+    s = FORWARD_SOLVER_CONVERGENT_BORN(p)        # production settings: the iteration loop replays a hipGraph
     s.set_RI(RI)
+    if not args.no_tune:
+        s.autotune(1, 6)                         # one angle: the plan that fills the device best
     s.solve(u)                                   # kernels, plans, warm-up
     times = []
     for _ in range(max(args.steps, 3)):

@@ -944,7 +944,19 @@ struct Solver {
     Plan best_plan;
     int best_order = 0;
     opt.block_order = 0;
+    auto time_plan = [&]() {
+      born_batch(src.p, dst.p, nb, 0);      // warm-up (and graph capture)
+      HIP_CHECK(hipEventRecord(e0, stream));
+      born_batch(src.p, dst.p, nb, 0);
+      HIP_CHECK(hipEventRecord(e1, stream));
+      HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      return (double)ms;
+    };
     try {
+      // every kernel pair in block order 0, then the block orders of the three fastest
+      std::vector<std::pair<double, Plan>> timed_plans;
       for (auto& cd : cands) {
         Plan p;
         p.form = CBS_FORM_FUSED;
@@ -955,37 +967,25 @@ struct Solver {
         p.t = cd.second->t;
         plan = p;
         build_tables();
-        born_batch(src.p, dst.p, nb, 0);      // warm-up
-        HIP_CHECK(hipEventRecord(e0, stream));
-        born_batch(src.p, dst.p, nb, 0);
-        HIP_CHECK(hipEventRecord(e1, stream));
-        HIP_CHECK(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) {
-          best = ms;
-          best_plan = p;
-        }
+        timed_plans.emplace_back(time_plan(), p);
       }
-      // then the block orders of the two passes for the chosen kernels
-      plan = best_plan;
-      build_tables();
-      double best_o = 1e30;
-      for (int ord = 0; ord < 4; ++ord) {
-        opt.block_order = ord;
-        born_batch(src.p, dst.p, nb, 0);      // warm-up
-        HIP_CHECK(hipEventRecord(e0, stream));
-        born_batch(src.p, dst.p, nb, 0);
-        HIP_CHECK(hipEventRecord(e1, stream));
-        HIP_CHECK(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best_o) {
-          best_o = ms;
-          best_order = ord;
+      std::sort(timed_plans.begin(), timed_plans.end(),
+                [](const std::pair<double, Plan>& x, const std::pair<double, Plan>& y) { return x.first < y.first; });
+      const size_t ntop = std::min<size_t>(3, timed_plans.size());
+      for (size_t i = 0; i < ntop; ++i) {
+        plan = timed_plans[i].second;
+        build_tables();
+        for (int ord = 0; ord < 4; ++ord) {
+          opt.block_order = ord;
+          const double ms = ord == 0 ? timed_plans[i].first : time_plan();
+          if (ms < best) {
+            best = ms;
+            best_plan = plan;
+            best_order = ord;
+          }
         }
+        opt.block_order = 0;
       }
-      best = best_o;
     } catch (...) {
       prm.Bornmax = saved_iters;
       flip_state = saved_flip;
