@@ -53,7 +53,10 @@ typedef struct {
   uint32_t field_size[3];      /* xy size of the 2D fields; [2] ignored */
   uint32_t conv_size[3];       /* xy convolution grid (field + RI); [2] ignored */
   int32_t device;              /* HIP device ordinal */
-  int32_t RI_center[2];        /* xy offset of the RI grid in the field (RI_center, :40-41) */
+  int32_t RI_center[2];        /* xy offset of the RI grid in the field (RI_center, :40-41); the
+                                  I/O kernels honour it (:77-106), but cbs_set_io_kernels refuses
+                                  values other than 0, as the reference's crop_field2RI (:173-179)
+                                  stops every 2D solve with such a value */
   int32_t pole_num;            /* field components: 1 scalar, 3 vector with the dyadic Green's
                                   function (FORWARD_SOLVER_CONVERGENT_BORN.m:294-297; 0 means 1).
                                   With 3, every per-rhs array holds the x, y, z components in turn:

@@ -757,7 +757,10 @@ struct Solver {
     require(fx == rx && fy == ry, CBS_ERR_ARG,
             "2D projections need the field xy size equal to the RI xy size (as the reference's crop_field2RI assumes)");
     require(cx >= fx && cy >= fy, CBS_ERR_ARG, "conv_size must be at least field_size");
-    if (cx == fx) require(rc_x == 0 && rc_y == 0, CBS_ERR_ARG, "RI_center must be 0 with a cyclic xy boundary");
+    // FORWARD_SOLVER_CONVERGENT_BORN.m:173-179 crops parameters.size out of an array of that same size,
+    // offset by RI_center, so every 2D solve with RI_center other than [0 0] stops there (:554)
+    require(rc_x == 0 && rc_y == 0, CBS_ERR_ARG,
+            "RI_center other than [0 0]: the reference's crop_field2RI indexes outside the field grid");
     io_rows = find_fft_rows(cx);
     io_cols = find_fft_strided(cy, 0);
     require(io_rows && io_cols, CBS_ERR_UNSUPPORTED,

@@ -184,6 +184,12 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
         if self._io_ready:
             return
         ref, kt, kr = build_io_kernels(self.parameters, self.expected_RI_size, self.cyclic_boundary_xy)
+        want = (self.conv_size[0], self.conv_size[1], self.expected_RI_size[2])
+        if ref.shape != want:
+            # a negative RI_center shrinks the cropped kernels (:82-85), and the refocusing
+            # product of :550 then stops on mismatched dimensions
+            raise ValueError('refocusing kernel is %s but the convolution grid is %s (RI_center %s)'
+                             % (ref.shape, want, list(self.parameters.get('RI_center', [0, 0]))))
         self._io_arrays = [np.asfortranarray(a.astype(np.complex64)) for a in (ref, kt, kr)]
         check(lib.cbs_set_io_kernels(self._h, *[_fptr(a) for a in self._io_arrays], _lib.CBS_MEM_HOST))
         self._io_ready = True

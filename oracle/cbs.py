@@ -147,10 +147,13 @@ class ForwardSolverConvergentBorn:
         return m[st[0]:st[0] + sz[0], st[1]:st[1] + sz[1]]
 
     def crop_field2RI(self, m):
-        """:173-179 (crops to parameters.size(1:2), offset by RI_center, as the reference does)."""
+        """:173-179 (crops to parameters.size(1:2), offset by RI_center, as the reference does;
+        MATLAB stops with an index error when the window leaves the array)."""
         sz = self.parameters['size']
         rc = self.parameters['RI_center']
         st = [m.shape[j] // 2 - sz[j] // 2 + rc[j] for j in range(2)]
+        if any(st[j] < 0 or st[j] + sz[j] > m.shape[j] for j in range(2)):
+            raise IndexError('crop_field2RI: RI_center moves the crop outside the array (index exceeds dimensions)')
         return m[st[0]:st[0] + sz[0], st[1]:st[1] + sz[1]]
 
     def crop_conv2RI(self, m):

@@ -24,7 +24,8 @@ def _setup(ri, cyclic=False, iters=16, A=3):
 
 
 @pytest.mark.parametrize('cyclic,opts', [(False, {}), (False, dict(max_batch=2)), (True, {}),
-                                         (False, dict(form=1))])
+                                         (False, dict(form=1)), (False, dict(form=2)),
+                                         (False, dict(form=2, max_batch=2))])
 def test_solve_matches_oracle(cyclic, opts):
     from inverse_solver_amd import FORWARD_SOLVER_CONVERGENT_BORN
     from oracle.cbs import ForwardSolverConvergentBorn as Oracle
@@ -39,6 +40,8 @@ def test_solve_matches_oracle(cyclic, opts):
     et, er, ef = rel(t, t_ref), rel(r, r_ref), rel(f, f_ref)
     print('cyclic', cyclic, opts, s.plan(), 'trans %.2e ref %.2e 3d %.2e' % (et, er, ef))
     assert t.shape == t_ref.shape and r.shape == r_ref.shape and f.shape == f_ref.shape
+    if opts.get('form') == 2:
+        assert s.plan()['form'] == 'rows'
     assert ef < 1e-5
     assert et < 1e-5
     assert er < 1e-4
@@ -57,4 +60,21 @@ def test_return_flags_and_repeat_solve():
     assert r1 is None and f1 is None and t1.shape == (16, 16, 1, 2)
     t2, _, _ = s.solve(u)       # even Bornmax: the shift state returns to its start
     np.testing.assert_allclose(t1, t2, rtol=0, atol=1e-6 * np.abs(t1).max())
+    s.close()
+
+
+@pytest.mark.parametrize('rc', [[1, 0], [0, 2], [0, -2]])
+def test_ri_center_refused_like_the_reference(rc):
+    """Every 2D solve of the reference stops when RI_center is not [0 0]: crop_field2RI
+    (:173-179, reached from :554) indexes outside the field grid for a positive offset, and
+    the refocusing product (:550) meets a smaller cropped kernel for a negative one."""
+    from inverse_solver_amd import FORWARD_SOLVER_CONVERGENT_BORN
+    from inverse_solver_amd._lib import CbsError
+    ri = (16, 16, 16)
+    p, RI, u = _setup(ri, A=1)
+    p['RI_center'] = rc
+    s = FORWARD_SOLVER_CONVERGENT_BORN(p)
+    s.set_RI(RI)
+    with pytest.raises((CbsError, ValueError)):
+        s.solve(u)
     s.close()

@@ -44,6 +44,8 @@ def test_vector_born_loop_matches_oracle(ri_size, iters, opts):
     got = s.solve_incident(np.stack(fields, axis=-1))
     err = rel(got, ref)
     print(ri_size, opts, s.plan(), 'vector rel err', err, flush=True)
+    if opts.get('form') == 2:
+        assert s.plan()['form'] == 'rows'
     assert got.shape == ref.shape
     assert err < TOL
     s.close()
@@ -73,20 +75,22 @@ def _circle_pol(ri, n_angles, pol_seed):
     return u[:, :, None, :] * w[None, None, :, :]                       # (x, y, 2, A)
 
 
-def test_vector_solve_2d_matches_oracle():
-    ri = (16, 16, 17)
+@pytest.mark.parametrize('ri,opts', [((16, 16, 17), {}), ((16, 16, 16), dict(form=2))])
+def test_vector_solve_2d_matches_oracle(ri, opts):
     p = params(ri, iterations=12, vector_simulation=True)
     RI = sphere_ri(ri, 0.5, n_s=1.41)
     u = _circle_pol(ri, 3, 1)
     o = Oracle(p)
     o.set_RI(RI)
     t_ref, r_ref, f_ref = o.solve(u)
-    s = _gpu()(p)
+    s = _gpu()(p, options=opts)
     s.set_RI(RI)
     t, r, f = s.solve(u)
+    if opts.get('form') == 2:
+        assert s.plan()['form'] == 'rows'
     print('vector 2D: trans %.2e  ref %.2e  3D %.2e' % (rel(t, t_ref), rel(r, r_ref), rel(f, f_ref)), flush=True)
     assert t.shape == t_ref.shape == (16, 16, 2, 3)
-    assert f.shape == f_ref.shape == (16, 16, 17, 3, 3)
+    assert f.shape == f_ref.shape == ri + (3, 3)
     assert rel(t, t_ref) < TOL
     assert rel(f, f_ref) < TOL
     assert rel(r, r_ref) < 1e-4
