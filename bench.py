@@ -51,15 +51,23 @@ def pass_real_bytes(nvox, A, M):
 
     Per launch and voxel: the spectrum is read when the launch completes an
     iteration and written when it starts the next one; f is read by the full
-    update (jj >= 3) and as the source term p0 (jj = 0, 1) and written when the
-    next p is gamma*f; Field is read by the accumulating updates (jj >= 2) and
-    written by every completing launch; V (shared by the A right-hand sides)
-    is read once per voxel."""
+    updates (jj >= 3) and as the source term p0 (jj = 0, 1) and written when the
+    next p is gamma*f; Field is read and written by the accumulating updates,
+    which from jj = 4 on come in pairs (an even jj below M leaves Field alone and
+    the odd jj after it adds both increments); V (shared by the A right-hand
+    sides) is read once per voxel."""
+    def kind(jj):       # 0 none, 1 set, 2 add, 3 full3, 4 full, 5 defer, 6 pair
+        if jj <= 3:
+            return jj
+        if jj % 2 == 0 and jj < M:
+            return 5
+        return 6 if (jj % 2 == 1 and jj >= 5) else 4
+
     def one(jj):
-        upd = 0 if jj == 0 else (1 if jj == 1 else (2 if jj == 2 else 3))
+        upd = kind(jj)
         fwd = 0 if jj == M else (1 if jj <= 1 else 2)
-        rd = (upd != 0) + (upd == 3 or fwd == 1) + (upd >= 2)
-        wr = (upd != 0) + (fwd == 2) + (fwd != 0)
+        rd = (upd != 0) + (upd >= 3 or fwd == 1) + (upd in (2, 3, 4, 6))
+        wr = (upd not in (0, 5)) + (fwd == 2) + (fwd != 0)
         return 8 * A * (rd + wr) + 8
     return nvox * np.array([one(jj) for jj in range(0, M + 1)], dtype=np.float64)
 

@@ -27,6 +27,11 @@ enum UpdKind : int {
   UPD_VQ_ADD = 2,    // jj = 2 : f2 = A V q;            Field += f2; x = f2
   UPD_FULL3 = 3,     // jj = 3 : f3 = A (x - gx + V q); Field += f3; x = f3 + F1 (F1 = Field - x)
   UPD_FULL = 4,      // jj > 3 : fn = A (x - gx + V q); Field += fn; x = fn
+  // Field accumulation in pairs (jj >= 4): an even jj below the last leaves Field
+  // alone, and the odd jj after it adds both increments, the earlier one being the
+  // x it reads.  Field then crosses HBM every other iteration instead of every one.
+  UPD_DEFER = 5,     // fn = A (x - gx + V q); x = fn                 (Field untouched)
+  UPD_PAIR = 6,      // fn = A (x - gx + V q); Field += x + fn; x = fn
 };
 enum FwdKind : int {
   FWD_NONE = 0,      // last iteration: nothing to transform
@@ -78,14 +83,23 @@ struct StateArgs {
 __device__ __forceinline__ int flip_of(int jj, const uint8_t* parity, int r) { return (jj + parity[r]) & 1; }
 
 // Which state arrays a pointwise step reads.
-__host__ __device__ constexpr bool reads_f(int upd, int fwd) { return upd == UPD_FULL3 || upd == UPD_FULL || fwd == FWD_P0; }
-__host__ __device__ constexpr bool reads_F(int upd) { return upd == UPD_VQ_ADD || upd == UPD_FULL3 || upd == UPD_FULL; }
+__host__ __device__ constexpr bool reads_f(int upd, int fwd) { return upd >= UPD_FULL3 || fwd == FWD_P0; }
+__host__ __device__ constexpr bool reads_F(int upd) {
+  return upd == UPD_VQ_ADD || upd == UPD_FULL3 || upd == UPD_FULL || upd == UPD_PAIR;
+}
+__host__ __device__ constexpr bool writes_F(int upd) { return upd != UPD_NONE && upd != UPD_DEFER; }
+
+// Update kind of iteration jj of a series of M iterations (see UpdKind).
+__host__ __device__ constexpr int upd_kind(int jj, int M) {
+  return jj == 1 ? UPD_VQ_SET : jj == 2 ? UPD_VQ_ADD : jj == 3 ? UPD_FULL3
+       : (jj % 2 == 0 && jj < M) ? UPD_DEFER : (jj % 2 == 1 && jj >= 5) ? UPD_PAIR : UPD_FULL;
+}
 
 // Pointwise step on one voxel.
 //   q   : G_s applied to the previous p, before the conj ramp (unused for UPD_NONE)
 //   r   : sigma=+1 ramp at the voxel, m: attenuation mask, v: potential
 //   xo  : fstate value (p0 for FWD_P0, x for the full update), Fo: Field value
-// Outputs: Fn (new Field, when upd != NONE), xn (new fstate, when fwd == GAMMA),
+// Outputs: Fn (new Field, when writes_F(upd)), xn (new fstate, when fwd == GAMMA),
 // returns the next p times the forward ramp (0 when fwd == NONE).
 __device__ __forceinline__ float2 born_step(int upd, int fwd, float2 q, float2 r, float m, float2 v, float2 xo,
                                             float2 Fo, float inv_eps, int flip_inv, int flip_fwd, float2& Fn,
@@ -104,7 +118,7 @@ __device__ __forceinline__ float2 born_step(int upd, int fwd, float2 q, float2 r
       const float2 vx = cmul(v, xo);
       const float2 gx = make_float2(-vx.y * inv_eps, vx.x * inv_eps);   // gamma x = (i/eps) v x
       fn = cscale(cadd(csub(xo, gx), vq), m);
-      Fn = cadd(Fo, fn);
+      Fn = upd == UPD_PAIR ? cadd(Fo, cadd(xo, fn)) : cadd(Fo, fn);
       xn = (upd == UPD_FULL3) ? cadd(fn, csub(Fo, xo)) : fn;
     }
     nrm += (double)fn.x * fn.x + (double)fn.y * fn.y;

@@ -116,7 +116,7 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
 
   // pointwise update, CH pairs per thread at a time
   const bool rf = reads_f(upd, fwd), rF = reads_F(upd);
-  const bool wF = upd != UPD_NONE, wf = fwd == FWD_GAMMA;
+  const bool wF = writes_F(upd), wf = fwd == FWD_GAMMA;
   const float2* __restrict__ V = args.st.V;
   float2* __restrict__ F = args.st.field + roff;
   float2* __restrict__ f = args.st.fstate + roff;

@@ -465,7 +465,7 @@ struct Solver {
     return s;
   }
 
-  static int upd_of(int jj) { return jj == 1 ? UPD_VQ_SET : jj == 2 ? UPD_VQ_ADD : jj == 3 ? UPD_FULL3 : UPD_FULL; }
+  static int upd_of(int jj, int M) { return upd_kind(jj, M); }
   int fwd_of(int jj, int M) const { return jj == M ? FWD_NONE : (jj == 1 ? FWD_P0 : FWD_GAMMA); }
 
   // fused schedule for one batch already seeded in fstate
@@ -499,7 +499,7 @@ struct Solver {
     for (int jj = 1; jj <= M; ++jj) {
       pass_k(jj);
       const int fwd = fwd_of(jj, M);
-      pass_r(upd_of(jj), fwd, jj, fwd == FWD_NONE ? 0 : jj + 1);
+      pass_r(upd_of(jj, M), fwd, jj, fwd == FWD_NONE ? 0 : jj + 1);
     }
     HIP_CHECK(hipGetLastError());
   }
@@ -548,7 +548,7 @@ struct Solver {
       pass_k(jj);
       fft_y(1);
       const int fwd = fwd_of(jj, M);
-      pass_r(upd_of(jj), fwd, jj, fwd == FWD_NONE ? 0 : jj + 1);
+      pass_r(upd_of(jj, M), fwd, jj, fwd == FWD_NONE ? 0 : jj + 1);
     }
     HIP_CHECK(hipGetLastError());
   }
@@ -602,7 +602,7 @@ struct Solver {
                                           flip); });
       for (int q = 0; q < P; ++q) fft3(sp0 + (long long)q * nvox, 1);
       const int fwd = fwd_of(jj, M);
-      update(upd_of(jj), fwd, jj, fwd == FWD_NONE ? 0 : jj + 1);
+      update(upd_of(jj, M), fwd, jj, fwd == FWD_NONE ? 0 : jj + 1);
     }
     HIP_CHECK(hipGetLastError());
   }

@@ -88,14 +88,18 @@ def test_odd_bornmax_alternation_persists_across_rhs():
     s.close()
 
 
-@pytest.mark.parametrize('iters', [1, 2, 3])
-def test_short_series(iters):
+@pytest.mark.parametrize('form', [0, 1, 2])
+@pytest.mark.parametrize('iters', [1, 2, 3, 4, 5, 6, 7])
+def test_short_series(iters, form):
+    """Series lengths around the special first iterations and the paired Field
+    accumulation (from jj = 4, an even jj below the last defers its increment to the
+    odd jj after it; a series ending on an even jj adds its own)."""
     ri = (16, 16, 16)
     p = params(ri, iterations=iters)
     RI = sphere_ri(ri, 0.5)
     u = plane_wave(ri)
     ref, _ = _oracle_loop(p, RI, [u])
-    s = _gpu()(p)
+    s = _gpu()(p, options=dict(form=form))
     s.set_RI(RI)
     assert rel(s.solve_incident(u), ref[..., 0]) < TOL
     s.close()
