@@ -28,7 +28,11 @@ struct PassRArgs {
 // for grids without a two-pass plan): the tile is A contiguous rows
 // y = y2 * A + y1, only the x transforms are done here, and the spectrum stays
 // in the natural [z][y][x] layout for the separate y and z passes.
-template <int NX, int A, int NT, int ROWS>
+//
+// STEADY = 1: the launch is known to be a steady-state one (update UPD_DEFER or
+// UPD_PAIR, next p = gamma x), so the pointwise step compiles to straight-line
+// code; STEADY = 0 takes every update kind at run time.
+template <int NX, int A, int NT, int ROWS, int STEADY>
 __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
   static_assert(NX % 2 == 0, "pass_real processes x in pairs");
   constexpr int PR = row_pitch(NX);
@@ -62,7 +66,8 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
   }
   const long long roff = (long long)r * args.st.rhs_stride;
   float4* __restrict__ spec4 = reinterpret_cast<float4*>(args.st.spec + roff + (long long)(z * c.b + y2) * (A * NX));
-  const int upd = args.upd, fwd = args.fwd;
+  const int upd = STEADY ? (args.upd == UPD_PAIR ? UPD_PAIR : UPD_DEFER) : args.upd;
+  const int fwd = STEADY ? FWD_GAMMA : args.fwd;
   const int flip_inv = args.jj_inv ? flip_of(args.jj_inv, args.st.parity, r) : 0;
   const int flip_fwd = args.jj_fwd ? flip_of(args.jj_fwd, args.st.parity, r) : 0;
 

@@ -23,7 +23,8 @@ using PassKFn = void (*)(struct PassKArgs);
 using RowsFn = void (*)(float2*, long long, const float2*, int);
 using StridedFn = void (*)(float2*, int, long long, long long, int, const float2*, int);
 
-struct PassRInfo { PassRFn fn; int nx, a, nt, lds, rows; };
+// fn takes every update kind; fn_steady only UPD_DEFER / UPD_PAIR with FWD_GAMMA
+struct PassRInfo { PassRFn fn, fn_steady; int nx, a, nt, lds, rows; };
 struct PassKInfo { PassKFn fn; int b, nz, t, nt, lds, pole; };
 struct RowsInfo { RowsFn fn; int n, rows, nt; };
 struct StridedInfo { StridedFn fn; int n, t, nt; };
