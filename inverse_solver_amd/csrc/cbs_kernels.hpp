@@ -98,26 +98,27 @@ __host__ __device__ constexpr int upd_kind(int jj, int M) {
 // Pointwise step on one voxel.
 //   q   : G_s applied to the previous p, before the conj ramp (unused for UPD_NONE)
 //   r   : sigma=+1 ramp at the voxel, m: attenuation mask, v: potential
+//   sg_inv, sg_fwd: ramp orientation of the inverse and forward sides, +1 or -1:
+//         q is multiplied by (r.x, sg_inv r.y) (= conj(ramp_s) for the inverse flip
+//         state) and the next p by (r.x, sg_fwd r.y) (= ramp_s of the forward one)
 //   xo  : fstate value (p0 for FWD_P0, x for the full update), Fo: Field value
 // Outputs: Fn (new Field, when writes_F(upd)), xn (new fstate, when fwd == GAMMA),
 // returns the next p times the forward ramp (0 when fwd == NONE).
 __device__ __forceinline__ float2 born_step(int upd, int fwd, float2 q, float2 r, float m, float2 v, float2 xo,
-                                            float2 Fo, float inv_eps, int flip_inv, int flip_fwd, float2& Fn,
+                                            float2 Fo, float inv_eps, float sg_inv, float sg_fwd, float2& Fn,
                                             float2& xn, double& nrm) {
   xn = make_float2(0.f, 0.f);
   if (upd != UPD_NONE) {
-    // q * conj(ramp_s): ramp_s = r (sigma=+1) or conj(r) (sigma=-1)
-    const float2 qq = flip_inv ? cmul(q, r) : cmulc(q, r);
-    const float2 vq = cmul(v, qq);
+    const float2 qq = cmul(q, make_float2(r.x, sg_inv * r.y));
     float2 fn;
     if (upd == UPD_VQ_SET || upd == UPD_VQ_ADD) {
-      fn = cscale(vq, m);
+      fn = cscale(cmul(v, qq), m);
       Fn = (upd == UPD_VQ_SET) ? fn : cadd(Fo, fn);
       xn = fn;
     } else {
-      const float2 vx = cmul(v, xo);
-      const float2 gx = make_float2(-vx.y * inv_eps, vx.x * inv_eps);   // gamma x = (i/eps) v x
-      fn = cscale(cadd(csub(xo, gx), vq), m);
+      // A (x - gamma x + V q) with gamma x = (i/eps) V x, as A (x + V (q - (i/eps) x))
+      const float2 t = make_float2(fmaf(xo.y, inv_eps, qq.x), fmaf(-xo.x, inv_eps, qq.y));
+      fn = cscale(cadd(xo, cmul(v, t)), m);
       Fn = upd == UPD_PAIR ? cadd(Fo, cadd(xo, fn)) : cadd(Fo, fn);
       xn = (upd == UPD_FULL3) ? cadd(fn, csub(Fo, xo)) : fn;
     }
@@ -131,8 +132,12 @@ __device__ __forceinline__ float2 born_step(int upd, int fwd, float2 q, float2 r
     const float2 vx = cmul(v, xn);
     p = make_float2(-vx.y * inv_eps, vx.x * inv_eps);
   }
-  return flip_fwd ? cmulc(p, r) : cmul(p, r);
+  return cmul(p, make_float2(r.x, sg_fwd * r.y));
 }
+
+// Ramp orientation signs for born_step from the flip states (1 = Greenp flipped).
+__device__ __forceinline__ float sign_inv(int flip_inv) { return flip_inv ? 1.f : -1.f; }
+__device__ __forceinline__ float sign_fwd(int flip_fwd) { return flip_fwd ? -1.f : 1.f; }
 
 // Wave-level reduction of a double then one atomic per wave.
 __device__ __forceinline__ void flush_norm(double v, double* dst) {

@@ -68,8 +68,8 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
   float4* __restrict__ spec4 = reinterpret_cast<float4*>(args.st.spec + roff + (long long)(z * c.b + y2) * (A * NX));
   const int upd = STEADY ? (args.upd == UPD_PAIR ? UPD_PAIR : UPD_DEFER) : args.upd;
   const int fwd = STEADY ? FWD_GAMMA : args.fwd;
-  const int flip_inv = args.jj_inv ? flip_of(args.jj_inv, args.st.parity, r) : 0;
-  const int flip_fwd = args.jj_fwd ? flip_of(args.jj_fwd, args.st.parity, r) : 0;
+  const float sg_inv = sign_inv(args.jj_inv ? flip_of(args.jj_inv, args.st.parity, r) : 0);
+  const float sg_fwd = sign_fwd(args.jj_fwd ? flip_of(args.jj_fwd, args.st.parity, r) : 0);
 
   // bulk load of the spectrum tile (all loads in flight before LDS writes)
   float4 reg[L];
@@ -158,9 +158,9 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
       const float2 q0 = upd != UPD_NONE ? lds[y1 * PR + x] : make_float2(0.f, 0.f);
       const float2 q1 = upd != UPD_NONE ? lds[y1 * PR + x + 1] : make_float2(0.f, 0.f);
       const float2 p0 = born_step(upd, fwd, q0, cmul(s_rx[x], ryz), s_mx[x] * myz, lo2(vv[k]), lo2(xo4), lo2(Fo4),
-                                  c.inv_eps, flip_inv, flip_fwd, Fn0, xn0, nrm);
+                                  c.inv_eps, sg_inv, sg_fwd, Fn0, xn0, nrm);
       const float2 p1 = born_step(upd, fwd, q1, cmul(s_rx[x + 1], ryz), s_mx[x + 1] * myz, hi2(vv[k]), hi2(xo4),
-                                  hi2(Fo4), c.inv_eps, flip_inv, flip_fwd, Fn1, xn1, nrm);
+                                  hi2(Fo4), c.inv_eps, sg_inv, sg_fwd, Fn1, xn1, nrm);
       if (wF) *reinterpret_cast<float4*>(F + gi[k]) = pack2(Fn0, Fn1);
       if (wf) *reinterpret_cast<float4*>(f + gi[k]) = pack2(xn0, xn1);
       if (fwd != FWD_NONE) {

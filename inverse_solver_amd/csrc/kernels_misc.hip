@@ -53,8 +53,8 @@ __global__ void update_plain(StateArgs st, Tables tb, Consts c, int upd, int fwd
   float2* spec = st.spec + ro;
   float2* F = st.field + ro;
   float2* f = st.fstate + ro;
-  const int flip_inv = jj_inv ? flip_of(jj_inv, st.parity, rhs) : 0;
-  const int flip_fwd = jj_fwd ? flip_of(jj_fwd, st.parity, rhs) : 0;
+  const float sg_inv = sign_inv(jj_inv ? flip_of(jj_inv, st.parity, rhs) : 0);
+  const float sg_fwd = sign_fwd(jj_fwd ? flip_of(jj_fwd, st.parity, rhs) : 0);
   GRID_STRIDE(i, c.nvox) {
     const int x = (int)(i % c.nx);
     const long long yz = i / c.nx;
@@ -67,7 +67,7 @@ __global__ void update_plain(StateArgs st, Tables tb, Consts c, int upd, int fwd
     const float2 xo = reads_f(upd, fwd) ? f[i] : zero;
     const float2 Fo = reads_F(upd) ? F[i] : zero;
     float2 Fn, xn;
-    const float2 p = born_step(upd, fwd, q, r, m, st.V[i], xo, Fo, c.inv_eps, flip_inv, flip_fwd, Fn, xn, nrm);
+    const float2 p = born_step(upd, fwd, q, r, m, st.V[i], xo, Fo, c.inv_eps, sg_inv, sg_fwd, Fn, xn, nrm);
     if (writes_F(upd)) F[i] = Fn;
     if (fwd == FWD_GAMMA) f[i] = xn;
     if (fwd != FWD_NONE) spec[i] = p;
