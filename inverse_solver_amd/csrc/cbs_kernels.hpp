@@ -9,11 +9,12 @@
 //                                  the a-part in k space and the b-part in real space)
 //
 // One Born iteration (Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:577-643) is two launches:
-//   pass_real  : inverse cross-twiddle + inverse a-DFT (y) + inverse x-FFT, the
-//                pointwise update with ramp/mask/potential, Field accumulation,
-//                gamma*f with the next ramp, forward x-FFT + a-DFT + cross-twiddle
-//   pass_kspace: b-DFT (y) + z-FFT, Green's multiply with the +-1/4-bin shifted
-//                grid and 1/N, inverse z-FFT + inverse b-DFT
+//   pass_real  : inverse a-DFT (y) + inverse x-FFT, the pointwise update with
+//                ramp/mask/potential, Field accumulation, gamma*f with the next
+//                ramp, forward x-FFT + a-DFT
+//   pass_kspace: cross twiddle + b-DFT (y) + z-FFT, Green's multiply with the
+//                +-1/4-bin shifted grid and 1/N, inverse z-FFT + inverse b-DFT +
+//                conjugate cross twiddle
 #pragma once
 #include "fft_core.hpp"
 

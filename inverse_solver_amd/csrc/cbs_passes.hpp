@@ -24,7 +24,7 @@ struct PassRArgs {
 };
 
 // ROWS = 0: the tile is the y coset y = y1 * b + y2 and the pass does the a-point
-// part of the y transform (with the cross twiddles).  ROWS = 1 (four-pass form
+// part of the y transform (pass_kspace applies the cross twiddles).  ROWS = 1 (four-pass form
 // for grids without a two-pass plan): the tile is A contiguous rows
 // y = y2 * A + y1, only the x transforms are done here, and the spectrum stays
 // in the natural [z][y][x] layout for the separate y and z passes.
@@ -46,7 +46,6 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
   __shared__ float s_mx[NX];
   __shared__ float2 s_ryz[A];
   __shared__ float s_myz[A];
-  __shared__ float2 s_cr[A];
 
   const Consts& c = args.c;
   const Tables& tb = args.tb;
@@ -90,7 +89,6 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
     const int y = ROWS ? y2 * A + i : i * c.b + y2;
     s_ryz[i] = cmul(tb.ramp_y[y], tb.ramp_z[z]);
     s_myz[i] = tb.mask_y[y] * tb.mask_z[z];
-    s_cr[i] = ROWS ? make_float2(1.f, 0.f) : tb.cross[y2 * A + i];
   }
   __syncthreads();
 
@@ -104,14 +102,8 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
       if (i < E2) {
         const int e = 2 * i;
         const int p1 = e / NX, px = e - p1 * NX;
-        float2 v0 = lo2(reg[k]), v1 = hi2(reg[k]);
-        if (!ROWS && c.b > 1) {
-          const float2 w = s_cr[p1];
-          v0 = cmulc(v0, w);
-          v1 = cmulc(v1, w);
-        }
-        lds[p1 * PR + px] = v0;
-        lds[p1 * PR + px + 1] = v1;
+        lds[p1 * PR + px] = lo2(reg[k]);
+        lds[p1 * PR + px + 1] = hi2(reg[k]);
       }
     }
     __syncthreads();
@@ -180,13 +172,7 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
     if (i < E2) {
       const int e = 2 * i;
       const int p1 = e / NX, px = e - p1 * NX;
-      float2 v0 = lds[p1 * PR + px], v1 = lds[p1 * PR + px + 1];
-      if (!ROWS && c.b > 1) {
-        const float2 w = s_cr[p1];
-        v0 = cmul(v0, w);
-        v1 = cmul(v1, w);
-      }
-      spec4[i] = pack2(v0, v1);
+      spec4[i] = pack2(lds[p1 * PR + px], lds[p1 * PR + px + 1]);
     }
   }
 }
@@ -244,6 +230,12 @@ __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
   const int flip = flip_of(args.jj, args.parity, P * r);
   const int tvalid = c.nx - x0 < T ? c.nx - x0 : T;      // even: nx and x0 are even
 
+  // cross twiddles W_ny^(y2 k1) of the four-step y transform for this block's a-position
+  // (y2 = 0 needs none); applied before the forward b-DFT and, conjugated, after the inverse
+  float2 wcr[B];
+#pragma unroll
+  for (int y2 = 0; y2 < B; ++y2) wcr[y2] = y2 ? tb.cross[y2 * c.a + p1] : make_float2(1.f, 0.f);
+
   float4 reg[L][P][B];
 #pragma unroll
   for (int k = 0; k < L; ++k) {
@@ -272,6 +264,10 @@ __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
         for (int y2 = 0; y2 < B; ++y2) {
           u0[y2] = lo2(reg[k][q][y2]);
           u1[y2] = hi2(reg[k][q][y2]);
+          if (y2) {
+            u0[y2] = cmul(u0[y2], wcr[y2]);
+            u1[y2] = cmul(u1[y2], wcr[y2]);
+          }
         }
         Dft<B, false>::run(u0);
         Dft<B, false>::run(u1);
@@ -340,7 +336,13 @@ __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
         Dft<B, true>::run(u1);
         float2* dst = spec + (long long)q * args.rhs_stride + (long long)z * B * rowstride + 2 * tp;
 #pragma unroll
-        for (int y2 = 0; y2 < B; ++y2) *reinterpret_cast<float4*>(dst + y2 * rowstride) = pack2(u0[y2], u1[y2]);
+        for (int y2 = 0; y2 < B; ++y2) {
+          if (y2) {
+            u0[y2] = cmulc(u0[y2], wcr[y2]);
+            u1[y2] = cmulc(u1[y2], wcr[y2]);
+          }
+          *reinterpret_cast<float4*>(dst + y2 * rowstride) = pack2(u0[y2], u1[y2]);
+        }
       }
     }
   }
