@@ -104,10 +104,11 @@ __host__ __device__ constexpr int upd_kind(int jj, int M) {
 //         state) and the next p by (r.x, sg_fwd r.y) (= ramp_s of the forward one)
 //   xo  : fstate value (p0 for FWD_P0, x for the full update), Fo: Field value
 // Outputs: Fn (new Field, when writes_F(upd)), xn (new fstate, when fwd == GAMMA),
-// returns the next p times the forward ramp (0 when fwd == NONE).
+// |fn|^2 added to nrm in FP64 when want_norm; returns the next p times the forward
+// ramp (0 when fwd == NONE).
 __device__ __forceinline__ float2 born_step(int upd, int fwd, float2 q, float2 r, float m, float2 v, float2 xo,
                                             float2 Fo, float inv_eps, float sg_inv, float sg_fwd, float2& Fn,
-                                            float2& xn, double& nrm) {
+                                            float2& xn, bool want_norm, double& nrm) {
   xn = make_float2(0.f, 0.f);
   if (upd != UPD_NONE) {
     const float2 qq = cmul(q, make_float2(r.x, sg_inv * r.y));
@@ -123,7 +124,7 @@ __device__ __forceinline__ float2 born_step(int upd, int fwd, float2 q, float2 r
       Fn = upd == UPD_PAIR ? cadd(Fo, cadd(xo, fn)) : cadd(Fo, fn);
       xn = (upd == UPD_FULL3) ? cadd(fn, csub(Fo, xo)) : fn;
     }
-    nrm += (double)fn.x * fn.x + (double)fn.y * fn.y;
+    if (want_norm) nrm += (double)fn.x * fn.x + (double)fn.y * fn.y;
   }
   if (fwd == FWD_NONE) return make_float2(0.f, 0.f);
   float2 p;

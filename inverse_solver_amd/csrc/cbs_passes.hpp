@@ -30,8 +30,8 @@ struct PassRArgs {
 // in the natural [z][y][x] layout for the separate y and z passes.
 //
 // STEADY = 1: the launch is known to be a steady-state one (update UPD_DEFER or
-// UPD_PAIR, next p = gamma x), so the pointwise step compiles to straight-line
-// code; STEADY = 0 takes every update kind at run time.
+// UPD_PAIR, next p = gamma x, no norm tracking), so the pointwise step compiles to
+// straight-line code; STEADY = 0 takes every update kind at run time.
 template <int NX, int A, int NT, int ROWS, int STEADY>
 __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
   static_assert(NX % 2 == 0, "pass_real processes x in pairs");
@@ -117,6 +117,8 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
   const float2* __restrict__ V = args.st.V;
   float2* __restrict__ F = args.st.field + roff;
   float2* __restrict__ f = args.st.fstate + roff;
+  // the steady instantiation never tracks norms (the host launches the general one then)
+  const bool want_norm = !STEADY && args.st.norms != nullptr;
   double nrm = 0.0;
 #pragma unroll
   for (int k0 = 0; k0 < L; k0 += CH) {
@@ -150,9 +152,9 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
       const float2 q0 = upd != UPD_NONE ? lds[y1 * PR + x] : make_float2(0.f, 0.f);
       const float2 q1 = upd != UPD_NONE ? lds[y1 * PR + x + 1] : make_float2(0.f, 0.f);
       const float2 p0 = born_step(upd, fwd, q0, cmul(s_rx[x], ryz), s_mx[x] * myz, lo2(vv[k]), lo2(xo4), lo2(Fo4),
-                                  c.inv_eps, sg_inv, sg_fwd, Fn0, xn0, nrm);
+                                  c.inv_eps, sg_inv, sg_fwd, Fn0, xn0, want_norm, nrm);
       const float2 p1 = born_step(upd, fwd, q1, cmul(s_rx[x + 1], ryz), s_mx[x + 1] * myz, hi2(vv[k]), hi2(xo4),
-                                  hi2(Fo4), c.inv_eps, sg_inv, sg_fwd, Fn1, xn1, nrm);
+                                  hi2(Fo4), c.inv_eps, sg_inv, sg_fwd, Fn1, xn1, want_norm, nrm);
       if (wF) *reinterpret_cast<float4*>(F + gi[k]) = pack2(Fn0, Fn1);
       if (wf) *reinterpret_cast<float4*>(f + gi[k]) = pack2(xn0, xn1);
       if (fwd != FWD_NONE) {
@@ -161,7 +163,7 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
       }
     }
   }
-  if (args.st.norms) flush_norm(nrm, args.st.norms + r);
+  if (want_norm) flush_norm(nrm, args.st.norms + r);
   if (fwd == FWD_NONE) return;
   __syncthreads();
   line_fft<NX, false, A, NT>(lds, row, s_twx, tid);

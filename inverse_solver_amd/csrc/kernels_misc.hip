@@ -67,7 +67,7 @@ __global__ void update_plain(StateArgs st, Tables tb, Consts c, int upd, int fwd
     const float2 xo = reads_f(upd, fwd) ? f[i] : zero;
     const float2 Fo = reads_F(upd) ? F[i] : zero;
     float2 Fn, xn;
-    const float2 p = born_step(upd, fwd, q, r, m, st.V[i], xo, Fo, c.inv_eps, sg_inv, sg_fwd, Fn, xn, nrm);
+    const float2 p = born_step(upd, fwd, q, r, m, st.V[i], xo, Fo, c.inv_eps, sg_inv, sg_fwd, Fn, xn, st.norms != nullptr, nrm);
     if (writes_F(upd)) F[i] = Fn;
     if (fwd == FWD_GAMMA) f[i] = xn;
     if (fwd != FWD_NONE) spec[i] = p;

@@ -485,7 +485,8 @@ struct Solver {
       args.upd = upd; args.fwd = fwd; args.jj_inv = jj_inv; args.jj_fwd = jj_fwd;
       args.order = opt.block_order & 1;
       const dim3 grid((unsigned)((long long)nv * plan.b * nz));
-      const PassRFn fn = (upd == UPD_DEFER || upd == UPD_PAIR) && fwd == FWD_GAMMA ? pr->fn_steady : pr->fn;
+      const bool steady = (upd == UPD_DEFER || upd == UPD_PAIR) && fwd == FWD_GAMMA && !args.st.norms;
+      const PassRFn fn = steady ? pr->fn_steady : pr->fn;
       timed(0, [&] { hipLaunchKernelGGL(fn, grid, dim3(pr->nt), 0, stream, args); });
     };
     auto pass_k = [&](int jj) {
@@ -525,7 +526,8 @@ struct Solver {
       args.upd = upd; args.fwd = fwd; args.jj_inv = jj_inv; args.jj_fwd = jj_fwd;
       args.order = 0;
       const dim3 grid((unsigned)((long long)nv * plan.b * nz));
-      const PassRFn fn = (upd == UPD_DEFER || upd == UPD_PAIR) && fwd == FWD_GAMMA ? pr->fn_steady : pr->fn;
+      const bool steady = (upd == UPD_DEFER || upd == UPD_PAIR) && fwd == FWD_GAMMA && !args.st.norms;
+      const PassRFn fn = steady ? pr->fn_steady : pr->fn;
       timed(0, [&] { hipLaunchKernelGGL(fn, grid, dim3(pr->nt), 0, stream, args); });
     };
     auto pass_k = [&](int jj) {

@@ -402,7 +402,7 @@ class ForwardSolverConvergentBorn:
         u_in = self.incident_field(source)
         return self.born_loop(u_in)
 
-    def born_loop(self, incident_field):
+    def born_loop(self, incident_field, increments=None):
         """:564-647: the two-chain, alternating-shift Born series.
 
         Scalar: incident_field (x, y, z).  Vector (pole_num = 3): (x, y, z, 3), with
@@ -414,6 +414,10 @@ class ForwardSolverConvergentBorn:
         exactly as the reference's handle properties do (:579-585); it is reset by
         set_RI.  For odd Bornmax this makes consecutive angles start on opposite
         shifts, as in the reference.
+
+        ``increments``: an optional list that receives, per iteration, the FP64 sum
+        over the simulation grid (and the components) of |Field_n|^2 at the moment
+        Field_n is added to Field (:638), i.e. the norm of each series term.
         """
         p = self.parameters
         dt = self.dtype
@@ -470,6 +474,8 @@ class ForwardSolverConvergentBorn:
             if jj == 2:
                 temp = Field.copy()                                            # :634-637
             Field = Field + Field_n                                            # :638
+            if increments is not None:
+                increments.append(float(np.sum(np.abs(Field_n.astype(np.complex128)) ** 2)))
             if jj == 3:
                 Field_n = Field_n + temp                                       # :639-642
         if p['acyclic'] and self.Bornmax % 2:

@@ -123,3 +123,40 @@ def test_graph_replay_equals_eager():
         s.solve_incident(fields[..., :1])       # eager work on another handle in between
     s.close()
     g.close()
+
+
+@pytest.mark.parametrize('form,P', [(0, 1), (1, 1), (2, 1), (0, 3)])
+def test_increment_norms_match_oracle(form, P):
+    """track_norms: the FP64 |f_n|^2 of every series term over the simulation grid
+    (summed over the components of a vector run) equals the oracle's, and tracking
+    them leaves the fields unchanged (the steady-state kernels do not track norms,
+    so those launches fall back to the general ones)."""
+    ri = (16, 16, 16)
+    M = 11
+    p = params(ri, iterations=M, vector_simulation=(P == 3))
+    RI = sphere_ri(ri, 0.5, n_s=1.42)
+    if P == 1:
+        fields = [random_field(ri, 70 + a) for a in range(2)]
+    else:
+        fields = [np.stack([random_field(ri, 80 + 3 * a + j) for j in range(3)], axis=-1) for a in range(2)]
+    o = Oracle(p, io_kernels=False)
+    o.set_RI(RI)
+    inc_ref = []
+    for f in fields:
+        inc = []
+        o.born_loop(f, increments=inc)
+        inc_ref.append(inc)
+    inc_ref = np.array(inc_ref)
+    s = _gpu()(p, options=dict(form=form))
+    s.set_RI(RI)
+    plain = s.solve_incident(np.stack(fields, axis=-1))
+    s.close()
+    s = _gpu()(p, options=dict(form=form, track_norms=1))
+    s.set_RI(RI)
+    got = s.solve_incident(np.stack(fields, axis=-1))
+    nrm = s.norms(2)
+    print(form, P, s.plan()['form'], 'norm rel err', np.max(np.abs(nrm - inc_ref) / inc_ref), flush=True)
+    assert nrm.shape == inc_ref.shape == (2, M)
+    np.testing.assert_allclose(nrm, inc_ref, rtol=1e-5)
+    assert rel(got, plain) < 1e-6
+    s.close()
