@@ -13,7 +13,9 @@ __host__ __device__ constexpr int row_pitch(int n) { return (n % 2 == 0) ? n + 1
 // One block per (rhs, y2, z).  Tile = A rows (y1) of NX points.
 // Global traffic is issued in bulk: each thread loads its share of the tile as
 // 16-byte pairs before touching LDS, and the pointwise step loads V, f and Field
-// for CH pairs at a time.  Twiddles and per-axis ramp/mask factors live in LDS.
+// for CH pairs at a time.  Twiddles and the y/z ramp and mask factors live in LDS;
+// the x ramp and mask pairs are read through the cache, which keeps the 40 x 160
+// tile at three blocks per CU.
 struct PassRArgs {
   Tables tb;
   Consts c;
@@ -38,12 +40,10 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
   constexpr int PR = row_pitch(NX);
   constexpr int E2 = A * NX / 2;              // complex pairs in the tile
   constexpr int L = (E2 + NT - 1) / NT;       // pairs per thread
-  constexpr int CH = L < 4 ? L : 4;           // pointwise chunk (pairs)
+  constexpr int CH = 1;                       // pointwise chunk (pairs): one keeps the VGPRs at three blocks per CU
   __shared__ float2 lds[A * PR];
   __shared__ float2 s_twx[NX];
   __shared__ float2 s_twa[A];
-  __shared__ float2 s_rx[NX];
-  __shared__ float s_mx[NX];
   __shared__ float2 s_ryz[A];
   __shared__ float s_myz[A];
 
@@ -81,8 +81,6 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
   }
   for (int i = tid; i < NX; i += NT) {
     s_twx[i] = tb.tw_x[i];
-    s_rx[i] = tb.ramp_x[i];
-    s_mx[i] = tb.mask_x[i];
   }
   for (int i = tid; i < A; i += NT) {
     s_twa[i] = tb.tw_a[i];
@@ -122,7 +120,8 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
   double nrm = 0.0;
 #pragma unroll
   for (int k0 = 0; k0 < L; k0 += CH) {
-    float4 vv[CH], xx[CH], FF[CH];
+    float4 vv[CH], xx[CH], FF[CH], rx[CH];
+    float2 mx[CH];
     long long gi[CH];
 #pragma unroll
     for (int k = 0; k < CH; ++k) {
@@ -134,6 +133,9 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
         const long long g = ((long long)z * c.ny + (ROWS ? y2 * A + y1 : y1 * c.b + y2)) * NX + x;
         gi[k] = g;
         vv[k] = *reinterpret_cast<const float4*>(V + g);
+        // per-axis x ramp and mask pairs (read through the cache: the LDS holds three tiles per CU)
+        rx[k] = *reinterpret_cast<const float4*>(tb.ramp_x + x);
+        mx[k] = *reinterpret_cast<const float2*>(tb.mask_x + x);
         if (rf) xx[k] = *reinterpret_cast<const float4*>(f + g);
         if (rF) FF[k] = *reinterpret_cast<const float4*>(F + g);
       }
@@ -151,9 +153,9 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
       float2 Fn0, Fn1, xn0, xn1;
       const float2 q0 = upd != UPD_NONE ? lds[y1 * PR + x] : make_float2(0.f, 0.f);
       const float2 q1 = upd != UPD_NONE ? lds[y1 * PR + x + 1] : make_float2(0.f, 0.f);
-      const float2 p0 = born_step(upd, fwd, q0, cmul(s_rx[x], ryz), s_mx[x] * myz, lo2(vv[k]), lo2(xo4), lo2(Fo4),
+      const float2 p0 = born_step(upd, fwd, q0, cmul(lo2(rx[k]), ryz), mx[k].x * myz, lo2(vv[k]), lo2(xo4), lo2(Fo4),
                                   c.inv_eps, sg_inv, sg_fwd, Fn0, xn0, want_norm, nrm);
-      const float2 p1 = born_step(upd, fwd, q1, cmul(s_rx[x + 1], ryz), s_mx[x + 1] * myz, hi2(vv[k]), hi2(xo4),
+      const float2 p1 = born_step(upd, fwd, q1, cmul(hi2(rx[k]), ryz), mx[k].y * myz, hi2(vv[k]), hi2(xo4),
                                   hi2(Fo4), c.inv_eps, sg_inv, sg_fwd, Fn1, xn1, want_norm, nrm);
       if (wF) *reinterpret_cast<float4*>(F + gi[k]) = pack2(Fn0, Fn1);
       if (wf) *reinterpret_cast<float4*>(f + gi[k]) = pack2(xn0, xn1);

@@ -55,9 +55,8 @@ constexpr int kNT = 256;
 constexpr int rows_for(int n) { return n <= 64 ? 16 : (n <= 256 ? 8 : (n <= 576 ? 4 : 2)); }
 constexpr int strided_t(int n) { return n <= 512 ? 16 : 8; }
 
-// LDS of pass_real: the tile plus its per-axis tables (x: twiddle, ramp, mask; rows: twiddle,
-// ramp, mask)
-constexpr int pr_lds(int nx, int a) { return a * row_pitch(nx) * 8 + nx * 20 + a * 20; }
+// LDS of pass_real: the tile plus its per-axis tables (x: twiddle; rows: twiddle, ramp, mask)
+constexpr int pr_lds(int nx, int a) { return a * row_pitch(nx) * 8 + nx * 8 + a * 20; }
 #define PR_ENTRY(NX, A, NT) {&pass_real<NX, A, NT, 0, 0>, &pass_real<NX, A, NT, 0, 1>, NX, A, NT, pr_lds(NX, A), 0},
 #define PRR_ENTRY(NX, A, NT) {&pass_real<NX, A, NT, 1, 0>, &pass_real<NX, A, NT, 1, 1>, NX, A, NT, pr_lds(NX, A), 1},
 #define PK_ENTRY(B, NZ, T, NT) {&pass_kspace<B, NZ, T, NT, 1>, B, NZ, T, NT, (int)(NZ * B * T * sizeof(float2)), 1},
