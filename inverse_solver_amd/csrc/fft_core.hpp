@@ -287,6 +287,16 @@ __device__ __forceinline__ void line_fft(float2* __restrict__ lds, const Addr& a
   }
 }
 
+// Line-group index t / NL of task t.  When NL and NT are multiples of the wave
+// size, every lane of a wave has the same quotient; saying so with readfirstlane
+// lets the compiler fetch twiddles and per-position factors that depend only on it
+// with scalar loads instead of one vector load per lane.
+template <int NL, int NT>
+__device__ __forceinline__ int group_of(int t) {
+  if constexpr (NL % 64 == 0 && NT % 64 == 0) return __builtin_amdgcn_readfirstlane(t / NL);
+  else return t / NL;
+}
+
 // Forward FFT, pointwise multiply, inverse FFT on NL lines in LDS, with the
 // last forward step, the multiply and the first inverse step fused in
 // registers (they act on the same R2 values of one task).  mul(l, p, v)
@@ -312,7 +322,7 @@ __device__ __forceinline__ void line_fft_mul_ifft(float2* __restrict__ lds, cons
     __syncthreads();
   } else {
     for (int t = tid; t < NL * R2; t += NT) {           // forward step A
-      const int l = t % NL, n2 = t / NL;
+      const int l = t % NL, n2 = group_of<NL, NT>(t);
       float2 v[R1];
 #pragma unroll
       for (int n1 = 0; n1 < R1; ++n1) v[n1] = lds[addr(l, R2 * n1 + n2)];
@@ -324,7 +334,7 @@ __device__ __forceinline__ void line_fft_mul_ifft(float2* __restrict__ lds, cons
     }
     __syncthreads();
     for (int t = tid; t < NL * R1; t += NT) {           // step B, multiply, inverse step B
-      const int l = t % NL, k1 = t / NL;
+      const int l = t % NL, k1 = group_of<NL, NT>(t);
       float2 v[R2];
 #pragma unroll
       for (int n2 = 0; n2 < R2; ++n2) v[n2] = lds[addr(l, R2 * k1 + n2)];
@@ -337,7 +347,7 @@ __device__ __forceinline__ void line_fft_mul_ifft(float2* __restrict__ lds, cons
     }
     __syncthreads();
     for (int t = tid; t < NL * R2; t += NT) {           // inverse step A
-      const int l = t % NL, n2 = t / NL;
+      const int l = t % NL, n2 = group_of<NL, NT>(t);
       float2 v[R1];
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1) v[k1] = lds[addr(l, R2 * k1 + n2)];
@@ -388,7 +398,7 @@ __device__ __forceinline__ void line_fft_mul3_ifft(float2* __restrict__ lds, con
     __syncthreads();
   } else {
     for (int t = tid; t < 3 * NL * R2; t += NT) {       // forward step A, all components
-      const int l = t % (3 * NL), n2 = t / (3 * NL);
+      const int l = t % (3 * NL), n2 = group_of<3 * NL, NT>(t);
       float2 v[R1];
 #pragma unroll
       for (int n1 = 0; n1 < R1; ++n1) v[n1] = lds[addr(l, R2 * n1 + n2)];
@@ -400,7 +410,7 @@ __device__ __forceinline__ void line_fft_mul3_ifft(float2* __restrict__ lds, con
     }
     __syncthreads();
     for (int t = tid; t < NL * R1; t += NT) {           // step B, 3x3 multiply, inverse step B
-      const int l = t % NL, k1 = t / NL;
+      const int l = t % NL, k1 = group_of<NL, NT>(t);
       float2 v0[R2], v1[R2], v2[R2];
 #pragma unroll
       for (int n2 = 0; n2 < R2; ++n2) {
@@ -425,7 +435,7 @@ __device__ __forceinline__ void line_fft_mul3_ifft(float2* __restrict__ lds, con
     }
     __syncthreads();
     for (int t = tid; t < 3 * NL * R2; t += NT) {       // inverse step A, all components
-      const int l = t % (3 * NL), n2 = t / (3 * NL);
+      const int l = t % (3 * NL), n2 = group_of<3 * NL, NT>(t);
       float2 v[R1];
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1) v[k1] = lds[addr(l, R2 * k1 + n2)];

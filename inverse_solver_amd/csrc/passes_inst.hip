@@ -29,7 +29,7 @@ constexpr int kNT = 256;
   X(1, 96, 32, 256) X(1, 96, 32, 512) X(1, 96, 16, 256) X(2, 96, 16, 256) X(2, 96, 32, 512) X(3, 96, 16, 256) \
   X(2, 135, 32, 256) X(2, 135, 32, 512) X(2, 135, 16, 256) X(4, 135, 8, 256) \
   X(2, 128, 32, 256) X(4, 128, 16, 256) X(6, 128, 8, 256) X(4, 128, 8, 256) \
-  X(4, 160, 16, 256) X(4, 160, 16, 512) X(5, 160, 8, 256) X(8, 160, 8, 256) X(4, 160, 8, 256) \
+  X(4, 160, 16, 256) X(4, 160, 16, 512) X(4, 160, 16, 1024) X(5, 160, 8, 256) X(8, 160, 8, 256) X(4, 160, 8, 256) \
   X(2, 160, 32, 512) X(2, 160, 32, 256) \
   X(9, 160, 4, 256) \
   X(1, 320, 16, 256) X(1, 320, 32, 512) X(1, 256, 16, 256) X(1, 256, 32, 512)
