@@ -149,6 +149,14 @@ __device__ __forceinline__ void flush_norm(double v, double* dst) {
   if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(dst, v);
 }
 
+// Streaming 16-byte store that is not kept in the caches (the arrays written by the
+// passes are next read a whole launch later, after every other tile).
+__device__ __forceinline__ void store_stream(float4* dst, float4 v) {
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  const v4f t = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(t, reinterpret_cast<v4f*>(dst));
+}
+
 __device__ __forceinline__ float2 lo2(float4 v) { return make_float2(v.x, v.y); }
 __device__ __forceinline__ float2 hi2(float4 v) { return make_float2(v.z, v.w); }
 __device__ __forceinline__ float4 pack2(float2 a, float2 b) { return make_float4(a.x, a.y, b.x, b.y); }

@@ -157,8 +157,8 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
                                   c.inv_eps, sg_inv, sg_fwd, Fn0, xn0, want_norm, nrm);
       const float2 p1 = born_step(upd, fwd, q1, cmul(hi2(rx[k]), ryz), mx[k].y * myz, hi2(vv[k]), hi2(xo4),
                                   hi2(Fo4), c.inv_eps, sg_inv, sg_fwd, Fn1, xn1, want_norm, nrm);
-      if (wF) *reinterpret_cast<float4*>(F + gi[k]) = pack2(Fn0, Fn1);
-      if (wf) *reinterpret_cast<float4*>(f + gi[k]) = pack2(xn0, xn1);
+      if (wF) store_stream(reinterpret_cast<float4*>(F + gi[k]), pack2(Fn0, Fn1));
+      if (wf) store_stream(reinterpret_cast<float4*>(f + gi[k]), pack2(xn0, xn1));
       if (fwd != FWD_NONE) {
         lds[y1 * PR + x] = p0;
         lds[y1 * PR + x + 1] = p1;
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
     if (i < E2) {
       const int e = 2 * i;
       const int p1 = e / NX, px = e - p1 * NX;
-      spec4[i] = pack2(lds[p1 * PR + px], lds[p1 * PR + px + 1]);
+      store_stream(spec4 + i, pack2(lds[p1 * PR + px], lds[p1 * PR + px + 1]));
     }
   }
 }
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
             u0[y2] = cmulc(u0[y2], wcr[y2]);
             u1[y2] = cmulc(u1[y2], wcr[y2]);
           }
-          *reinterpret_cast<float4*>(dst + y2 * rowstride) = pack2(u0[y2], u1[y2]);
+          store_stream(reinterpret_cast<float4*>(dst + y2 * rowstride), pack2(u0[y2], u1[y2]));
         }
       }
     }
