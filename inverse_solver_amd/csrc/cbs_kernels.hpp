@@ -157,6 +157,13 @@ __device__ __forceinline__ void store_stream(float4* dst, float4 v) {
   __builtin_nontemporal_store(t, reinterpret_cast<v4f*>(dst));
 }
 
+// Streaming 16-byte load of data read once per launch.
+__device__ __forceinline__ float4 load_stream(const float4* src) {
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  const v4f t = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(src));
+  return make_float4(t.x, t.y, t.z, t.w);
+}
+
 __device__ __forceinline__ float2 lo2(float4 v) { return make_float2(v.x, v.y); }
 __device__ __forceinline__ float2 hi2(float4 v) { return make_float2(v.z, v.w); }
 __device__ __forceinline__ float4 pack2(float2 a, float2 b) { return make_float4(a.x, a.y, b.x, b.y); }

@@ -76,7 +76,7 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
 #pragma unroll
     for (int k = 0; k < L; ++k) {
       const int i = tid + k * NT;
-      if (i < E2) reg[k] = spec4[i];
+      if (i < E2) reg[k] = load_stream(spec4 + i);
     }
   }
   for (int i = tid; i < NX; i += NT) {
@@ -136,8 +136,8 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
         // per-axis x ramp and mask pairs (read through the cache: the LDS holds three tiles per CU)
         rx[k] = *reinterpret_cast<const float4*>(tb.ramp_x + x);
         mx[k] = *reinterpret_cast<const float2*>(tb.mask_x + x);
-        if (rf) xx[k] = *reinterpret_cast<const float4*>(f + g);
-        if (rF) FF[k] = *reinterpret_cast<const float4*>(F + g);
+        if (rf) xx[k] = load_stream(reinterpret_cast<const float4*>(f + g));
+        if (rF) FF[k] = load_stream(reinterpret_cast<const float4*>(F + g));
       }
     }
 #pragma unroll
