@@ -208,9 +208,12 @@ __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
   constexpr int NTASK = NZ * TP;              // (z, pair) tasks
   constexpr int L = (NTASK + NT - 1) / NT;
   constexpr int PLW = P * LW;                 // LDS row width (all components)
-  // the tile alone fills the LDS budget, so the small per-axis tables are read
-  // through the L1 cache instead
-  __shared__ float2 lds[NZ * PLW];
+  // scalar runs: the tile alone fills the LDS budget of two blocks per CU, so the
+  // small per-axis tables are read through the caches.  Vector runs: the tile is
+  // three times larger and a CU holds one block anyway, so the z twiddles and the
+  // z Green's and direction factors of this block's flip state sit after it.
+  constexpr int ZTAB = P == 3 ? 2 * NZ : 0;
+  __shared__ float2 lds[NZ * PLW + ZTAB];
   const Consts& c = args.c;
   const Tables& tb = args.tb;
   const int tid = threadIdx.x;
@@ -234,6 +237,12 @@ __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
   const int flip = flip_of(args.jj, args.parity, P * r);
   const int tvalid = c.nx - x0 < T ? c.nx - x0 : T;      // even: nx and x0 are even
 
+  if constexpr (P == 3) {
+    for (int i = tid; i < NZ; i += NT) {
+      lds[NZ * PLW + i] = tb.tw_z[i];
+      lds[NZ * PLW + NZ + i] = make_float2(tb.green_z[flip * c.nz + i], tb.rad_z[flip * c.nz + i]);
+    }
+  }
   // cross twiddles W_ny^(y2 k1) of the four-step y transform for this block's a-position
   // (y2 = 0 needs none); applied before the forward b-DFT and, conjugated, after the inverse
   float2 wcr[B];
@@ -301,7 +310,7 @@ __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
       };
       line_fft_mul_ifft<NZ, LW, NT>(lds, zl, tb.tw_z, tid, green);
     } else {
-      const float* rz = tb.rad_z + flip * c.nz;
+      const float2* gr_z = lds + NZ * PLW + NZ;                // (green_z, rad_z) of this flip state
       const float* rx = tb.rad_x + flip * c.nx + x0;
       const float* ry = tb.rad_y + flip * c.ny + p1;
       const float2 cc = make_float2(c.c_re, c.c_im);
@@ -309,17 +318,18 @@ __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
       auto dyadic = [&](int l, int pz, float2& v0, float2& v1, float2& v2) {
         const int k2 = l / T, t = l - k2 * T;
         const int tx = t < tvalid ? t : 0;
-        const float d = gx[tx] + gy[k2 * c.a] + gz[pz] - km2;
+        const float2 grz = gr_z[pz];
+        const float d = gx[tx] + gy[k2 * c.a] + grz.x - km2;
         const float s = inv_n * __builtin_amdgcn_rcpf(fmaf(d, d, eps2));   // hardware reciprocal (1 ulp)
         const float2 g = make_float2(d * s, eps * s);
-        const float r0 = rx[tx], r1 = ry[k2 * c.a], r2 = rz[pz];
+        const float r0 = rx[tx], r1 = ry[k2 * c.a], r2 = grz.y;
         const float2 dot = make_float2(r0 * v0.x + r1 * v1.x + r2 * v2.x, r0 * v0.y + r1 * v1.y + r2 * v2.y);
         const float2 cd = cmul(cc, dot);
         v0 = cmul(g, make_float2(v0.x - r0 * cd.x, v0.y - r0 * cd.y));
         v1 = cmul(g, make_float2(v1.x - r1 * cd.x, v1.y - r1 * cd.y));
         v2 = cmul(g, make_float2(v2.x - r2 * cd.x, v2.y - r2 * cd.y));
       };
-      line_fft_mul3_ifft<NZ, LW, NT>(lds, zl, tb.tw_z, tid, dyadic);
+      line_fft_mul3_ifft<NZ, LW, NT>(lds, zl, lds + NZ * PLW, tid, dyadic);
     }
   }
 #pragma unroll

@@ -61,7 +61,7 @@ constexpr int pr_lds(int nx, int a) { return a * row_pitch(nx) * 8 + nx * 8 + a 
 #define PRR_ENTRY(NX, A, NT) {&pass_real<NX, A, NT, 1, 0>, &pass_real<NX, A, NT, 1, 1>, NX, A, NT, pr_lds(NX, A), 1},
 #define PK_ENTRY(B, NZ, T, NT) {&pass_kspace<B, NZ, T, NT, 1>, B, NZ, T, NT, (int)(NZ * B * T * sizeof(float2)), 1},
 #define PKV_ENTRY(B, NZ, T, NT) \
-  {&pass_kspace<B, NZ, T, NT, 3>, B, NZ, T, NT, (int)(3 * NZ * B * T * sizeof(float2)), 3},
+  {&pass_kspace<B, NZ, T, NT, 3>, B, NZ, T, NT, (int)((3 * NZ * B * T + 2 * NZ) * sizeof(float2)), 3},
 #define RW_ENTRY(N) {&fft_rows<N, rows_for(N), kNT>, N, rows_for(N), kNT},
 #define ST_ENTRY(N) {&fft_strided<N, strided_t(N), kNT>, N, strided_t(N), kNT},
 
