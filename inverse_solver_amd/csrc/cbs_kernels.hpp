@@ -149,19 +149,27 @@ __device__ __forceinline__ void flush_norm(double v, double* dst) {
   if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(dst, v);
 }
 
-// Streaming 16-byte store that is not kept in the caches (the arrays written by the
-// passes are next read a whole launch later, after every other tile).
-__device__ __forceinline__ void store_stream(float4* dst, float4 v) {
-  typedef float v4f __attribute__((ext_vector_type(4)));
-  const v4f t = {v.x, v.y, v.z, v.w};
-  __builtin_nontemporal_store(t, reinterpret_cast<v4f*>(dst));
+// 16-byte store of a state array.  With nt set (state larger than the caches: the
+// array is next read a whole launch later, after every other tile) it is marked
+// nontemporal; a state that fits the 256 MB Infinity Cache keeps ordinary stores.
+__device__ __forceinline__ void store_stream(float4* dst, float4 v, bool nt) {
+  if (nt) {
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const v4f t = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(t, reinterpret_cast<v4f*>(dst));
+  } else {
+    *dst = v;
+  }
 }
 
-// Streaming 16-byte load of data read once per launch.
-__device__ __forceinline__ float4 load_stream(const float4* src) {
-  typedef float v4f __attribute__((ext_vector_type(4)));
-  const v4f t = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(src));
-  return make_float4(t.x, t.y, t.z, t.w);
+// 16-byte load of a state array read once per launch (nontemporal when nt, as above).
+__device__ __forceinline__ float4 load_stream(const float4* src, bool nt) {
+  if (nt) {
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const v4f t = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(src));
+    return make_float4(t.x, t.y, t.z, t.w);
+  }
+  return *src;
 }
 
 __device__ __forceinline__ float2 lo2(float4 v) { return make_float2(v.x, v.y); }

@@ -23,6 +23,7 @@ struct PassRArgs {
   int upd, fwd, jj_inv, jj_fwd;
   int order;   // block order: 0 = rhs fastest (V rows reused by consecutive blocks),
                // 1 = y2 fastest, then rhs (the b blocks of one (rhs, z) slab run together)
+  int nt;      // 1: nontemporal access to the state arrays (they exceed the caches)
 };
 
 // ROWS = 0: the tile is the y coset y = y1 * b + y2 and the pass does the a-point
@@ -76,7 +77,7 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
 #pragma unroll
     for (int k = 0; k < L; ++k) {
       const int i = tid + k * NT;
-      if (i < E2) reg[k] = load_stream(spec4 + i);
+      if (i < E2) reg[k] = load_stream(spec4 + i, args.nt);
     }
   }
   for (int i = tid; i < NX; i += NT) {
@@ -136,8 +137,8 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
         // per-axis x ramp and mask pairs (read through the cache: the LDS holds three tiles per CU)
         rx[k] = *reinterpret_cast<const float4*>(tb.ramp_x + x);
         mx[k] = *reinterpret_cast<const float2*>(tb.mask_x + x);
-        if (rf) xx[k] = load_stream(reinterpret_cast<const float4*>(f + g));
-        if (rF) FF[k] = load_stream(reinterpret_cast<const float4*>(F + g));
+        if (rf) xx[k] = load_stream(reinterpret_cast<const float4*>(f + g), args.nt);
+        if (rF) FF[k] = load_stream(reinterpret_cast<const float4*>(F + g), args.nt);
       }
     }
 #pragma unroll
@@ -157,8 +158,8 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
                                   c.inv_eps, sg_inv, sg_fwd, Fn0, xn0, want_norm, nrm);
       const float2 p1 = born_step(upd, fwd, q1, cmul(hi2(rx[k]), ryz), mx[k].y * myz, hi2(vv[k]), hi2(xo4),
                                   hi2(Fo4), c.inv_eps, sg_inv, sg_fwd, Fn1, xn1, want_norm, nrm);
-      if (wF) store_stream(reinterpret_cast<float4*>(F + gi[k]), pack2(Fn0, Fn1));
-      if (wf) store_stream(reinterpret_cast<float4*>(f + gi[k]), pack2(xn0, xn1));
+      if (wF) store_stream(reinterpret_cast<float4*>(F + gi[k]), pack2(Fn0, Fn1), args.nt);
+      if (wf) store_stream(reinterpret_cast<float4*>(f + gi[k]), pack2(xn0, xn1), args.nt);
       if (fwd != FWD_NONE) {
         lds[y1 * PR + x] = p0;
         lds[y1 * PR + x + 1] = p1;
@@ -176,7 +177,7 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
     if (i < E2) {
       const int e = 2 * i;
       const int p1 = e / NX, px = e - p1 * NX;
-      store_stream(spec4 + i, pack2(lds[p1 * PR + px], lds[p1 * PR + px + 1]));
+      store_stream(spec4 + i, pack2(lds[p1 * PR + px], lds[p1 * PR + px + 1]), args.nt);
     }
   }
 }
@@ -193,6 +194,7 @@ struct PassKArgs {
   int jj;
   int order;   // block order: 0 = rhs fastest, then a-index, then x tile;
                // 1 = x tile fastest, then a-index, then rhs (whole rows read together)
+  int nt;      // 1: nontemporal stores of the spectrum (it exceeds the caches)
 };
 
 // P = 1: scalar Green's function.  P = 3: vector run; the block holds the three
@@ -355,7 +357,7 @@ __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
             u0[y2] = cmulc(u0[y2], wcr[y2]);
             u1[y2] = cmulc(u1[y2], wcr[y2]);
           }
-          store_stream(reinterpret_cast<float4*>(dst + y2 * rowstride), pack2(u0[y2], u1[y2]));
+          store_stream(reinterpret_cast<float4*>(dst + y2 * rowstride), pack2(u0[y2], u1[y2]), args.nt);
         }
       }
     }
@@ -396,6 +398,8 @@ template <int N, int T, int NT>
 __global__ __launch_bounds__(NT) void fft_strided(float2* data, int nx, long long es, long long os, int nouter,
                                                   const float2* tw, int inverse) {
   __shared__ float2 lds[N * T];
+  __shared__ float2 s_tw[N];   // twiddles: a task's R1 - 1 of them differ from lane to lane
+  for (int i = threadIdx.x; i < N; i += NT) s_tw[i] = tw[i];
   const int ntile = (nx + T - 1) / T;
   const int xt = blockIdx.x % ntile;
   const int o = blockIdx.x / ntile;
@@ -434,8 +438,8 @@ __global__ __launch_bounds__(NT) void fft_strided(float2* data, int nx, long lon
     }
   }
   __syncthreads();
-  if (inverse) line_fft<N, true, T, NT>(lds, ad, tw, threadIdx.x);
-  else line_fft<N, false, T, NT>(lds, ad, tw, threadIdx.x);
+  if (inverse) line_fft<N, true, T, NT>(lds, ad, s_tw, threadIdx.x);
+  else line_fft<N, false, T, NT>(lds, ad, s_tw, threadIdx.x);
   if (pairs) {
 #pragma unroll
     for (int k = 0; k < LP; ++k) {

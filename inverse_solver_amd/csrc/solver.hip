@@ -466,6 +466,9 @@ struct Solver {
   }
 
   static int upd_of(int jj, int M) { return upd_kind(jj, M); }
+  // nontemporal state access only when Field, f and the spectrum of nv slots plus V
+  // exceed what the 256 MB Infinity Cache can keep between launches
+  int stream_hint(int nv) const { return (3.0 * nv + 1.0) * (double)nvox * sizeof(float2) > 192.0e6 ? 1 : 0; }
   int fwd_of(int jj, int M) const { return jj == M ? FWD_NONE : (jj == 1 ? FWD_P0 : FWD_GAMMA); }
 
   // fused schedule for one batch already seeded in fstate
@@ -484,6 +487,7 @@ struct Solver {
       }
       args.upd = upd; args.fwd = fwd; args.jj_inv = jj_inv; args.jj_fwd = jj_fwd;
       args.order = opt.block_order & 1;
+      args.nt = stream_hint(nv);
       const dim3 grid((unsigned)((long long)nv * plan.b * nz));
       const bool steady = (upd == UPD_DEFER || upd == UPD_PAIR) && fwd == FWD_GAMMA && !args.st.norms;
       const PassRFn fn = steady ? pr->fn_steady : pr->fn;
@@ -493,6 +497,7 @@ struct Solver {
       PassKArgs args;
       args.tb = tb; args.c = ck; args.spec = spec.p; args.parity = parity.p; args.rhs_stride = nvox; args.jj = jj;
       args.order = (opt.block_order >> 1) & 1;
+      args.nt = stream_hint(nv);
       const int ntile = (nx + pk->t - 1) / pk->t;
       const dim3 grid((unsigned)((long long)nb * plan.a * ntile));
       timed(1, [&] { hipLaunchKernelGGL(pk->fn, grid, dim3(pk->nt), 0, stream, args); });
@@ -525,6 +530,7 @@ struct Solver {
       if (opt.track_norms && jj_inv > 0) args.st.norms = norms.p + (size_t)(jj_inv - 1) * nv;
       args.upd = upd; args.fwd = fwd; args.jj_inv = jj_inv; args.jj_fwd = jj_fwd;
       args.order = 0;
+      args.nt = stream_hint(nv);
       const dim3 grid((unsigned)((long long)nv * plan.b * nz));
       const bool steady = (upd == UPD_DEFER || upd == UPD_PAIR) && fwd == FWD_GAMMA && !args.st.norms;
       const PassRFn fn = steady ? pr->fn_steady : pr->fn;
@@ -534,6 +540,7 @@ struct Solver {
       PassKArgs args;
       args.tb = tb; args.c = ck; args.spec = spec.p; args.parity = parity.p; args.rhs_stride = nvox; args.jj = jj;
       args.order = 0;
+      args.nt = stream_hint(nv);
       const int ntile = (nx + pk->t - 1) / pk->t;
       const dim3 grid((unsigned)((long long)nb * ny * ntile));
       timed(1, [&] { hipLaunchKernelGGL(pk->fn, grid, dim3(pk->nt), 0, stream, args); });
