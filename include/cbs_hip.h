@@ -115,6 +115,24 @@ int cbs_get_options(cbs_handle h, cbs_options* opt);
  * shift state, as set_RI's init() does (:280-396). */
 int cbs_set_potential(cbs_handle h, const cbs_potential_params* prm, const void* V, int mem);
 
+/* The host half of set_RI moved to the device.  The RI grid (RI_size, x fastest)
+ * is FP64: real values (ri_complex = 0) or interleaved complex (1), on the host or
+ * the device (mem).  Vt = k^2 (RI^2 / RI_bg^2 - 1), k = 2 pi RI_bg / wavelength
+ * (Codes/sub_code/RI2potential.m:1-4), evaluated in FP64 with the operation order
+ * of the host formula, so real RI gives the same V bit for bit.
+ *
+ * cbs_potential_max: max |Vt| over the grid (eps_imag = 1.01 of it,
+ * FORWARD_SOLVER_CONVERGENT_BORN.m:205; the caller also derives Bornmax from it,
+ * :326-336).  The same for conj(RI).
+ * cbs_set_potential_ri: as cbs_set_potential, with V = pad(Vt) - i eps_imag built
+ * on the device (:212-215, create_boundary_RI :238-241) from the RI grid instead of
+ * uploaded; flip_conj = 1 uses conj(RI) reversed along z, the adjoint potential of
+ * BACKWARD_SOLVER_MULTI.m:48, so one resident RI serves both solves. */
+int cbs_potential_max(cbs_handle h, const void* ri, int ri_complex, double wavelength, double RI_bg, int mem,
+                      double* vmax);
+int cbs_set_potential_ri(cbs_handle h, const cbs_potential_params* prm, const void* ri, int ri_complex, int flip_conj,
+                         double wavelength, double RI_bg, int mem);
+
 /* Born loop from the incident 3D field: u_out = Field[ROI] + u_in for n_rhs
  * right-hand sides (RI grid each).  Replaces CudaBorn action 2 (main.cpp:139-163)
  * and the MATLAB loop FORWARD_SOLVER_CONVERGENT_BORN.solve_raw :564-647.

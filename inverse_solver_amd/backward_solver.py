@@ -4,8 +4,9 @@ reconstruction loop (Codes/BACKWARD_SOLVER_MULTI.m).
 
 The forward potential V and the adjoint potential V' (from conj(flip_z(RI)),
 :48) live on the device in two solver handles that share one set of 2D
-projection kernels, so a gradient needs no potential re-upload beyond the two
-set_RI calls the reference also makes.
+projection kernels.  A gradient uploads the RI grid once; both potentials are
+formed from it on the device (the adjoint one by reading it conjugated and
+reversed along z), which replaces the two host-side set_RI calls of the reference.
 """
 from __future__ import annotations
 
@@ -70,6 +71,21 @@ class BACKWARD_SOLVER_MULTI:
             self._nbuf = A
         return self._bufs
 
+    def _ri_device(self, RI):
+        """The RI grid as FP64 on the device (x fastest; complex only when it has an
+        imaginary part), in a buffer kept between gradients."""
+        from . import device as dv
+        RI = np.asarray(RI)
+        cplx = bool(np.iscomplexobj(RI) and np.any(np.imag(RI) != 0))
+        arr = (np.asfortranarray(RI, dtype=np.complex128) if cplx
+               else np.asfortranarray(np.real(RI), dtype=np.float64))
+        buf = getattr(self, '_ri_buf', None)
+        if buf is None or buf.nbytes != arr.nbytes:
+            self._ri_buf = None
+            self._ri_buf = dv.DeviceArray((arr.size,), arr.dtype)
+        self._ri_buf.upload(arr)
+        return self._ri_buf, cplx
+
     def get_gradiant_static(self, RI, input_field, output_field):
         """Codes/BACKWARD_SOLVER_MULTI.m:30-54: forward solve, residual, adjoint solve on
         conj(flip_z(RI)) with conj(residual), gradient -i mean(conj(u) w).  Both 3D fields
@@ -81,11 +97,15 @@ class BACKWARD_SOLVER_MULTI:
         inp = self.forward._inputs(input_field)
         A = inp.shape[3]
         du, dw, dg = self._fields(A)
-        self.forward.set_RI(RI)                                                   # :32
+        if list(RI.shape[:3]) != self.forward.expected_RI_size:
+            raise ValueError('The refractiv index does not have the expected size : %d %d %d'
+                             % tuple(self.forward.expected_RI_size))
+        ri_dev, cplx = self._ri_device(RI.reshape(self.forward.expected_RI_size))
+        vmax = self.forward.set_RI_device(ri_dev.ptr, cplx)                      # :32
         trans, _ = self.forward.solve_keep_3d(inp, du)                            # :34
         diff = trans.astype(np.complex128) - np.asarray(output_field)            # :36
         err = float(np.sum(np.abs(diff) ** 2))                                    # :37
-        self.adjoint.set_RI(np.conj(np.flip(RI, axis=2)))                         # :48
+        self.adjoint.set_RI_device(ri_dev.ptr, cplx, flip_conj=True, vmax=vmax)  # :48 conj(flip_z(RI))
         self.adjoint.solve_keep_3d(np.conj(diff), dw, return_2d=False)            # :49
         dims = (C.c_uint32 * 3)(*self.forward.expected_RI_size)
         # mean over polarisation components and angles (:53 averages dims 4 and 5)

@@ -115,4 +115,62 @@ __global__ void finalize_field(const float2* __restrict__ field, const float2* _
   }
 }
 
+// Vt = k^2 (RI^2 / n_m^2 - 1) (Codes/sub_code/RI2potential.m:1-4) in FP64, with the
+// operations in the order and rounding of the host restatement in complex128 NumPy
+// (no FMA contraction; the division of a complex by the real n_m^2 is a product with
+// its reciprocal there), so that both give the same potential bit for bit.
+// conj: use conj(RI).
+__device__ __forceinline__ double2 ri_potential(const RiGrid& g, long long i, int conj) {
+#pragma clang fp contract(off)
+  double re, im;
+  if (g.cplx) {
+    re = g.ri[2 * i];
+    im = conj ? -g.ri[2 * i + 1] : g.ri[2 * i + 1];
+  } else {
+    re = g.ri[i];
+    im = 0.0;
+  }
+  const double sr = re * re - im * im, si = re * im + im * re;
+  return make_double2(g.kk * (sr * g.inv_n2 - 1.0), g.kk * (si * g.inv_n2));
+}
+
+// Per-block max |Vt| over the RI grid (the max of eps = 1.01 max|Vt|,
+// Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:205); blockDim.x = 256.
+__global__ void potential_vmax(RiGrid g, double* __restrict__ block_max) {
+  __shared__ double s_max[4];
+  const long long n = (long long)g.nx * g.ny * g.nz;
+  double m = 0.0;
+  GRID_STRIDE(i, n) {
+    const double2 v = ri_potential(g, i, 0);
+    m = fmax(m, hypot(v.x, v.y));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) block_max[blockIdx.x] = fmax(fmax(s_max[0], s_max[1]), fmax(s_max[2], s_max[3]));
+}
+
+// V = Vt - i eps on the simulation grid: Vt of the RI grid inside the ROI, 0 in the
+// boundary (Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:212-215,238-241), rounded to complex64.
+// flip_conj: the RI taken as conj(RI) reversed along z, as the adjoint solve of
+// Codes/BACKWARD_SOLVER_MULTI.m:48 uses it.
+__global__ void potential_fill(RiGrid g, int flip_conj, RoiGeom rg, int snx, int sny, long long snvox, double eps,
+                               float2* __restrict__ V) {
+#pragma clang fp contract(off)
+  GRID_STRIDE(i, snvox) {
+    const int x = (int)(i % snx);
+    const long long yz = i / snx;
+    const int y = (int)(yz % sny);
+    const int z = (int)(yz / sny);
+    const int xr = x - rg.ox, yr = y - rg.oy, zr = z - rg.oz;
+    double2 vt = make_double2(0.0, 0.0);
+    if (xr >= 0 && xr < rg.nx && yr >= 0 && yr < rg.ny && zr >= 0 && zr < rg.nz) {
+      const int zs = flip_conj ? rg.nz - 1 - zr : zr;
+      vt = ri_potential(g, ((long long)zs * rg.ny + yr) * rg.nx + xr, flip_conj);
+    }
+    V[i] = make_float2((float)(vt.x - 0.0), (float)(vt.y - eps));
+  }
+}
+
 }  // namespace cbs

@@ -17,6 +17,18 @@ __global__ void seed_source(const float2* __restrict__ u_in, const float2* __res
 __global__ void finalize_field(const float2* __restrict__ field, const float2* __restrict__ u_in, float2* __restrict__ u_out,
                                Consts c, RoiGeom rg, long long rhs_stride, int nrhs);
 
+// RI grid given as FP64 values, x fastest: real (cplx = 0) or interleaved complex.
+struct RiGrid {
+  const double* ri;
+  int cplx;
+  int nx, ny, nz;        // RI grid size
+  double kk;             // k^2 with k = 2 pi n_m / lambda
+  double inv_n2;         // 1 / n_m^2
+};
+__global__ void potential_vmax(RiGrid g, double* __restrict__ block_max);
+__global__ void potential_fill(RiGrid g, int flip_conj, RoiGeom rg, int snx, int sny, long long snvox, double eps,
+                               float2* __restrict__ V);
+
 // ---- registry of templated pass kernels (passes_inst.hip)
 using PassRFn = void (*)(struct PassRArgs);
 using PassKFn = void (*)(struct PassKArgs);

@@ -1117,6 +1117,77 @@ struct Solver {
   void set_potential(const cbs_potential_params* p, const float2* v, int mem) {
     require(p != nullptr && v != nullptr, CBS_ERR_ARG, "null potential argument");
     require(mem == CBS_MEM_HOST || mem == CBS_MEM_DEVICE, CBS_ERR_ARG, "mem must be CBS_MEM_HOST or CBS_MEM_DEVICE");
+    check_params(p);
+    prm = *p;
+    drop_graph();
+    V.ensure((size_t)nvox);
+    HIP_CHECK(hipMemcpyAsync(V.p, v, (size_t)nvox * sizeof(float2),
+                             mem == CBS_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, stream));
+    finish_potential();
+  }
+
+  // ---- the RI -> V conversion of set_RI on the device
+  DevBuf<double> ri_stage, vmax_part;
+  RiGrid ri_grid(const void* ri, int cplx, double wavelength, double n_m, int mem) {
+    require(ri != nullptr, CBS_ERR_ARG, "null RI pointer");
+    require(cplx == 0 || cplx == 1, CBS_ERR_ARG, "ri_complex must be 0 or 1");
+    require(mem == CBS_MEM_HOST || mem == CBS_MEM_DEVICE, CBS_ERR_ARG, "mem must be CBS_MEM_HOST or CBS_MEM_DEVICE");
+    require(wavelength > 0 && n_m > 0 && std::isfinite(wavelength) && std::isfinite(n_m), CBS_ERR_ARG,
+            "wavelength and RI_bg must be > 0");
+    RiGrid g;
+    const size_t n = (size_t)nri * (cplx ? 2 : 1);
+    if (mem == CBS_MEM_DEVICE) {
+      g.ri = reinterpret_cast<const double*>(ri);
+    } else {
+      ri_stage.ensure(n);
+      HIP_CHECK(hipMemcpyAsync(ri_stage.p, ri, n * sizeof(double), hipMemcpyHostToDevice, stream));
+      g.ri = ri_stage.p;
+    }
+    g.cplx = cplx;
+    g.nx = rx; g.ny = ry; g.nz = rz;
+    const double k = 2.0 * M_PI * n_m / wavelength;   // the host restatement's k, then k * k
+    g.kk = k * k;
+    g.inv_n2 = 1.0 / (n_m * n_m);
+    return g;
+  }
+  double ri_vmax(const void* ri, int cplx, double wavelength, double n_m, int mem) {
+    const RiGrid g = ri_grid(ri, cplx, wavelength, n_m, mem);
+    const int nb = (int)std::min<long long>(1024, (nri + 255) / 256);
+    vmax_part.ensure((size_t)nb);
+    hipLaunchKernelGGL(potential_vmax, dim3(nb), dim3(256), 0, stream, g, vmax_part.p);
+    HIP_CHECK(hipGetLastError());
+    std::vector<double> part((size_t)nb);
+    HIP_CHECK(hipMemcpyAsync(part.data(), vmax_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    double m = 0.0;
+    for (double v : part) m = std::max(m, v);
+    return m;
+  }
+  void set_potential_ri(const cbs_potential_params* p, const void* ri, int cplx, int flip_conj, double wavelength,
+                        double n_m, int mem) {
+    require(p != nullptr, CBS_ERR_ARG, "null potential argument");
+    require(flip_conj == 0 || flip_conj == 1, CBS_ERR_ARG, "flip_conj must be 0 or 1");
+    check_params(p);
+    const RiGrid g = ri_grid(ri, cplx, wavelength, n_m, mem);
+    prm = *p;
+    drop_graph();
+    V.ensure((size_t)nvox);
+    const RoiGeom rg = roi();
+    hipLaunchKernelGGL(potential_fill, dim3(grid_for(nvox)), dim3(256), 0, stream, g, flip_conj, rg, nx, ny,
+                       (long long)nvox, p->eps_imag, V.p);
+    HIP_CHECK(hipGetLastError());
+    finish_potential();
+  }
+
+  void finish_potential() {
+    make_plan();
+    build_tables();
+    HIP_CHECK(hipStreamSynchronize(stream));
+    flip_state = 0;
+    have_potential = true;
+  }
+
+  void check_params(const cbs_potential_params* p) const {
     require(p->eps_imag > 0 && std::isfinite(p->eps_imag), CBS_ERR_ARG,
             "eps_imag must be > 0 (a homogeneous RI has eps_imag = 0 and is not solvable by CBS)");
     require(p->k0_nm > 0, CBS_ERR_ARG, "k0_nm must be > 0");
@@ -1131,16 +1202,6 @@ struct Solver {
       require(lo - 1 == dims[j] - hi, CBS_ERR_ARG, "ROI must be centred (equal boundary on both sides) on axis " +
                                                        std::to_string(j));
     }
-    prm = *p;
-    drop_graph();
-    V.ensure((size_t)nvox);
-    HIP_CHECK(hipMemcpyAsync(V.p, v, (size_t)nvox * sizeof(float2),
-                             mem == CBS_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, stream));
-    make_plan();
-    build_tables();
-    HIP_CHECK(hipStreamSynchronize(stream));
-    flip_state = 0;
-    have_potential = true;
   }
 };
 
@@ -1347,6 +1408,25 @@ int cbs_set_potential(cbs_handle h, const cbs_potential_params* prm, const void*
     Solver* s = lookup(h);
     DeviceGuard dg(s->device);
     s->set_potential(prm, reinterpret_cast<const float2*>(V), mem);
+  });
+}
+
+int cbs_potential_max(cbs_handle h, const void* ri, int ri_complex, double wavelength, double RI_bg, int mem,
+                      double* vmax) {
+  return guarded([&] {
+    require(vmax != nullptr, CBS_ERR_ARG, "null vmax pointer");
+    Solver* s = lookup(h);
+    DeviceGuard dg(s->device);
+    *vmax = s->ri_vmax(ri, ri_complex, wavelength, RI_bg, mem);
+  });
+}
+
+int cbs_set_potential_ri(cbs_handle h, const cbs_potential_params* prm, const void* ri, int ri_complex, int flip_conj,
+                         double wavelength, double RI_bg, int mem) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    DeviceGuard dg(s->device);
+    s->set_potential_ri(prm, ri, ri_complex, flip_conj, wavelength, RI_bg, mem);
   });
 }
 

@@ -73,6 +73,13 @@ class DeviceArray:
         _chk(hip.hipMemcpy(C.c_void_p(d.ptr), a.ctypes.data_as(C.c_void_p), C.c_size_t(d.nbytes), _H2D), 'H2D')
         return d
 
+    def upload(self, a):
+        """Copy a host array of the same byte size (in its memory order) into the buffer."""
+        a = np.asarray(a)
+        if not (a.flags['C_CONTIGUOUS'] or a.flags['F_CONTIGUOUS']) or a.nbytes != self.nbytes:
+            raise ValueError('upload needs a contiguous array of %d bytes' % self.nbytes)
+        _chk(hip.hipMemcpy(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), C.c_size_t(self.nbytes), _H2D), 'H2D')
+
     def to_host(self):
         out = np.empty(self.shape, self.dtype)
         _chk(hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), C.c_size_t(self.nbytes), _D2H),

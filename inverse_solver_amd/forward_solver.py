@@ -21,7 +21,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, lib
-from .params import (basic_optical_parameter, update_struct, round_half_away, ri_to_potential, pupil,
+from .params import (basic_optical_parameter, update_struct, round_half_away, pupil,
                      centered_fft2, centered_ifft2, polarisation_basis)
 from .io_kernels import build_io_kernels
 
@@ -132,27 +132,51 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
                 'block_order': o.block_order}
 
     # ---------------------------------------------------------------- potential (:180-396)
-    def set_RI(self, RI):
-        """Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:180-198 (scalar RI)."""
-        p = self.parameters
+    def set_RI(self, RI, flip_conj=False):
+        """Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:180-198 (scalar RI).  The RI grid goes to
+        the device once; RI2potential, eps = 1.01 max|Vt| and the padded V = Vt - i eps
+        are formed there (cbs_potential_max, cbs_set_potential_ri).  flip_conj: use
+        conj(RI) reversed along z (the adjoint potential of BACKWARD_SOLVER_MULTI.m:48)."""
         RI = np.asarray(RI)
         if list(RI.shape[:3]) != self.expected_RI_size or (RI.ndim > 3 and int(np.prod(RI.shape[3:])) != 1):
             raise ValueError('The refractiv index does not have the expected size : %d %d %d'
                              % tuple(self.expected_RI_size))
         RI = RI.reshape(self.expected_RI_size)
-        Vt = ri_to_potential(RI.astype(np.complex128), p['wavelength'], p['RI_bg'])
-        eps = float(np.max(np.abs(Vt)) * 1.01)                              # :205
+        cplx = bool(np.iscomplexobj(RI) and np.any(np.imag(RI) != 0))
+        arr = (np.asfortranarray(RI, dtype=np.complex128) if cplx
+               else np.asfortranarray(np.real(RI), dtype=np.float64))
+        self.set_RI_device(arr.ctypes.data, cplx, flip_conj, mem=_lib.CBS_MEM_HOST)
+        self.RI = RI
+
+    def set_RI_device(self, ri_ptr, ri_complex, flip_conj=False, vmax=None, mem=None):
+        """set_RI from an FP64 RI grid (x fastest; real, or interleaved complex when
+        ri_complex) at ri_ptr, device memory unless mem says otherwise.  vmax: a max|Vt|
+        already measured on the same grid (conj and flip do not change it).  Returns it."""
+        p = self.parameters
+        mem = _lib.CBS_MEM_DEVICE if mem is None else mem
+        ptr = C.c_void_p(int(ri_ptr))
+        if vmax is None:
+            out = C.c_double()
+            check(lib.cbs_potential_max(self._h, ptr, int(bool(ri_complex)), float(p['wavelength']),
+                                        float(p['RI_bg']), mem, C.byref(out)))
+            vmax = out.value
+        prm = self._potential_params(float(vmax * 1.01))                   # :205
+        check(lib.cbs_set_potential_ri(self._h, C.byref(prm), ptr, int(bool(ri_complex)), int(bool(flip_conj)),
+                                       float(p['wavelength']), float(p['RI_bg']), mem))
+        return vmax
+
+    def _potential_params(self, eps):
+        """The scalars of set_RI / init for eps_imag = eps (:199-222, :280-396); sets
+        eps_imag, ROI and Bornmax on the object."""
+        p = self.parameters
         k2 = (2 * np.pi * p['RI_bg'] / p['wavelength']) ** 2
         if not eps > 1e-12 * k2:
             raise ValueError('homogeneous RI: eps_imag = 0 and the Born series is undefined '
                              '(the reference would divide by zero)')
         b = self.boundary_thickness_pixel
-        Vpad = np.pad(Vt, [(int(b[0]),) * 2, (int(b[1]),) * 2, (int(b[2]),) * 2])   # :238-241
-        V = (Vpad - 1j * eps).astype(np.complex64)                          # :212-215
-        self.RI = RI
+        rs = self.expected_RI_size
         self.eps_imag = eps
-        self.Vt_roi = Vt
-        self.ROI = [b[0] + 1, b[0] + RI.shape[0], b[1] + 1, b[1] + RI.shape[1], b[2] + 1, b[2] + RI.shape[2]]
+        self.ROI = [b[0] + 1, b[0] + rs[0], b[1] + 1, b[1] + rs[1], b[2] + 1, b[2] + rs[2]]
         k0_nm = p['RI_bg'] / p['wavelength']
         step = abs(2 * (2 * np.pi * k0_nm) / eps)                            # :326
         res = np.array(p['resolution'], float)
@@ -175,9 +199,7 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
         prm.acyclic = int(acyclic)
         prm.shift_axes = shift_axes
         prm.ramp_axes = ramp_axes
-        Vf = np.asfortranarray(V)
-        check(lib.cbs_set_potential(self._h, C.byref(prm), _fptr(Vf), _lib.CBS_MEM_HOST))
-        self._Vf = Vf
+        return prm
 
     # ---------------------------------------------------------------- 2D in / 2D out (:397-520)
     def _ensure_io(self):
