@@ -32,8 +32,13 @@ def get_default_parameters(init=None):
 class BACKWARD_SOLVER_MULTI:
     get_default_parameters = staticmethod(get_default_parameters)
 
-    def __init__(self, params, device=0, options=None):
+    def __init__(self, params, device=0, options=None, autotune=None):
+        """autotune: measure the kernel plans once, at the first gradient, on the
+        forward handle and give the adjoint handle the same plan (default: when no
+        options force a plan)."""
         self.parameters = dict(params)
+        self._autotune = (not options) if autotune is None else bool(autotune)
+        self._tuned = False
         fp = dict(self.parameters['forward_solver_parameters'])
         if not (fp.get('return_transmission') and fp.get('return_reflection') and fp.get('return_3D')):
             raise ValueError('need to set all return parameter to true in the forward solver')   # :57-66
@@ -86,6 +91,17 @@ class BACKWARD_SOLVER_MULTI:
         self._ri_buf.upload(arr)
         return self._ri_buf, cplx
 
+    def _tune(self, A):
+        from ._lib import CbsError
+        self._tuned = True
+        try:
+            self.forward.autotune(A, 6)
+        except CbsError:
+            return                                   # no fused plan on this grid: keep the planner's choice
+        pl = self.forward.plan()
+        self.adjoint.set_options(form=0, split_a=pl['a'], tile_t=pl['tile_t'], threads_real=pl['threads_real'],
+                                 threads_kspace=pl['threads_kspace'], block_order=pl['block_order'])
+
     def get_gradiant_static(self, RI, input_field, output_field):
         """Codes/BACKWARD_SOLVER_MULTI.m:30-54: forward solve, residual, adjoint solve on
         conj(flip_z(RI)) with conj(residual), gradient -i mean(conj(u) w).  Both 3D fields
@@ -102,6 +118,8 @@ class BACKWARD_SOLVER_MULTI:
                              % tuple(self.forward.expected_RI_size))
         ri_dev, cplx = self._ri_device(RI.reshape(self.forward.expected_RI_size))
         vmax = self.forward.set_RI_device(ri_dev.ptr, cplx)                      # :32
+        if self._autotune and not self._tuned:
+            self._tune(A)
         trans, _ = self.forward.solve_keep_3d(inp, du)                            # :34
         diff = trans.astype(np.complex128) - np.asarray(output_field)            # :36
         err = float(np.sum(np.abs(diff) ** 2))                                    # :37
