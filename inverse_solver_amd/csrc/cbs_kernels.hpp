@@ -16,6 +16,8 @@
 //                +-1/4-bin shifted grid and 1/N, inverse z-FFT + inverse b-DFT +
 //                conjugate cross twiddle
 #pragma once
+#include <type_traits>
+
 #include "fft_core.hpp"
 
 namespace cbs {
@@ -170,6 +172,13 @@ __device__ __forceinline__ float4 load_stream(const float4* src, bool nt) {
     return make_float4(t.x, t.y, t.z, t.w);
   }
   return *src;
+}
+
+// T* at a 32-bit byte offset from a (block-uniform) base pointer
+template <class T, class B>
+__device__ __forceinline__ T* byte_at(B* base, unsigned byte_off) {
+  using C = typename std::conditional<std::is_const<B>::value, const char, char>::type;
+  return reinterpret_cast<T*>(reinterpret_cast<C*>(base) + byte_off);
 }
 
 __device__ __forceinline__ float2 lo2(float4 v) { return make_float2(v.x, v.y); }

@@ -113,9 +113,11 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
   // pointwise update, CH pairs per thread at a time
   const bool rf = reads_f(upd, fwd), rF = reads_F(upd);
   const bool wF = writes_F(upd), wf = fwd == FWD_GAMMA;
-  const float2* __restrict__ V = args.st.V;
-  float2* __restrict__ F = args.st.field + roff;
-  float2* __restrict__ f = args.st.fstate + roff;
+  // block-uniform bases of this z plane; per-lane offsets inside a plane fit in 32 bits
+  const long long zoff = (long long)z * c.ny * NX;
+  const float2* __restrict__ V = args.st.V + zoff;
+  float2* __restrict__ F = args.st.field + roff + zoff;
+  float2* __restrict__ f = args.st.fstate + roff + zoff;
   // the steady instantiation never tracks norms (the host launches the general one then)
   const bool want_norm = !STEADY && args.st.norms != nullptr;
   double nrm = 0.0;
@@ -123,27 +125,29 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
   for (int k0 = 0; k0 < L; k0 += CH) {
     float4 vv[CH], xx[CH], FF[CH], rx[CH];
     float2 mx[CH];
-    long long gi[CH];
+    unsigned gi[CH];     // byte offsets inside the z plane (< 2^32): base + 32-bit offset addressing
+    bool ok[CH];
 #pragma unroll
     for (int k = 0; k < CH; ++k) {
       const int i = tid + (k0 + k) * NT;
-      gi[k] = -1;
-      if (k0 + k < L && i < E2) {
-        const int e = 2 * i;
-        const int y1 = e / NX, x = e - y1 * NX;
-        const long long g = ((long long)z * c.ny + (ROWS ? y2 * A + y1 : y1 * c.b + y2)) * NX + x;
+      ok[k] = k0 + k < L && i < E2;
+      gi[k] = 0;
+      if (ok[k]) {
+        const unsigned e = 2u * (unsigned)i;
+        const unsigned y1 = e / NX, x = e - y1 * NX;
+        const unsigned g = ((ROWS ? (unsigned)y2 * A + y1 : y1 * (unsigned)c.b + (unsigned)y2) * NX + x) * 8u;
         gi[k] = g;
-        vv[k] = *reinterpret_cast<const float4*>(V + g);
+        vv[k] = *byte_at<const float4>(V, g);
         // per-axis x ramp and mask pairs (read through the cache: the LDS holds three tiles per CU)
-        rx[k] = *reinterpret_cast<const float4*>(tb.ramp_x + x);
-        mx[k] = *reinterpret_cast<const float2*>(tb.mask_x + x);
-        if (rf) xx[k] = load_stream(reinterpret_cast<const float4*>(f + g), args.nt);
-        if (rF) FF[k] = load_stream(reinterpret_cast<const float4*>(F + g), args.nt);
+        rx[k] = *byte_at<const float4>(tb.ramp_x, x * 8u);
+        mx[k] = *byte_at<const float2>(tb.mask_x, x * 4u);
+        if (rf) xx[k] = load_stream(byte_at<const float4>(f, g), args.nt);
+        if (rF) FF[k] = load_stream(byte_at<const float4>(F, g), args.nt);
       }
     }
 #pragma unroll
     for (int k = 0; k < CH; ++k) {
-      if (gi[k] < 0) continue;
+      if (!ok[k]) continue;
       const int i = tid + (k0 + k) * NT;
       const int e = 2 * i;
       const int y1 = e / NX, x = e - y1 * NX;
@@ -158,8 +162,8 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
                                   c.inv_eps, sg_inv, sg_fwd, Fn0, xn0, want_norm, nrm);
       const float2 p1 = born_step(upd, fwd, q1, cmul(hi2(rx[k]), ryz), mx[k].y * myz, hi2(vv[k]), hi2(xo4),
                                   hi2(Fo4), c.inv_eps, sg_inv, sg_fwd, Fn1, xn1, want_norm, nrm);
-      if (wF) store_stream(reinterpret_cast<float4*>(F + gi[k]), pack2(Fn0, Fn1), args.nt);
-      if (wf) store_stream(reinterpret_cast<float4*>(f + gi[k]), pack2(xn0, xn1), args.nt);
+      if (wF) store_stream(byte_at<float4>(F, gi[k]), pack2(Fn0, Fn1), args.nt);
+      if (wf) store_stream(byte_at<float4>(f, gi[k]), pack2(xn0, xn1), args.nt);
       if (fwd != FWD_NONE) {
         lds[y1 * PR + x] = p0;
         lds[y1 * PR + x + 1] = p1;
