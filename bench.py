@@ -338,8 +338,12 @@ def reference_form_gpu(s, u0, M, log_fn):
     b = [int(x) for x in s.boundary_thickness_pixel]
     p = s.parameters
     eps = float(s.eps_imag)
-    # arrays held as (z, y, x) so x is the fastest axis, as in MATLAB's column-major order
-    V = torch.from_numpy(np.ascontiguousarray(s._Vf.T)).to(dev)
+    # arrays held as (z, y, x) so x is the fastest axis, as in MATLAB's column-major order;
+    # V formed on the host as set_RI does in MATLAB (:205-215, :238-241, RI2potential.m)
+    from inverse_solver_amd.params import ri_to_potential
+    Vt = ri_to_potential(np.asarray(s.RI).astype(np.complex128), p['wavelength'], p['RI_bg'])
+    Vh = (np.pad(Vt, [(b[0],) * 2, (b[1],) * 2, (b[2],) * 2]) - 1j * eps).astype(np.complex64)
+    V = torch.from_numpy(np.ascontiguousarray(Vh.T)).to(dev)
     def axis_shape(j):
         shp = [1, 1, 1]
         shp[2 - j] = -1
@@ -538,6 +542,11 @@ def main():
         print(json.dumps(out), flush=True)
         return
 
+    if not args.no_refform:
+        # the reference-form baseline runs on torch; its HIP context is created before the
+        # solver library's (in the other order torch finds no device on these machines)
+        import torch
+        torch.zeros(1, device='cuda:0')
     from inverse_solver_amd import FORWARD_SOLVER_CONVERGENT_BORN
     from inverse_solver_amd import device as dv
     from inverse_solver_amd.phantoms import spheres, incident_plane_waves

@@ -32,3 +32,16 @@ def test_argument_errors_without_gpu():
     assert lib.cbs_destroy(C.c_void_p(12345)) == _lib.CBS_ERR_HANDLE
     assert 'handle' in lib.cbs_last_error().decode()
     assert lib.cbs_solve_3d(None, None, 1, None, 0) == _lib.CBS_ERR_HANDLE
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU; --gpus N > 1 exits with status 2 and a JSON message."""
+    import json
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, 'bench.py'), '--gpus', '2'], capture_output=True,
+                       text=True, timeout=120, cwd=root)
+    assert r.returncode == 2
+    assert 'one GPU' in json.loads(r.stdout.strip().splitlines()[-1])['error']
