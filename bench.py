@@ -532,8 +532,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({'error': 'this build runs on one GPU; --gpus %d is not supported (the angle-parallel '
-                                   'sharding across GPUs is described in DESIGN.md)' % args.gpus}))
+        print(json.dumps({'error': 'bench.py runs on one GPU; --gpus %d is not supported (the angle-parallel '
+                                   'sharding across GPUs is designed but not built)' % args.gpus}))
         sys.exit(2)
 
     if args.workload != 'batched':
