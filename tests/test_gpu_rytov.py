@@ -4,7 +4,7 @@
   arithmetic) on random maps (many residues), vortices and smooth maps, odd sizes.
 * cbs_phase_integrate equals the oracle's raster unwrap exactly, with zero and with
   random step maps.
-* unwrap_phase on maps with residues (the branch cuts are this build's own):
+* unwrap_phase on maps with residues (the branch-cut placement is this library's own):
   the result is congruent to the input modulo 2 pi, and every neighbour difference
   between pixels that are not on a cut is the wrapped difference.
 * BACKWARD_SOLVER_RYTOV equals the oracle's restatement (residue-free inputs) to 1e-6.
