@@ -136,7 +136,9 @@ int cbs_set_potential_ri(cbs_handle h, const cbs_potential_params* prm, const vo
 /* Born loop from the incident 3D field: u_out = Field[ROI] + u_in for n_rhs
  * right-hand sides (RI grid each).  Replaces CudaBorn action 2 (main.cpp:139-163)
  * and the MATLAB loop FORWARD_SOLVER_CONVERGENT_BORN.solve_raw :564-647.
- * u_in and u_out are both host or both device pointers (mem); they may alias. */
+ * u_in and u_out are both host or both device pointers (mem); they may alias.
+ * With host pointers and enough right-hand sides (about six chunks of at least 8),
+ * the copies of the neighbouring chunks run on a second stream while a chunk solves. */
 int cbs_solve_3d(cbs_handle h, const void* u_in, uint32_t n_rhs, void* u_out, int mem);
 
 /* Uploads the refocusing and transmission kernels for the 2D input/output

@@ -142,10 +142,18 @@ struct Solver {
   std::vector<std::pair<int, int>> ev_used;   // (kind, index of start event)
   double ktime_ms[3] = {0, 0, 0};
   uint64_t kcount[3] = {0, 0, 0};
-  // graph cache
-  hipGraphExec_t graph_exec = nullptr;
-  hipGraph_t graph = nullptr;
-  std::vector<long long> graph_key;
+  // graph cache: two captured loops (a pipelined host solve alternates between its chunk
+  // width and a shorter last chunk)
+  struct GraphSlot {
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    std::vector<long long> key;
+  };
+  GraphSlot gslot[2];
+  int gslot_next = 0;
+  // overlapped host I/O: a copy stream and its events (created on first use)
+  hipStream_t cstream = nullptr;
+  hipEvent_t ev_up[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr}, ev_down[2] = {nullptr, nullptr};
   // I/O projection state
   IoState io;
   int rc_x = 0, rc_y = 0;
@@ -157,20 +165,24 @@ struct Solver {
   const StridedInfo* io_cols = nullptr;
 
   void drop_graph() {
-    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    graph_exec = nullptr;
-    graph = nullptr;
-    graph_key.clear();
+    for (auto& g : gslot) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      if (g.graph) (void)hipGraphDestroy(g.graph);
+      g = GraphSlot();
+    }
+    gslot_next = 0;
   }
 
   ~Solver() {
     int prev = 0;
     (void)hipGetDevice(&prev);
     (void)hipSetDevice(device);
-    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    drop_graph();
     for (auto e : ev_pool) (void)hipEventDestroy(e);
+    for (int k = 0; k < 2; ++k)
+      for (hipEvent_t e : {ev_up[k], ev_done[k], ev_down[k]})
+        if (e) (void)hipEventDestroy(e);
+    if (cstream) (void)hipStreamDestroy(cstream);
     if (stream) (void)hipStreamDestroy(stream);
     (void)hipSetDevice(prev);
   }
@@ -619,7 +631,7 @@ struct Solver {
   }
   std::vector<uint8_t> host_parity;
 
-  void ensure_workspace(int nb, bool host_io) {
+  void ensure_workspace(int nb, bool host_io, int host_bufs = 1) {
     const size_t nv = (size_t)nb * P;
     const size_t n = nv * nvox;
     field.ensure(n);
@@ -627,8 +639,8 @@ struct Solver {
     spec.ensure(n);
     parity.ensure(nv);
     if (host_io) {
-      uin.ensure(nv * nri);
-      uout.ensure(nv * nri);
+      uin.ensure((size_t)host_bufs * nv * nri);
+      uout.ensure((size_t)host_bufs * nv * nri);
     }
   }
 
@@ -678,20 +690,26 @@ struct Solver {
         const std::vector<long long> key = {nb, M, (long long)field.p, (long long)fstate.p, (long long)spec.p,
                                             (long long)parity.p, (long long)(opt.track_norms ? norms.p : nullptr),
                                             (long long)plan.pr, (long long)plan.pk, (long long)opt.block_order};
-        if (!graph_exec || key != graph_key) {
-          drop_graph();
+        GraphSlot* gs = nullptr;
+        for (auto& g : gslot)
+          if (g.exec && g.key == key) gs = &g;
+        if (!gs) {
+          gs = &gslot[gslot_next];
+          gslot_next ^= 1;
+          if (gs->exec) (void)hipGraphExecDestroy(gs->exec);
+          if (gs->graph) (void)hipGraphDestroy(gs->graph);
+          *gs = GraphSlot();
           HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
           if (plan.form == CBS_FORM_ROWS) run_rows(nb, M);
           else run_fused(nb, M);
-          hipError_t e = hipStreamEndCapture(stream, &graph);
-          if (e != hipSuccess) {
-            graph = nullptr;
-            throw CbsError(CBS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-          }
-          HIP_CHECK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
-          graph_key = key;
+          hipGraph_t captured = nullptr;
+          hipError_t e = hipStreamEndCapture(stream, &captured);
+          if (e != hipSuccess) throw CbsError(CBS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+          gs->graph = captured;
+          HIP_CHECK(hipGraphInstantiate(&gs->exec, gs->graph, nullptr, nullptr, 0));
+          gs->key = key;
         }
-        HIP_CHECK(hipGraphLaunch(graph_exec, stream));
+        HIP_CHECK(hipGraphLaunch(gs->exec, stream));
       } else if (plan.form == CBS_FORM_ROWS) {
         run_rows(nb, M);
       } else {
@@ -719,13 +737,70 @@ struct Solver {
     }
   }
 
+  void ensure_copy_stream() {
+    if (cstream) return;
+    HIP_CHECK(hipStreamCreateWithFlags(&cstream, hipStreamNonBlocking));
+    for (int k = 0; k < 2; ++k) {
+      HIP_CHECK(hipEventCreateWithFlags(&ev_up[k], hipEventDisableTiming));
+      HIP_CHECK(hipEventCreateWithFlags(&ev_done[k], hipEventDisableTiming));
+      HIP_CHECK(hipEventCreateWithFlags(&ev_down[k], hipEventDisableTiming));
+    }
+  }
+
+  // Host buffers, K >= 2 chunks of nbc right-hand sides: the upload of chunk k+1 and the
+  // download of chunk k-1 run on the copy stream while chunk k computes, through two
+  // device staging buffers each way.
+  void solve_3d_host_pipelined(const float2* u_in, uint32_t n_rhs, float2* u_out, int nbc) {
+    ensure_copy_stream();
+    const size_t per = (size_t)P * nri;            // elements per rhs ([rhs][component][z][y][x])
+    const size_t half = (size_t)nbc * per;         // one staging buffer
+    const int K = (int)((n_rhs + (uint32_t)nbc - 1) / (uint32_t)nbc);
+    auto rhs_of = [&](int k) { return (int)std::min<uint32_t>((uint32_t)nbc, n_rhs - (uint32_t)k * (uint32_t)nbc); };
+    HIP_CHECK(hipMemcpyAsync(uin.p, u_in, (size_t)rhs_of(0) * per * sizeof(float2), hipMemcpyHostToDevice, cstream));
+    HIP_CHECK(hipEventRecord(ev_up[0], cstream));
+    for (int k = 0; k < K; ++k) {
+      const int b = k & 1, o = b ^ 1;
+      HIP_CHECK(hipStreamWaitEvent(stream, ev_up[b], 0));                   // chunk k is on the device
+      if (k >= 2) HIP_CHECK(hipStreamWaitEvent(stream, ev_down[b], 0));     // chunk k-2 left uout[b]
+      born_batch(uin.p + b * half, uout.p + b * half, rhs_of(k), k * nbc);
+      HIP_CHECK(hipEventRecord(ev_done[b], stream));
+      if (k + 1 < K) {                                                      // chunk k-1 no longer reads uin[o]
+        if (k >= 1) HIP_CHECK(hipStreamWaitEvent(cstream, ev_done[o], 0));
+        HIP_CHECK(hipMemcpyAsync(uin.p + o * half, u_in + (size_t)(k + 1) * half,
+                                 (size_t)rhs_of(k + 1) * per * sizeof(float2), hipMemcpyHostToDevice, cstream));
+        HIP_CHECK(hipEventRecord(ev_up[o], cstream));
+      }
+      if (k >= 1) {
+        HIP_CHECK(hipStreamWaitEvent(cstream, ev_done[o], 0));
+        HIP_CHECK(hipMemcpyAsync(u_out + (size_t)(k - 1) * half, uout.p + o * half,
+                                 (size_t)rhs_of(k - 1) * per * sizeof(float2), hipMemcpyDeviceToHost, cstream));
+        HIP_CHECK(hipEventRecord(ev_down[o], cstream));
+      }
+    }
+    const int bl = (K - 1) & 1;
+    HIP_CHECK(hipStreamWaitEvent(cstream, ev_done[bl], 0));
+    HIP_CHECK(hipMemcpyAsync(u_out + (size_t)(K - 1) * half, uout.p + bl * half,
+                             (size_t)rhs_of(K - 1) * per * sizeof(float2), hipMemcpyDeviceToHost, cstream));
+    HIP_CHECK(hipStreamSynchronize(cstream));
+  }
+
   void solve_3d(const float2* u_in, uint32_t n_rhs, float2* u_out, int mem) {
     require(have_potential, CBS_ERR_STATE, "cbs_set_potential must be called before solving");
     require(n_rhs >= 1, CBS_ERR_ARG, "n_rhs must be >= 1");
     require(u_in && u_out, CBS_ERR_ARG, "null field pointer");
     require(mem == CBS_MEM_HOST || mem == CBS_MEM_DEVICE, CBS_ERR_ARG, "mem must be CBS_MEM_HOST or CBS_MEM_DEVICE");
-    const int nbw = batch_width(n_rhs);
-    ensure_workspace(nbw, mem == CBS_MEM_HOST);
+    int nbw = batch_width(n_rhs);
+    // host buffers: at least two chunks of >= 8 rhs (about six for large batches) so that
+    // the PCIe copies overlap the solve
+    bool pipe = false;
+    if (mem == CBS_MEM_HOST) {
+      const int nbc = std::min(nbw, std::max((int)((n_rhs + 5) / 6), std::min((int)n_rhs, 8)));
+      if ((int)n_rhs > nbc) {
+        pipe = true;
+        nbw = nbc;
+      }
+    }
+    ensure_workspace(nbw, mem == CBS_MEM_HOST, pipe ? 2 : 1);
     const int M = (int)prm.Bornmax;
     if (opt.track_norms) {
       last_norms.assign((size_t)n_rhs * M, 0.0);
@@ -733,6 +808,13 @@ struct Solver {
       last_norm_iter = M;
     }
     ev_used.clear();
+    if (pipe) {
+      solve_3d_host_pipelined(u_in, n_rhs, u_out, nbw);
+      HIP_CHECK(hipStreamSynchronize(stream));
+      if (opt.time_kernels) resolve_times();
+      if (prm.acyclic) flip_state = (int)((flip_state + (long long)n_rhs * M) & 1);
+      return;
+    }
     for (uint32_t r0 = 0; r0 < n_rhs; r0 += nbw) {
       const int nb = (int)std::min<uint32_t>(nbw, n_rhs - r0);
       const float2* din;

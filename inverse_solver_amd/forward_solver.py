@@ -373,7 +373,7 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
             raise ValueError('incident field must have the RI grid size %s%s' % (
                 self.expected_RI_size, '' if P == 1 else ' x 3 components'))
         A = u.shape[4]
-        uf = np.asfortranarray(u.astype(np.complex64))
+        uf = np.asfortranarray(u, dtype=np.complex64)      # no copy when already complex64, x fastest
         out = np.empty_like(uf, order='F')
         check(lib.cbs_solve_3d(self._h, _fptr(uf), A, _fptr(out), _lib.CBS_MEM_HOST))
         if P == 1:
