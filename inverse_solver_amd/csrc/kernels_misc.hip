@@ -140,15 +140,17 @@ __global__ void potential_vmax(RiGrid g, double* __restrict__ block_max) {
   __shared__ double s_max[4];
   const long long n = (long long)g.nx * g.ny * g.nz;
   double m = 0.0;
+  // a non-finite RI value makes the maximum NaN (as the host max does), so the caller rejects it
+  auto vmax = [](double a, double b) { return (a != a || b != b) ? __builtin_nan("") : fmax(a, b); };
   GRID_STRIDE(i, n) {
     const double2 v = ri_potential(g, i, 0);
-    m = fmax(m, hypot(v.x, v.y));
+    m = vmax(m, hypot(v.x, v.y));
   }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  for (int off = 32; off > 0; off >>= 1) m = vmax(m, __shfl_xor(m, off, 64));
   if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = m;
   __syncthreads();
-  if (threadIdx.x == 0) block_max[blockIdx.x] = fmax(fmax(s_max[0], s_max[1]), fmax(s_max[2], s_max[3]));
+  if (threadIdx.x == 0) block_max[blockIdx.x] = vmax(vmax(s_max[0], s_max[1]), vmax(s_max[2], s_max[3]));
 }
 
 // V = Vt - i eps on the simulation grid: Vt of the RI grid inside the ROI, 0 in the

@@ -1242,7 +1242,7 @@ struct Solver {
     HIP_CHECK(hipMemcpyAsync(part.data(), vmax_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
     HIP_CHECK(hipStreamSynchronize(stream));
     double m = 0.0;
-    for (double v : part) m = std::max(m, v);
+    for (double v : part) m = (std::isnan(v) || std::isnan(m)) ? v + m : std::max(m, v);
     return m;
   }
   void set_potential_ri(const cbs_potential_params* p, const void* ri, int cplx, int flip_conj, double wavelength,

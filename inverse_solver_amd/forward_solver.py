@@ -160,6 +160,8 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
             check(lib.cbs_potential_max(self._h, ptr, int(bool(ri_complex)), float(p['wavelength']),
                                         float(p['RI_bg']), mem, C.byref(out)))
             vmax = out.value
+        if not np.isfinite(vmax):
+            raise ValueError('the refractive index holds non-finite values')
         prm = self._potential_params(float(vmax * 1.01))                   # :205
         check(lib.cbs_set_potential_ri(self._h, C.byref(prm), ptr, int(bool(ri_complex)), int(bool(flip_conj)),
                                        float(p['wavelength']), float(p['RI_bg']), mem))

@@ -110,6 +110,13 @@ def test_device_potential_from_device_buffer_and_errors():
         check(lib.cbs_set_potential_ri(b._h, C.byref(prm), C.c_void_p(buf.ptr), 0, 2, 0.532, 1.336, CBS_MEM_DEVICE))
     with pytest.raises(ValueError):                     # homogeneous RI: eps = 0
         b.set_RI(np.full(ri, 1.336))
+    bad = RI.copy()
+    bad[3, 4, 5] = np.nan
+    with pytest.raises(ValueError, match='non-finite'):
+        b.set_RI(bad)
+    bad[3, 4, 5] = np.inf
+    with pytest.raises(ValueError, match='non-finite'):
+        b.set_RI(bad)
     buf.free()
     a.close()
     b.close()
