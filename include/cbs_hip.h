@@ -201,6 +201,24 @@ int cbs_phase_residues(int device, const float* wrapped, float* residues, const 
 int cbs_phase_integrate(int device, const float* wrapped, const int32_t* step1, const int32_t* step2, float* unwrapped,
                         const uint32_t dims[3], int mem);
 
+/* Goldstein branch cuts: from the residue map (as cbs_phase_residues gives it, not
+ * modified), the step maps that make cbs_phase_integrate unwrap around the cuts:
+ * step1[n1 x n2 x n3] and step2[n2 x n3] are overwritten.  Dipoles (opposite residues in
+ * adjacent loops) are cancelled first; the remaining residues are joined to their nearest
+ * residues or to the border until every tree balances, with the reference's search order,
+ * and the branches are rastered into the step maps.  The dipole pass, the nearest-residue
+ * lists and the raster run on the device, the branch search on the host.  Replaces
+ * remove_dipoles_gpu and goldstein_branch_cut (Codes/sub_code/goldstein_GPU/
+ * remove_dipoles_cuda.cu:50-130, goldstein_branch_cut.m:1-87, get_shortest_sort_cuda.cu,
+ * get_branches_goldstein_compute_v2.cpp:16-203, raster_branches_cuda.cu:65-135). */
+int cbs_phase_branch_cuts(int device, const float* residues, int32_t* step1, int32_t* step2, const uint32_t dims[3],
+                          int mem);
+
+/* The whole unwrap of n3 column-major n1 x n2 wrapped phase maps: residues, branch cuts,
+ * raster integration.  Replaces unwrapp2_gpu (goldstein_GPU/unwrapp2_gpu.m:1-5), as
+ * called by BACKWARD_SOLVER_RYTOV.m:49.  unwrapped may alias wrapped. */
+int cbs_phase_unwrap(int device, const float* wrapped, float* unwrapped, const uint32_t dims[3], int mem);
+
 /* FP64 per-iteration |f_n|^2 of the last solve (track_norms=1): out[r * n_iter + j]. */
 int cbs_get_norms(cbs_handle h, double* out, uint32_t n_rhs, uint32_t n_iter);
 

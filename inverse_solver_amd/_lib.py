@@ -21,7 +21,7 @@ EXPORTED = [
     'cbs_set_potential_ri', 'cbs_solve_3d',
     'cbs_set_io_kernels', 'cbs_solve_2d', 'cbs_get_norms', 'cbs_get_kernel_times', 'cbs_get_stream',
     'cbs_footprint', 'cbs_get_plan', 'cbs_autotune', 'cbs_tv_prox', 'cbs_tv_prox_complex', 'cbs_field_gradient',
-    'cbs_phase_residues', 'cbs_phase_integrate',
+    'cbs_phase_residues', 'cbs_phase_integrate', 'cbs_phase_branch_cuts', 'cbs_phase_unwrap',
     'cbs_last_error', 'cbs_abi_version',
 ]
 
@@ -82,6 +82,8 @@ def _load():
         'cbs_field_gradient': ([vp, vp, vp, u32, C.POINTER(C.c_uint32), vp, i32], i32),
         'cbs_phase_residues': ([i32, vp, vp, C.POINTER(C.c_uint32), i32], i32),
         'cbs_phase_integrate': ([i32, vp, vp, vp, vp, C.POINTER(C.c_uint32), i32], i32),
+        'cbs_phase_branch_cuts': ([i32, vp, vp, vp, C.POINTER(C.c_uint32), i32], i32),
+        'cbs_phase_unwrap': ([i32, vp, vp, C.POINTER(C.c_uint32), i32], i32),
         'cbs_last_error': ([], C.c_char_p),
         'cbs_abi_version': ([], i32),
     }

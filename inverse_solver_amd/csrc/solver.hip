@@ -15,6 +15,7 @@
 
 #include "../../include/cbs_hip.h"
 #include "cbs_passes.hpp"
+#include "goldstein.hpp"
 #include "io_kernels.hpp"
 #include "kernels_misc.hpp"
 #include "tv_kernels.hpp"
@@ -1328,6 +1329,62 @@ static int guarded(F&& f) {
 
 using namespace cbs;
 
+// ------------------------------------------------------------------ Goldstein branch cuts (host part)
+namespace {
+
+// remove_dipoles + goldstein_branch_cut on device-resident data: dres (m1 x m2 x n3,
+// consumed), dstep1 / dstep2 (n1 x n2 x n3, n2 x n3) overwritten with the step maps.
+void gs_step_maps(float* dres, int32_t* dstep1, int32_t* dstep2, int n1, int n2, int n3) {
+  const int m1 = n1 - 1, m2 = n2 - 1;
+  const long long m = (long long)m1 * m2 * n3;
+  HIP_CHECK(hipMemset(dstep1, 0, (size_t)n1 * n2 * n3 * sizeof(int32_t)));
+  HIP_CHECK(hipMemset(dstep2, 0, (size_t)n2 * n3 * sizeof(int32_t)));
+  std::vector<float> res((size_t)m);
+  HIP_CHECK(hipMemcpy(res.data(), dres, m * sizeof(float), hipMemcpyDeviceToHost));
+  std::vector<int32_t> pos;
+  for (long long i = 0; i < m; ++i)
+    if (res[i] > 0.0f) pos.push_back((int32_t)(i + 1));
+  if (!pos.empty()) {
+    DevBuf<int32_t> dpos;
+    dpos.upload(pos);
+    const unsigned grid = (unsigned)((pos.size() + 255) / 256);
+    for (int dir = 0; dir < 4; ++dir)
+      hipLaunchKernelGGL(phase_dipoles, dim3(grid), dim3(256), 0, 0, dres, dpos.p, dstep1, (int)pos.size(), m1, m2, n3,
+                         dir);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpy(res.data(), dres, m * sizeof(float), hipMemcpyDeviceToHost));
+  }
+  const gs::GsList L = gs::gs_list(res, m1, m2, n3);
+  if (L.n == 0) return;
+  DevBuf<int32_t> drows, dtab, dzs, dsq, dsl;
+  drows.upload(L.rows);
+  dtab.upload(L.table);
+  dzs.upload(L.zstart);
+  dsq.upload(L.square);
+  dsl.ensure((size_t)L.n * 2 * gs::kNearest);
+  hipLaunchKernelGGL(phase_nearest, dim3((L.n + 127) / 128), dim3(128), 0, 0, drows.p, L.n, dtab.p, (int)L.table.size(),
+                     dzs.p, dsq.p, m1, m2, dsl.p);
+  HIP_CHECK(hipGetLastError());
+  std::vector<int32_t> sl((size_t)L.n * 2 * gs::kNearest);
+  HIP_CHECK(hipMemcpy(sl.data(), dsl.p, sl.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  const std::vector<int32_t> br = gs::gs_branches(L, sl, m1, m2);
+  const int nb = (int)(br.size() / 6);
+  if (nb > 0) {
+    DevBuf<int32_t> dbr;
+    dbr.upload(br);
+    hipLaunchKernelGGL(phase_raster, dim3((nb + 127) / 128), dim3(128), 0, 0, dbr.p, nb, dstep1, dstep2, n1, n2, n3);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipDeviceSynchronize());
+  }
+}
+
+void check_phase_dims(const uint32_t dims[3]) {
+  require(dims[0] >= 2 && dims[1] >= 2 && dims[2] >= 1, CBS_ERR_ARG, "phase maps need at least 2 x 2 pixels");
+  require((unsigned long long)dims[0] * dims[1] * dims[2] < (1ull << 31), CBS_ERR_UNSUPPORTED,
+          "phase maps of 2^31 pixels or more");
+}
+}  // namespace
+
 extern "C" {
 
 // ------------------------------------------------------------------ phase unwrapping (handle-free)
@@ -1401,6 +1458,72 @@ int cbs_phase_integrate(int device, const float* wrapped, const int32_t* step1, 
                        (int)dims[0], (int)dims[1], (int)dims[2]);
     HIP_CHECK(hipGetLastError());
     if (mem == CBS_MEM_HOST) HIP_CHECK(hipMemcpy(unwrapped, bp.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipDeviceSynchronize());
+  });
+}
+
+int cbs_phase_branch_cuts(int device, const float* residues, int32_t* step1, int32_t* step2, const uint32_t dims[3],
+                          int mem) {
+  return guarded([&] {
+    require(residues && step1 && step2 && dims, CBS_ERR_ARG, "null argument");
+    require(mem == CBS_MEM_HOST || mem == CBS_MEM_DEVICE, CBS_ERR_ARG, "mem must be CBS_MEM_HOST or CBS_MEM_DEVICE");
+    check_phase_dims(dims);
+    DevSel ds(device);
+    const int n1 = (int)dims[0], n2 = (int)dims[1], n3 = (int)dims[2];
+    const long long m = (long long)(n1 - 1) * (n2 - 1) * n3;
+    DevBuf<float> dres;
+    DevBuf<int32_t> b1, b2;
+    dres.ensure((size_t)m);
+    HIP_CHECK(hipMemcpy(dres.p, residues, m * sizeof(float),
+                        mem == CBS_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice));
+    int32_t* d1 = step1;
+    int32_t* d2 = step2;
+    if (mem == CBS_MEM_HOST) {
+      b1.ensure((size_t)n1 * n2 * n3);
+      b2.ensure((size_t)n2 * n3);
+      d1 = b1.p;
+      d2 = b2.p;
+    }
+    gs_step_maps(dres.p, d1, d2, n1, n2, n3);
+    if (mem == CBS_MEM_HOST) {
+      HIP_CHECK(hipMemcpy(step1, b1.p, (size_t)n1 * n2 * n3 * sizeof(int32_t), hipMemcpyDeviceToHost));
+      HIP_CHECK(hipMemcpy(step2, b2.p, (size_t)n2 * n3 * sizeof(int32_t), hipMemcpyDeviceToHost));
+    }
+    HIP_CHECK(hipDeviceSynchronize());
+  });
+}
+
+int cbs_phase_unwrap(int device, const float* wrapped, float* unwrapped, const uint32_t dims[3], int mem) {
+  return guarded([&] {
+    require(wrapped && unwrapped && dims, CBS_ERR_ARG, "null argument");
+    require(mem == CBS_MEM_HOST || mem == CBS_MEM_DEVICE, CBS_ERR_ARG, "mem must be CBS_MEM_HOST or CBS_MEM_DEVICE");
+    check_phase_dims(dims);
+    DevSel ds(device);
+    const int n1 = (int)dims[0], n2 = (int)dims[1], n3 = (int)dims[2];
+    const long long n = (long long)n1 * n2 * n3;
+    const long long m = (long long)(n1 - 1) * (n2 - 1) * n3;
+    DevBuf<float> bp, dres;
+    DevBuf<int32_t> d1, d2;
+    float* dp = unwrapped;
+    if (mem == CBS_MEM_HOST) {
+      bp.ensure((size_t)n);
+      dp = bp.p;
+      HIP_CHECK(hipMemcpy(dp, wrapped, n * sizeof(float), hipMemcpyHostToDevice));
+    } else if (unwrapped != wrapped) {
+      HIP_CHECK(hipMemcpy(dp, wrapped, n * sizeof(float), hipMemcpyDeviceToDevice));
+    }
+    dres.ensure((size_t)m);
+    d1.ensure((size_t)n);
+    d2.ensure((size_t)n2 * n3);
+    const unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>((m + 255) / 256, 1 << 20));
+    hipLaunchKernelGGL(phase_residues, dim3(grid), dim3(256), 0, 0, dp, dres.p, n1, n2, n3);
+    HIP_CHECK(hipGetLastError());
+    gs_step_maps(dres.p, d1.p, d2.p, n1, n2, n3);
+    hipLaunchKernelGGL(phase_integrate_column, dim3((n3 + 63) / 64), dim3(64), 0, 0, dp, d2.p, n1, n2, n3);
+    hipLaunchKernelGGL(phase_integrate_rows, dim3((unsigned)(((long long)n2 * n3 + 127) / 128)), dim3(128), 0, 0, dp,
+                       d1.p, n1, n2, n3);
+    HIP_CHECK(hipGetLastError());
+    if (mem == CBS_MEM_HOST) HIP_CHECK(hipMemcpy(unwrapped, dp, n * sizeof(float), hipMemcpyDeviceToHost));
     HIP_CHECK(hipDeviceSynchronize());
   });
 }

@@ -10,4 +10,14 @@ __global__ void phase_residues(const float* __restrict__ p, float* __restrict__ 
 __global__ void phase_integrate_column(float* __restrict__ p, const int32_t* __restrict__ step2, int n1, int n2, int n3);
 __global__ void phase_integrate_rows(float* __restrict__ p, const int32_t* __restrict__ step1, int n1, int n2, int n3);
 
+// Goldstein branch cuts (see goldstein.hpp): dipole removal in one direction, the nearest
+// lists of the listed residues, and rastering of the branches into the step maps.
+__global__ void phase_dipoles(float* __restrict__ res, int32_t* __restrict__ pos, int32_t* __restrict__ step1,
+                              int npos, int m1, int m2, int n3, int direction);
+__global__ void phase_nearest(const int32_t* __restrict__ rows, int n, const int32_t* __restrict__ table,
+                              int table_size, const int32_t* __restrict__ zstart, const int32_t* __restrict__ square,
+                              int s1, int s2, int32_t* __restrict__ out);
+__global__ void phase_raster(const int32_t* __restrict__ br, int nb, int32_t* __restrict__ step1,
+                             int32_t* __restrict__ step2, int n1, int n2, int n3);
+
 }  // namespace cbs

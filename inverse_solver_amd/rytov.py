@@ -4,19 +4,12 @@ phase unwrapping on the GPU.
 
 Phase unwrapping (Codes/sub_code/goldstein_GPU/unwrapp2_gpu.m) is split as the
 reference splits it:
-- the residue map runs on the device (cbs_phase_residues);
-- branch cuts are placed on the host, as get_branches_goldstein_v2 runs on the CPU
-  there;
-- the raster integration with 2 pi step maps runs on the device
-  (cbs_phase_integrate).
-
-A residue-free map needs no cuts (zero steps), and its unwrap is path independent.
-With residues, the cuts come from the classical Goldstein box search (connect each
-residue to its neighbours inside a growing box until the charge balances, or to the
-border), and a flood fill from pixel (0, 0) that does not cross a cut fixes the step
-maps.  This placement is this module's own, not the reference's sort/lookup search,
-so on maps with residues the unwrapped phase may differ from the reference's by 2 pi
-multiples across cuts.
+- the residue map, the dipole removal, the nearest-residue lists and the rastering of
+  branches into 2 pi step maps run on the device;
+- the branch search (which residues a cut joins) runs on the host, as
+  get_branches_goldstein_v2 runs on the CPU there, with the reference's ordering and
+  search rules, so the step maps are the reference's;
+- the raster integration with the step maps runs on the device (cbs_phase_integrate).
 
 The k-space mapping follows the MATLAB code, including two behaviours:
 - the illumination peak is measured relative to the first angle;
@@ -25,8 +18,6 @@ The k-space mapping follows the MATLAB code, including two behaviours:
 from __future__ import annotations
 
 import ctypes as C
-from collections import deque
-
 import numpy as np
 
 from ._lib import check, lib, CBS_MEM_HOST
@@ -55,127 +46,34 @@ def phase_residues(wrapped, device=0):
     return out
 
 
-def _wrap(d):
-    return (d + np.pi) % (2 * np.pi) - np.pi
-
-
-def _goldstein_cut_mask(res, max_box=None):
-    """Classical Goldstein branch cuts on one map: a pixel mask (n1, n2) of cut pixels.
-    res: (n1 - 1, n2 - 1) residues; the residue of loop (i, j) is drawn at pixel (i, j)."""
-    m1, m2 = res.shape
-    n1, n2 = m1 + 1, m2 + 1
-    cut = np.zeros((n1, n2), bool)
-    balanced = np.zeros((m1, m2), bool)
-    pos = list(zip(*np.nonzero(res)))
-    max_box = max_box or 2 * max(n1, n2) + 1
-
-    def line(a, b):
-        (i0, j0), (i1, j1) = a, b
-        steps = max(abs(i1 - i0), abs(j1 - j0), 1)
-        for t in range(steps + 1):
-            cut[int(round(i0 + (i1 - i0) * t / steps)), int(round(j0 + (j1 - j0) * t / steps))] = True
-
-    def to_border(a):
-        i, j = a
-        d = [(i, (0, j)), (n1 - 1 - i, (n1 - 1, j)), (j, (i, 0)), (n2 - 1 - j, (i, n2 - 1))]
-        line(a, min(d)[1])
-
-    for r0 in pos:
-        if balanced[r0]:
-            continue
-        balanced[r0] = True
-        charge = int(res[r0])
-        active = [r0]
-        box = 3
-        while charge != 0 and box <= max_box:
-            h = box // 2
-            k = 0
-            while k < len(active) and charge != 0:
-                ai, aj = active[k]
-                k += 1
-                if ai - h < 0 or aj - h < 0 or ai + h > m1 - 1 or aj + h > m2 - 1:
-                    to_border((ai, aj))
-                    charge = 0
-                    break
-                for i in range(ai - h, ai + h + 1):
-                    for j in range(aj - h, aj + h + 1):
-                        if res[i, j] == 0 or (i, j) in active:
-                            continue
-                        line((ai, aj), (i, j))
-                        active.append((i, j))
-                        if not balanced[i, j]:
-                            balanced[i, j] = True
-                            charge += int(res[i, j])
-                            if charge == 0:
-                                break
-                    if charge == 0:
-                        break
-            box += 2
-        if charge != 0:
-            to_border(r0)
-    return cut
-
-
-def _steps_from_flood_fill(w, cut):
-    """Unwrap one (n1, n2) map by a flood fill from (0, 0) that does not cross cut pixels
-    (cut pixels are filled last from any unwrapped neighbour); return the integer steps
-    that make the raster integration reproduce it: step1 (n1, n2), step2 (n2,)."""
-    n1, n2 = w.shape
-    u = np.full((n1, n2), np.nan)
-    u[0, 0] = w[0, 0]
-    q = deque([(0, 0)])
-    seen = np.zeros((n1, n2), bool)
-    seen[0, 0] = True
-    nb = ((1, 0), (-1, 0), (0, 1), (0, -1))
-    while q:
-        i, j = q.popleft()
-        if cut[i, j] and (i, j) != (0, 0):
-            continue
-        for di, dj in nb:
-            a, b = i + di, j + dj
-            if 0 <= a < n1 and 0 <= b < n2 and not seen[a, b]:
-                seen[a, b] = True
-                u[a, b] = u[i, j] + _wrap(w[a, b] - w[i, j])
-                q.append((a, b))
-    while np.isnan(u).any():                       # regions enclosed by cuts
-        todo = np.argwhere(np.isnan(u))
-        progressed = False
-        for i, j in todo:
-            for di, dj in nb:
-                a, b = i + di, j + dj
-                if 0 <= a < n1 and 0 <= b < n2 and not np.isnan(u[a, b]):
-                    u[i, j] = u[a, b] + _wrap(w[i, j] - w[a, b])
-                    progressed = True
-                    break
-        if not progressed:
-            break
-    step1 = np.zeros((n1, n2), np.int32)
-    step2 = np.zeros(n2, np.int32)
-    step2[1:] = np.round((u[0, :-1] + _wrap(w[0, 1:] - w[0, :-1]) - u[0, 1:]) / (2 * np.pi)).astype(np.int32)
-    step1[1:, :] = np.round((u[:-1, :] + _wrap(w[1:, :] - w[:-1, :]) - u[1:, :]) / (2 * np.pi)).astype(np.int32)
+def phase_branch_cuts(residues, device=0):
+    """Goldstein step maps from a residue map (n1 - 1, n2 - 1, n3): dipole removal and the
+    reference's branch search (goldstein_branch_cut.m:1-87).  Returns (step1 (n1, n2, n3),
+    step2 (n2, n3)) int32, column-major."""
+    r = np.asfortranarray(np.asarray(residues, np.float32))
+    if r.ndim == 2:
+        r = r[:, :, None]
+    m1, m2, n3 = r.shape
+    step1 = np.empty((m1 + 1, m2 + 1, n3), np.int32, order='F')
+    step2 = np.empty((m2 + 1, n3), np.int32, order='F')
+    dims = (C.c_uint32 * 3)(m1 + 1, m2 + 1, n3)
+    check(lib.cbs_phase_branch_cuts(int(device), r.ctypes.data_as(C.c_void_p), step1.ctypes.data_as(C.c_void_p),
+                                    step2.ctypes.data_as(C.c_void_p), dims, CBS_MEM_HOST))
     return step1, step2
 
 
 def unwrap_phase(wrapped, device=0):
-    """unwrapp2_gpu.m:1-5: (n1, n2[, n3]) wrapped phases -> unwrapped phases (float32)."""
+    """unwrapp2_gpu.m:1-5: (n1, n2[, n3]) wrapped phases -> unwrapped phases (float32):
+    residues, Goldstein branch cuts and the raster integration (cbs_phase_unwrap)."""
     w = np.asfortranarray(np.asarray(wrapped, np.float32))
     squeeze = w.ndim == 2
     if squeeze:
         w = w[:, :, None]
     n1, n2, n3 = w.shape
-    step1 = np.zeros((n1, n2, n3), np.int32, order='F')
-    step2 = np.zeros((n2, n3), np.int32, order='F')
-    res = phase_residues(w, device)
-    for z in np.nonzero(np.any(res != 0, axis=(0, 1)))[0]:
-        cut = _goldstein_cut_mask(res[:, :, z])
-        s1, s2 = _steps_from_flood_fill(w[:, :, z].astype(np.float64), cut)
-        step1[:, :, z] = s1
-        step2[:, z] = s2
     out = np.empty_like(w, order='F')
     dims = (C.c_uint32 * 3)(n1, n2, n3)
-    check(lib.cbs_phase_integrate(int(device), w.ctypes.data_as(C.c_void_p), step1.ctypes.data_as(C.c_void_p),
-                                  step2.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), dims,
-                                  CBS_MEM_HOST))
+    check(lib.cbs_phase_unwrap(int(device), w.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), dims,
+                               CBS_MEM_HOST))
     return out[:, :, 0] if squeeze else out
 
 

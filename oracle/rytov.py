@@ -5,9 +5,9 @@ Test infrastructure only (see oracle/__init__.py).
 Follows Codes/BACKWARD_SOLVER_RYTOV.m:22-127, Codes/sub_code/vector2scalarfield.m:1-14 and
 the residue / raster-unwrap kernels of Codes/sub_code/goldstein_GPU
 (get_residue_cuda.cu:35-66, unwrapp_phase_goldstein_cuda.cu:60-113).  The branch-cut
-placement of goldstein_branch_cut.m (get_shortest_sort, get_branches_goldstein_v2,
-raster_branches) is NOT restated: ``unwrap_residue_free`` refuses phase maps that hold
-residues, so the oracle pins the unwrap only where every path gives the same answer.
+placement (goldstein_branch_cut.m) is restated in oracle/goldstein.py, which
+``rytov_solve`` uses by default; ``unwrap_residue_free`` is the closed case with no
+residues (zero step maps).
 """
 from __future__ import annotations
 
@@ -55,10 +55,19 @@ def residues(wrapped):
     return np.round((d1_1 + d2_2 - d1_2 - d2_1) / tp).astype(np.float32)
 
 
+def _fma_step(x, step, tp):
+    """float32(x - step * tp) with one rounding: fmaf(-step, tp, x)."""
+    return np.float32(np.float64(np.float32(x)) - np.float64(step) * np.float64(tp))
+
+
 def raster_unwrap(wrapped, step1, step2):
     """unwrapp_phase_goldstein_cuda.cu:60-113: integrate the wrapped differences down the
     first column (dim 2, minus 2 pi step2) and then along every row (dim 1, minus 2 pi
-    step1), in single precision.  wrapped: (n1, n2, n3); step1: (n1, n2, n3); step2: (n2, n3)."""
+    step1), in single precision.  wrapped: (n1, n2, n3); step1: (n1, n2, n3); step2: (n2, n3).
+    The update last + d - step * 2 pi is one fused multiply-add, as nvcc contracts the
+    reference's expression by default: the product is not rounded on its own (formed in
+    float64, where it and the difference are exact for phases below 2^20 rad, then rounded
+    once to float32)."""
     p = np.array(wrapped, np.float32, copy=True)
     n1, n2, n3 = p.shape
     tp = np.float32(2.0) * PI32
@@ -67,7 +76,7 @@ def raster_unwrap(wrapped, step1, step2):
         for i in range(1, n2):
             cur = p[0, i, z]
             d = _mod32(cur - last + PI32, tp) - PI32
-            cur = np.float32(last + d - np.float32(step2[i, z]) * tp)
+            cur = _fma_step(last + d, step2[i, z], tp)
             p[0, i, z] = cur
             last = cur
     for z in range(n3):                                                      # unwrapp_kernel_2
@@ -76,7 +85,7 @@ def raster_unwrap(wrapped, step1, step2):
             for i in range(1, n1):
                 cur = p[i, j, z]
                 d = _mod32(cur - last + PI32, tp) - PI32
-                cur = np.float32(last + d - np.float32(step1[i, j, z]) * tp)
+                cur = _fma_step(last + d, step1[i, j, z], tp)
                 p[i, j, z] = cur
                 last = cur
     return p
@@ -91,7 +100,7 @@ def unwrap_residue_free(wrapped):
     return raster_unwrap(w, np.zeros((n1, n2, n3), np.int32), np.zeros((n2, n3), np.int32))
 
 
-def rytov_solve(params, input_field, output_field, unwrap=unwrap_residue_free):
+def rytov_solve(params, input_field, output_field, unwrap=None):
     """Codes/BACKWARD_SOLVER_RYTOV.m:22-127 (FP64 except the FP32 unwrap).
 
     input_field, output_field: (x, y, pol, A).  Returns (RI, ORytov)."""
@@ -111,6 +120,8 @@ def rytov_solve(params, input_field, output_field, unwrap=unwrap_residue_free):
         raise ValueError('x/y output size must be isotropic')
     if p['size'][0] != inp.shape[0] or p['size'][1] != inp.shape[1]:
         raise ValueError('declare size in the parameter must be the same as the field size')
+    if unwrap is None:
+        from .goldstein import unwrap
     bg, sp = vector2scalarfield(inp, out)                                     # :47
     ratio = sp / bg
     ret_phase = unwrap(np.angle(ratio).astype(np.float32)).astype(np.float64)   # :48-49

@@ -4,10 +4,12 @@
   arithmetic) on random maps (many residues), vortices and smooth maps, odd sizes.
 * cbs_phase_integrate equals the oracle's raster unwrap exactly, with zero and with
   random step maps.
-* unwrap_phase on maps with residues (the branch-cut placement is this library's own):
-  the result is congruent to the input modulo 2 pi, and every neighbour difference
-  between pixels that are not on a cut is the wrapped difference.
-* BACKWARD_SOLVER_RYTOV equals the oracle's restatement (residue-free inputs) to 1e-6.
+* cbs_phase_branch_cuts equals the oracle's Goldstein step maps exactly (dipole removal,
+  nearest lists, branch search, raster) on random maps (thousands of residues), a noisy
+  vortex field, and a map with a vortex pair, a dipole and vortices cut to both borders;
+  unwrap_phase (cbs_phase_unwrap) equals the oracle's unwrap exactly on the same maps.
+* BACKWARD_SOLVER_RYTOV equals the oracle's restatement to 1e-6, on measured fields and on
+  fields with phase noise that leaves residues.
 """
 import numpy as np
 import pytest
@@ -56,23 +58,62 @@ def test_gpu_raster_integration_matches_oracle_exactly():
             np.testing.assert_array_equal(out, R.raster_unwrap(m, s1, s2))
 
 
-def test_gpu_unwrap_with_residues_is_consistent():
-    from inverse_solver_amd.rytov import unwrap_phase, _goldstein_cut_mask
+def _residue_maps():
+    rng = np.random.default_rng(7)
+    out = [rng.uniform(-np.pi, np.pi, s).astype(np.float32) for s in ((40, 40, 2), (33, 29, 1), (9, 9, 2), (128, 128, 3))]
+    x, y = np.meshgrid(np.arange(100), np.arange(100), indexing='ij')
+    p = x + 1j * y
+    c = rng.uniform(5, 95, (30, 2))
+    phi = 0.005 * x * y + sum(np.angle(p - complex(*cc)) * s for cc, s in zip(c, rng.choice([-1, 1], 30)))
+    out.append(np.angle(np.exp(1j * (phi + 0.6 * rng.standard_normal(phi.shape))))[:, :, None].astype(np.float32))
+    # a far vortex pair, a dipole, and a vortex cut to each border
+    phi = np.angle((p - complex(19.5, 16.5)) / (p - complex(19.5, 22.5))) + np.angle(p - complex(3.5, 60.5)) \
+        - np.angle(-1j * (p - complex(60.5, 3.5)))
+    phi[50, 50], phi[51, 50] = -1.75, 1.75
+    out.append(np.angle(np.exp(1j * phi))[:, :, None].astype(np.float32))
+    return out
+
+
+def test_gpu_branch_cuts_match_oracle_exactly():
+    from oracle import goldstein as G
+    from inverse_solver_amd.rytov import phase_branch_cuts
+    for m in _residue_maps() + _maps():
+        s1_ref, s2_ref, info = G.step_maps(m)
+        s1, s2 = phase_branch_cuts(R.residues(m))
+        print('branch cuts', m.shape, 'residues', int(np.abs(info['residues']).sum()),
+              'branches', info['branches'].shape[1], flush=True)
+        np.testing.assert_array_equal(s1, s1_ref)
+        np.testing.assert_array_equal(s2, s2_ref)
+
+
+def test_gpu_unwrap_matches_oracle_exactly():
+    from oracle import goldstein as G
+    from inverse_solver_amd.rytov import unwrap_phase
+    for m in _residue_maps() + _maps():
+        u = unwrap_phase(m)
+        np.testing.assert_array_equal(u, G.unwrap(m))
+        k = (u - m) / (2 * np.pi)
+        np.testing.assert_allclose(k, np.round(k), atol=1e-4)              # congruent modulo 2 pi
     m = _maps()[1]
-    u = unwrap_phase(m)
-    k = (u - m) / (2 * np.pi)
-    np.testing.assert_allclose(k, np.round(k), atol=1e-4)              # congruent modulo 2 pi
-    res = R.residues(m)
-    for z in range(m.shape[2]):
-        cut = _goldstein_cut_mask(res[:, :, z])
-        free = ~cut
-        for ax in (0, 1):
-            du = np.diff(u[:, :, z], axis=ax)
-            dw = np.angle(np.exp(1j * np.diff(m[:, :, z].astype(np.float64), axis=ax)))
-            both = free[1:, :] & free[:-1, :] if ax == 0 else free[:, 1:] & free[:, :-1]
-            assert np.max(np.abs((du - dw)[both])) < 1e-3, (z, ax)
-    # a residue-free map unwraps exactly as the oracle's raster integration
+    # a residue-free map unwraps exactly as the oracle's raster integration; 2D input
     np.testing.assert_array_equal(unwrap_phase(m[:, :, 2:3]), R.unwrap_residue_free(m[:, :, 2:3]))
+    np.testing.assert_array_equal(unwrap_phase(m[:, :, 0]), G.unwrap(m[:, :, :1])[:, :, 0])
+    # no residues at all: a constant map picks up the single-precision wrap of a zero
+    # difference (fmod(pi + 2 pi, 2 pi) - pi = -2.4e-7 per step), as the reference's does
+    z = np.zeros((6, 5, 2), np.float32)
+    np.testing.assert_array_equal(unwrap_phase(z), G.unwrap(z))
+    assert np.max(np.abs(unwrap_phase(z))) < 1e-5
+
+
+def test_gpu_branch_cuts_reject_bad_sizes():
+    import ctypes as C
+    from inverse_solver_amd._lib import lib, CBS_MEM_HOST
+    r = np.zeros((1, 4, 1), np.float32, order='F')
+    s1 = np.zeros((2, 5, 1), np.int32, order='F')
+    s2 = np.zeros((5, 1), np.int32, order='F')
+    dims = (C.c_uint32 * 3)(1, 5, 1)
+    assert lib.cbs_phase_branch_cuts(0, r.ctypes.data_as(C.c_void_p), s1.ctypes.data_as(C.c_void_p),
+                                     s2.ctypes.data_as(C.c_void_p), dims, CBS_MEM_HOST) != 0
 
 
 def _measured(ri, n_s, radius, A, iters=20):
@@ -97,6 +138,24 @@ def test_gpu_rytov_matches_oracle(non_neg):
     RI, OR = BACKWARD_SOLVER_RYTOV(rp).solve(u, t)
     RI_ref, OR_ref = R.rytov_solve(rp, u, t)
     print('Rytov vs oracle: RI contrast %.2e  ORytov %.2e' % (rel(RI - 1.336, RI_ref - 1.336), rel(OR, OR_ref)))
+    assert rel(OR, OR_ref) < 1e-6
+    assert rel(RI - 1.336, RI_ref - 1.336) < 1e-6
+
+
+def test_gpu_rytov_with_residues_matches_oracle():
+    from oracle import goldstein as G
+    from inverse_solver_amd.rytov import BACKWARD_SOLVER_RYTOV, get_default_parameters
+    ri = (24, 24, 24)
+    p, u, t = _measured(ri, 1.35, 0.6, 5)
+    t = t * np.exp(1j * 1.4 * np.random.default_rng(11).standard_normal(t.shape))      # phase noise
+    n_res = int(np.abs(R.residues(np.angle(t[:, :, 0] / u[:, :, 0]).astype(np.float32))).sum())
+    assert n_res > 20
+    rp = get_default_parameters(dict(size=list(p['size']), wavelength=0.532, NA=1.2, RI_bg=1.336,
+                                     resolution=[0.1] * 3))
+    RI, OR = BACKWARD_SOLVER_RYTOV(rp).solve(u, t)
+    RI_ref, OR_ref = R.rytov_solve(rp, u, t, unwrap=G.unwrap)
+    print('Rytov with %d residues vs oracle: RI contrast %.2e  ORytov %.2e'
+          % (n_res, rel(RI - 1.336, RI_ref - 1.336), rel(OR, OR_ref)))
     assert rel(OR, OR_ref) < 1e-6
     assert rel(RI - 1.336, RI_ref - 1.336) < 1e-6
 
