@@ -368,6 +368,208 @@ __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
   }
 }
 
+// ------------------------------------------------------------------ pass_kspace, vector runs, one component at a time
+// The same work as pass_kspace<B, NZ, T, NT, 3>, with one component's T x B x NZ tile in LDS
+// at a time: each component is loaded, b-transformed and taken through the forward z
+// steps A and B; its step-B output stays in registers, so after the third component every
+// thread holds the three components of its frequencies and applies the dyadic factor
+// there.  The inverse steps then run component by component and store.  The LDS is a third
+// of the all-component tile, so an x tile of 16 (128-byte rows) fits two blocks per CU.
+// Every value goes through the same operations in the same order as in pass_kspace.
+template <int B, int NZ, int T, int NT>
+__global__ __launch_bounds__(NT) void pass_kspace_seq3(PassKArgs args) {
+  using LF = LineFft<NZ>;
+  constexpr int R1 = LF::R1, R2 = LF::R2;
+  static_assert(R2 > 1, "two-step z transform");
+  static_assert(T % 2 == 0, "x tile must hold pairs");
+  constexpr int LW = B * T;                   // z lines of one component
+  constexpr int TP = T / 2;
+  constexpr int NTASK = NZ * TP;
+  constexpr int L = (NTASK + NT - 1) / NT;
+  static_assert((LW * R1) % NT == 0, "step-B groups split evenly over the block");
+  constexpr int G = LW * R1 / NT;             // step-B groups per thread
+  __shared__ float2 lds[NZ * LW];
+  const Consts& c = args.c;
+  const Tables& tb = args.tb;
+  const int tid = threadIdx.x;
+  const int bid = blockIdx.x;
+  int r, p1, xt;
+  if (args.order == 0) {
+    r = bid % c.nrhs;
+    const int t2 = bid / c.nrhs;
+    p1 = t2 % c.a;
+    xt = t2 / c.a;
+  } else {
+    const int ntile = (c.nx + T - 1) / T;
+    xt = bid % ntile;
+    const int t2 = bid / ntile;
+    p1 = t2 % c.a;
+    r = t2 / c.a;
+  }
+  const int x0 = xt * T;
+  const long long rowstride = (long long)c.a * c.nx;
+  float2* __restrict__ spec = args.spec + (long long)(3 * r) * args.rhs_stride + (long long)p1 * c.nx + x0;
+  const int flip = flip_of(args.jj, args.parity, 3 * r);
+  const int tvalid = c.nx - x0 < T ? c.nx - x0 : T;
+  float2 wcr[B];
+#pragma unroll
+  for (int y2 = 0; y2 < B; ++y2) wcr[y2] = y2 ? tb.cross[y2 * c.a + p1] : make_float2(1.f, 0.f);
+  auto zl = [](int l, int e) { return e * LW + l; };
+  const float2* __restrict__ tw = tb.tw_z;
+  float2 acc[G][3][R2];
+
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    float4 reg[L][B];
+    const float2* srcq = spec + (long long)q * args.rhs_stride;
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+      const int task = tid + k * NT;
+      if (task < NTASK) {
+        const int tp = task % TP, z = task / TP;
+        const float2* src = srcq + (long long)z * B * rowstride + 2 * tp;
+#pragma unroll
+        for (int y2 = 0; y2 < B; ++y2)
+          reg[k][y2] = (2 * tp < tvalid) ? *reinterpret_cast<const float4*>(src + y2 * rowstride)
+                                         : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+      const int task = tid + k * NT;
+      if (task < NTASK) {
+        const int tp = task % TP, z = task / TP;
+        float2 u0[B], u1[B];
+#pragma unroll
+        for (int y2 = 0; y2 < B; ++y2) {
+          u0[y2] = lo2(reg[k][y2]);
+          u1[y2] = hi2(reg[k][y2]);
+          if (y2) {
+            u0[y2] = cmul(u0[y2], wcr[y2]);
+            u1[y2] = cmul(u1[y2], wcr[y2]);
+          }
+        }
+        Dft<B, false>::run(u0);
+        Dft<B, false>::run(u1);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) {
+          lds[z * LW + k2 * T + 2 * tp] = u0[k2];
+          lds[z * LW + k2 * T + 2 * tp + 1] = u1[k2];
+        }
+      }
+    }
+    __syncthreads();
+    for (int t = tid; t < LW * R2; t += NT) {            // forward step A
+      const int l = t % LW, n2 = group_of<LW, NT>(t);
+      float2 v[R1];
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) v[n1] = lds[zl(l, R2 * n1 + n2)];
+      Dft<R1, false>::run(v);
+#pragma unroll
+      for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], tw[n2 * k1]);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) lds[zl(l, R2 * k1 + n2)] = v[k1];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < G; ++g) {                         // forward step B, kept in registers
+      const int t = tid + g * NT;
+      const int l = t % LW, k1 = group_of<LW, NT>(t);
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) acc[g][q][n2] = lds[zl(l, R2 * k1 + n2)];
+      Dft<R2, false>::run(acc[g][q]);
+    }
+    __syncthreads();
+  }
+  {
+    // G (v - c r (r . v)) per frequency, as in pass_kspace
+    const float* gz = tb.green_z + flip * c.nz;
+    const float* rz = tb.rad_z + flip * c.nz;
+    const float* gx = tb.green_x + flip * c.nx + x0;
+    const float* gy = tb.green_y + flip * c.ny + p1;
+    const float* rx = tb.rad_x + flip * c.nx + x0;
+    const float* ry = tb.rad_y + flip * c.ny + p1;
+    const float km2 = c.km2, eps2 = c.eps * c.eps, inv_n = c.inv_n, eps = c.eps;
+    const float2 cc = make_float2(c.c_re, c.c_im);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int t = tid + g * NT;
+      const int l = t % LW, k1 = group_of<LW, NT>(t);
+      const int k2y = l / T, tt = l - k2y * T;
+      const int tx = tt < tvalid ? tt : 0;
+      const float r0 = rx[tx], r1 = ry[k2y * c.a];
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) {
+        const int pz = R2 * k1 + k2;
+        float2& v0 = acc[g][0][k2];
+        float2& v1 = acc[g][1][k2];
+        float2& v2 = acc[g][2][k2];
+        const float d = gx[tx] + gy[k2y * c.a] + gz[pz] - km2;
+        const float s = inv_n * __builtin_amdgcn_rcpf(fmaf(d, d, eps2));   // hardware reciprocal (1 ulp)
+        const float2 gg = make_float2(d * s, eps * s);
+        const float r2 = rz[pz];
+        const float2 dot = make_float2(r0 * v0.x + r1 * v1.x + r2 * v2.x, r0 * v0.y + r1 * v1.y + r2 * v2.y);
+        const float2 cd = cmul(cc, dot);
+        v0 = cmul(gg, make_float2(v0.x - r0 * cd.x, v0.y - r0 * cd.y));
+        v1 = cmul(gg, make_float2(v1.x - r1 * cd.x, v1.y - r1 * cd.y));
+        v2 = cmul(gg, make_float2(v2.x - r2 * cd.x, v2.y - r2 * cd.y));
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+#pragma unroll
+    for (int g = 0; g < G; ++g) {                         // inverse step B
+      const int t = tid + g * NT;
+      const int l = t % LW, k1 = group_of<LW, NT>(t);
+      Dft<R2, true>::run(acc[g][q]);
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) lds[zl(l, R2 * k1 + n2)] = acc[g][q][n2];
+    }
+    __syncthreads();
+    for (int t = tid; t < LW * R2; t += NT) {            // inverse step A
+      const int l = t % LW, n2 = group_of<LW, NT>(t);
+      float2 v[R1];
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) v[k1] = lds[zl(l, R2 * k1 + n2)];
+#pragma unroll
+      for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmulc(v[k1], tw[n2 * k1]);
+      Dft<R1, true>::run(v);
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) lds[zl(l, R2 * n1 + n2)] = v[n1];
+    }
+    __syncthreads();
+    float2* dstq = spec + (long long)q * args.rhs_stride;
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+      const int task = tid + k * NT;
+      if (task < NTASK) {
+        const int tp = task % TP, z = task / TP;
+        if (2 * tp < tvalid) {
+          float2 u0[B], u1[B];
+#pragma unroll
+          for (int k2 = 0; k2 < B; ++k2) {
+            u0[k2] = lds[z * LW + k2 * T + 2 * tp];
+            u1[k2] = lds[z * LW + k2 * T + 2 * tp + 1];
+          }
+          Dft<B, true>::run(u0);
+          Dft<B, true>::run(u1);
+          float2* dst = dstq + (long long)z * B * rowstride + 2 * tp;
+#pragma unroll
+          for (int y2 = 0; y2 < B; ++y2) {
+            if (y2) {
+              u0[y2] = cmulc(u0[y2], wcr[y2]);
+              u1[y2] = cmulc(u1[y2], wcr[y2]);
+            }
+            store_stream(reinterpret_cast<float4*>(dst + y2 * rowstride), pack2(u0[y2], u1[y2]), args.nt);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------------ plain form
 // Unfused single-rhs iteration: per-axis FFT launches on a natural-layout
 // spectrum, a Green's multiply launch and a pointwise update launch.

@@ -44,6 +44,9 @@ constexpr int kNT = 256;
   X(2, 160, 8, 512) X(4, 160, 8, 512) X(4, 160, 8, 1024) X(2, 160, 16, 1024) \
   X(1, 320, 8, 256) X(1, 320, 16, 512) X(1, 256, 16, 512)
 
+// vector runs, one component at a time through LDS (pass_kspace_seq3): (B, NZ, T, NT)
+#define PKS_LIST(X) X(4, 160, 16, 1024) X(4, 160, 16, 512)
+
 #define RW_LIST(X) \
   X(16) X(18) X(20) X(24) X(25) X(27) X(30) X(32) X(36) X(40) X(48) X(64) X(96) X(128) X(135) X(160) X(192) \
   X(256) X(288) X(320) X(512) X(576) X(1024)
@@ -62,11 +65,13 @@ constexpr int pr_lds(int nx, int a) { return a * row_pitch(nx) * 8 + nx * 8 + a 
 #define PK_ENTRY(B, NZ, T, NT) {&pass_kspace<B, NZ, T, NT, 1>, B, NZ, T, NT, (int)(NZ * B * T * sizeof(float2)), 1},
 #define PKV_ENTRY(B, NZ, T, NT) \
   {&pass_kspace<B, NZ, T, NT, 3>, B, NZ, T, NT, (int)((3 * NZ * B * T + 2 * NZ) * sizeof(float2)), 3},
+#define PKS_ENTRY(B, NZ, T, NT) \
+  {&pass_kspace_seq3<B, NZ, T, NT>, B, NZ, T, NT, (int)(NZ * B * T * sizeof(float2)), 3},
 #define RW_ENTRY(N) {&fft_rows<N, rows_for(N), kNT>, N, rows_for(N), kNT},
 #define ST_ENTRY(N) {&fft_strided<N, strided_t(N), kNT>, N, strided_t(N), kNT},
 
 static const PassRInfo kPR[] = {PR_LIST(PR_ENTRY) PRR_LIST(PRR_ENTRY)};
-static const PassKInfo kPK[] = {PK_LIST(PK_ENTRY) PKV_LIST(PKV_ENTRY)};
+static const PassKInfo kPK[] = {PK_LIST(PK_ENTRY) PKV_LIST(PKV_ENTRY) PKS_LIST(PKS_ENTRY)};
 static const RowsInfo kRW[] = {RW_LIST(RW_ENTRY)};
 static const StridedInfo kST[] = {ST_LIST(ST_ENTRY)};
 

@@ -115,3 +115,24 @@ def test_vector_gradient_matches_oracle():
     assert rel(g, g_ref) < 1e-4
     assert abs(err - err_ref) < 1e-4 * err_ref
     b.close()
+
+
+def test_vector_component_sequential_kspace_equals_all_component_kspace():
+    # 160^3 simulation grid (128^3 RI + 16-px boundary), a = 40, b = 4: the k-space pass with
+    # all three components of an 8-wide x tile in LDS (pass_kspace<4, 160, 8, 1024, 3>) and
+    # the one taking the components through LDS in turn with a 16-wide tile
+    # (pass_kspace_seq3<4, 160, 16, 1024>) apply the same operations to every value
+    ri = (128, 128, 128)
+    p = params(ri, iterations=6, vector_simulation=True, boundary_thickness=[8, 8, 8])
+    RI = sphere_ri(ri, 0.5, n_s=1.40)
+    fields = np.stack([np.stack([random_field(ri, 70 + 3 * a + j) for j in range(3)], axis=-1) for a in range(2)],
+                      axis=-1)
+    out = {}
+    for t in (8, 16):
+        s = _gpu()(p, options=dict(form=0, split_a=40, tile_t=t, threads_kspace=1024))
+        s.set_RI(RI)
+        out[t] = s.solve_incident(fields)
+        print('tile', t, s.plan(), flush=True)
+        assert s.plan()['tile_t'] == t and s.plan()['a'] == 40 and s.plan()['b'] == 4
+        s.close()
+    np.testing.assert_array_equal(out[16], out[8])
