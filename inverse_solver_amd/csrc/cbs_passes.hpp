@@ -377,7 +377,8 @@ __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
 // of the all-component tile, so an x tile of 16 (128-byte rows) fits two blocks per CU.
 // Every value goes through the same operations in the same order as in pass_kspace.
 template <int B, int NZ, int T, int NT>
-__global__ __launch_bounds__(NT) void pass_kspace_seq3(PassKArgs args) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(B * T * LineFft<NZ>::R1 == NT ? 4 : 1)))
+void pass_kspace_seq3(PassKArgs args) {
   using LF = LineFft<NZ>;
   constexpr int R1 = LF::R1, R2 = LF::R2;
   static_assert(R2 > 1, "two-step z transform");

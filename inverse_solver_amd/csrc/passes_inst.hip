@@ -41,11 +41,11 @@ constexpr int kNT = 256;
   X(1, 96, 16, 256) X(2, 96, 16, 512) X(3, 96, 8, 256) \
   X(2, 128, 16, 512) X(4, 128, 8, 512) X(2, 128, 8, 256) \
   X(2, 135, 16, 512) X(2, 135, 8, 256) X(4, 135, 8, 512) \
-  X(2, 160, 8, 512) X(4, 160, 8, 512) X(4, 160, 8, 1024) X(2, 160, 16, 1024) \
+  X(2, 160, 8, 512) X(4, 160, 8, 1024) X(2, 160, 16, 1024) \
   X(1, 320, 8, 256) X(1, 320, 16, 512) X(1, 256, 16, 512)
 
 // vector runs, one component at a time through LDS (pass_kspace_seq3): (B, NZ, T, NT)
-#define PKS_LIST(X) X(4, 160, 16, 1024) X(4, 160, 16, 512)
+#define PKS_LIST(X) X(4, 160, 16, 1024) X(4, 160, 8, 512)
 
 #define RW_LIST(X) \
   X(16) X(18) X(20) X(24) X(25) X(27) X(30) X(32) X(36) X(40) X(48) X(64) X(96) X(128) X(135) X(160) X(192) \

@@ -127,12 +127,14 @@ def test_vector_component_sequential_kspace_equals_all_component_kspace():
     RI = sphere_ri(ri, 0.5, n_s=1.40)
     fields = np.stack([np.stack([random_field(ri, 70 + 3 * a + j) for j in range(3)], axis=-1) for a in range(2)],
                       axis=-1)
+    # (8, 1024): the all-component kernel; (16, 1024) and (8, 512): the sequential one
     out = {}
-    for t in (8, 16):
-        s = _gpu()(p, options=dict(form=0, split_a=40, tile_t=t, threads_kspace=1024))
+    for t, nt in ((8, 1024), (16, 1024), (8, 512)):
+        s = _gpu()(p, options=dict(form=0, split_a=40, tile_t=t, threads_kspace=nt))
         s.set_RI(RI)
-        out[t] = s.solve_incident(fields)
-        print('tile', t, s.plan(), flush=True)
+        out[t, nt] = s.solve_incident(fields)
+        print('tile', t, nt, s.plan(), flush=True)
         assert s.plan()['tile_t'] == t and s.plan()['a'] == 40 and s.plan()['b'] == 4
         s.close()
-    np.testing.assert_array_equal(out[16], out[8])
+    np.testing.assert_array_equal(out[16, 1024], out[8, 1024])
+    np.testing.assert_array_equal(out[8, 512], out[8, 1024])
