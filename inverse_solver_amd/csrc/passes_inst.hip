@@ -56,7 +56,9 @@ constexpr int kNT = 256;
   X(256) X(288) X(320) X(512) X(576) X(1024)
 
 constexpr int rows_for(int n) { return n <= 64 ? 16 : (n <= 256 ? 8 : (n <= 576 ? 4 : 2)); }
-constexpr int strided_t(int n) { return n <= 512 ? 16 : 8; }
+// 16-wide column tiles (128-byte rows) up to n = 576 (78 KiB with the twiddles: two blocks
+// per CU); at 576 this took the y transforms of the 576^2 x 320 grid from 12.1 to 9.8 ms
+constexpr int strided_t(int n) { return n <= 576 ? 16 : 8; }
 
 // LDS of pass_real: the tile plus its per-axis tables (x: twiddle; rows: twiddle, ramp, mask)
 constexpr int pr_lds(int nx, int a) { return a * row_pitch(nx) * 8 + nx * 8 + a * 20; }
