@@ -41,9 +41,14 @@ the checks listed below instead of by reference-produced vectors:
               raster-unwrap kernels
               pinned by a phase vortex (one residue, winding sign), smooth-map
               unwrapping to the true phase, step-map closed forms, a weak bead
-              reconstructed from CBS-oracle fields, and the no-scatterer case.
-              The branch-cut placement is not restated (maps with residues are
-              refused): unwrapping with residues is parity unpinned.
+              reconstructed from CBS-oracle fields, and the no-scatterer case;
+              the raster update's single rounding against exact rationals.
+  goldstein.py the goldstein_GPU branch-cut placement (dipoles, sorted residue
+              lookup, 8-nearest box search, the CPU branch search, rastering)
+              pinned by closed-form vortex pairs and border vortices (signs and
+              cut geometry), edge-jump dipoles, brute-force nearest lists where
+              the box search is exhaustive (and a hand case of its ring bound),
+              and charge balance of every branch network.
 
 The seeded input builders (phantoms, illumination) live in the product package
 (inverse_solver_amd/phantoms.py) and are pinned by closed forms in

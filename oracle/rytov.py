@@ -95,7 +95,7 @@ def unwrap_residue_free(wrapped):
     """unwrapp2_gpu.m:1-5 for phase maps without residues (zero step maps)."""
     w = np.asarray(wrapped, np.float32)
     if np.any(residues(w) != 0):
-        raise ValueError('phase map holds residues: the branch-cut placement is not restated in the oracle')
+        raise ValueError('phase map holds residues: use oracle.goldstein.unwrap')
     n1, n2, n3 = w.shape
     return raster_unwrap(w, np.zeros((n1, n2, n3), np.int32), np.zeros((n2, n3), np.int32))
 
