@@ -13,7 +13,7 @@ constexpr int kNT = 256;
 // (NX, A, NT): x length, rows of the y split, threads per block
 #define PR_LIST(X) \
   X(24, 24, 256) X(24, 12, 256) X(24, 6, 256) X(32, 32, 256) X(32, 16, 256) X(40, 40, 256) X(40, 20, 256) \
-  X(96, 96, 256) X(96, 96, 512) X(96, 48, 256) X(96, 48, 512) X(96, 32, 256) \
+  X(96, 96, 256) X(96, 96, 512) X(96, 48, 256) X(96, 48, 512) X(96, 32, 256) X(96, 24, 256) \
   X(128, 64, 256) X(128, 64, 512) X(128, 32, 256) \
   X(160, 40, 256) X(160, 40, 512) X(160, 32, 256) X(160, 32, 512) X(160, 20, 256) X(160, 80, 512) \
   X(192, 48, 256) X(192, 48, 512) X(192, 32, 256) \
@@ -27,6 +27,7 @@ constexpr int kNT = 256;
   X(1, 18, 8, 256) X(1, 24, 8, 256) X(1, 24, 16, 256) X(2, 24, 8, 256) X(4, 24, 8, 256) X(1, 20, 8, 256) \
   X(1, 32, 16, 256) X(2, 32, 16, 256) X(1, 40, 16, 256) X(2, 40, 16, 256) X(1, 30, 8, 256) X(1, 36, 8, 256) \
   X(1, 96, 32, 256) X(1, 96, 32, 512) X(1, 96, 16, 256) X(2, 96, 16, 256) X(2, 96, 32, 512) X(3, 96, 16, 256) \
+  X(2, 96, 8, 256) X(3, 96, 8, 256) X(4, 96, 8, 256) X(4, 96, 16, 256) \
   X(2, 135, 32, 256) X(2, 135, 32, 512) X(2, 135, 16, 256) X(4, 135, 8, 256) \
   X(2, 128, 32, 256) X(4, 128, 16, 256) X(6, 128, 8, 256) X(4, 128, 8, 256) \
   X(4, 160, 16, 256) X(4, 160, 16, 512) X(4, 160, 16, 1024) X(5, 160, 8, 256) X(8, 160, 8, 256) X(4, 160, 8, 256) \

@@ -54,6 +54,28 @@ def test_born_loop_matches_oracle(ri_size, iters, opts):
     s.close()
 
 
+@pytest.mark.parametrize('opts', [dict(form=0, split_a=24, tile_t=8), dict(form=0, split_a=24, tile_t=16),
+                                  dict(form=0, split_a=32, tile_t=8), dict(form=0, split_a=48, tile_t=8)])
+def test_small_grid_plans_match_oracle(opts):
+    # 64^3 RI + 16-px boundary = 96^3 simulation grid (the single-angle latency case):
+    # the small-tile plans that give both passes more blocks than CUs
+    ri = (64, 64, 64)
+    p = params(ri, iterations=8, boundary_thickness=[8, 8, 8])
+    RI = sphere_ri(ri, 1.5, n_s=1.40)
+    u = plane_wave(ri, 0.15, 0.05)
+    ref, o = _oracle_loop(p, RI, [u])
+    s = _gpu()(p, options=opts)
+    s.set_RI(RI)
+    assert s.simulation_size == [96, 96, 96]
+    got = s.solve_incident(u)
+    plan = s.plan()
+    err = rel(got, ref[..., 0])
+    print(opts, plan, 'rel err', err, flush=True)
+    assert plan['a'] == opts['split_a'] and plan['tile_t'] == opts['tile_t']
+    assert err < TOL
+    s.close()
+
+
 def test_batch_equals_individual_and_oracle():
     ri = (16, 16, 16)
     p = params(ri, iterations=16)
