@@ -7,7 +7,8 @@
 * cbs_phase_branch_cuts equals the oracle's Goldstein step maps exactly (dipole removal,
   nearest lists, branch search, raster) on random maps (thousands of residues), a noisy
   vortex field, and a map with a vortex pair, a dipole and vortices cut to both borders;
-  unwrap_phase (cbs_phase_unwrap) equals the oracle's unwrap exactly on the same maps.
+  unwrap_phase (cbs_phase_unwrap) equals the oracle's unwrap exactly on the same maps;
+  both entry points give the same on device buffers (the unwrap also in place).
 * BACKWARD_SOLVER_RYTOV equals the oracle's restatement to 1e-6, on measured fields and on
   fields with phase noise that leaves residues.
 """
@@ -103,6 +104,37 @@ def test_gpu_unwrap_matches_oracle_exactly():
     z = np.zeros((6, 5, 2), np.float32)
     np.testing.assert_array_equal(unwrap_phase(z), G.unwrap(z))
     assert np.max(np.abs(unwrap_phase(z))) < 1e-5
+
+
+def test_gpu_unwrap_and_cuts_on_device_buffers():
+    # CBS_MEM_DEVICE: column-major device arrays (a row-major (n3, n2, n1) buffer), in place
+    # for the unwrap, and the residue map left unchanged by the branch cuts
+    import ctypes as C
+    from oracle import goldstein as G
+    from inverse_solver_amd._lib import check, lib, CBS_MEM_DEVICE
+    from inverse_solver_amd.device import DeviceArray
+    m = _residue_maps()[0]
+    n1, n2, n3 = m.shape
+    dims = (C.c_uint32 * 3)(n1, n2, n3)
+    ref = G.unwrap(m)
+    dw = DeviceArray.from_host(np.ascontiguousarray(m.T))
+    dout = DeviceArray((n3, n2, n1), np.float32)
+    check(lib.cbs_phase_unwrap(0, C.c_void_p(dw.ptr), C.c_void_p(dout.ptr), dims, CBS_MEM_DEVICE))
+    np.testing.assert_array_equal(dout.to_host().T, ref)
+    check(lib.cbs_phase_unwrap(0, C.c_void_p(dw.ptr), C.c_void_p(dw.ptr), dims, CBS_MEM_DEVICE))
+    np.testing.assert_array_equal(dw.to_host().T, ref)
+    res = R.residues(m)
+    dres = DeviceArray.from_host(np.ascontiguousarray(res.T))
+    d1 = DeviceArray((n3, n2, n1), np.int32)
+    d2 = DeviceArray((n3, n2), np.int32)
+    check(lib.cbs_phase_branch_cuts(0, C.c_void_p(dres.ptr), C.c_void_p(d1.ptr), C.c_void_p(d2.ptr), dims,
+                                    CBS_MEM_DEVICE))
+    s1, s2, _ = G.step_maps(m)
+    np.testing.assert_array_equal(d1.to_host().T, s1)
+    np.testing.assert_array_equal(d2.to_host().T, s2)
+    np.testing.assert_array_equal(dres.to_host().T, res)
+    for d in (dw, dout, dres, d1, d2):
+        d.free()
 
 
 def test_gpu_branch_cuts_reject_bad_sizes():
