@@ -76,6 +76,23 @@ def test_small_grid_plans_match_oracle(opts):
     s.close()
 
 
+def test_block_orders_give_identical_results():
+    # every block order (rhs or row numbering of each pass) computes the same values
+    ri = (16, 16, 16)
+    p = params(ri, iterations=10)
+    RI = sphere_ri(ri, 0.6, n_s=1.40)
+    fields = np.stack([random_field(ri, 90 + j) for j in range(8)], -1)
+    outs = []
+    for order in range(4):
+        s = _gpu()(p, options=dict(form=0, block_order=order))
+        s.set_RI(RI)
+        outs.append(s.solve_incident(fields))
+        assert s.plan()['block_order'] == order
+        s.close()
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0])
+
+
 def test_batch_equals_individual_and_oracle():
     ri = (16, 16, 16)
     p = params(ri, iterations=16)
