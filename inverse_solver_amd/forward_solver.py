@@ -218,16 +218,28 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
         check(lib.cbs_set_io_kernels(self._h, *[_fptr(a) for a in self._io_arrays], _lib.CBS_MEM_HOST))
         self._io_ready = True
 
+    def _pupil_factors(self):
+        """NA circle and the Abbe-sine factors of the current parameters, computed once per
+        parameter set (the per-solve filters below reuse them)."""
+        p = self.parameters
+        key = (tuple(int(v) for v in p['size'][:2]), tuple(float(v) for v in p['resolution'][:2]),
+               float(p['wavelength']), float(p['NA']), float(p['RI_bg']))
+        cache = getattr(self, '_pupil_cache', None)
+        if cache is None or cache[0] != key:
+            na, cos_t, _ = pupil(p['size'][:2], p['resolution'], p['wavelength'], p['NA'], p['RI_bg'])
+            inv = np.where(na, 1.0 / np.sqrt(np.where(na, cos_t, 1.0)), 0.0)
+            fwd = np.sqrt(cos_t) * na
+            cache = (key, na, inv, fwd)
+            self._pupil_cache = cache
+        return cache[1:]
+
     def _pupil_filter(self, F, inverse):
         p = self.parameters
-        na, cos_t, _ = pupil(p['size'][:2], p['resolution'], p['wavelength'], p['NA'], p['RI_bg'])
+        na, inv, fwd = self._pupil_factors()
         shp = na.shape + (1,) * (F.ndim - 2)
         F = F * na.reshape(shp)
         if p['use_abbe_sine']:
-            if inverse:
-                F = F * np.where(na, 1.0 / np.sqrt(np.where(na, cos_t, 1.0)), 0.0).reshape(shp)
-            else:
-                F = F * (np.sqrt(cos_t) * na).reshape(shp)
+            F = F * (inv if inverse else fwd).reshape(shp)
         return F
 
     def transform_field_3D(self, F):
