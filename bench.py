@@ -30,6 +30,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# roofline.traffic is not counted in this run: it is the algorithmic bytes scaled by the
+# ratio that the calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE runs of this same command gave
+TRAFFIC_SOURCE = ('profiles/pass_real_traffic_160cubed_49rhs.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE '
+                  'runs of bench.py on this workload and plan, calibrated on a device copy of known size')
 METRIC = 'Gvoxel-iter/s at ≤1e-5 rel err vs FP64 oracle; p50/p99 solve latency'
 
 
@@ -508,8 +512,9 @@ def main():
     ap.add_argument('--angles', type=int, default=49)
     ap.add_argument('--iters', type=int, default=200)
     ap.add_argument('--seed', type=int, default=2)
-    ap.add_argument('--check-iters', type=int, default=24,
-                    help='iterations of the FP64-oracle accuracy check (1 angle, same grid)')
+    ap.add_argument('--check-depths', type=int, nargs='*', default=[24, 100],
+                    help='series depths besides the full one at which the FP64-oracle check is stated '
+                         '(1 angle, same grid and plan)')
     ap.add_argument('--cpu-iters', type=int, default=250,
                     help='iterations of the complex64 CPU-port sample (1 angle, same grid)')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline and the oracle check')
@@ -517,6 +522,9 @@ def main():
     ap.add_argument('--no-refform', action='store_true',
                     help='skip the reference-form GPU loop (torch/rocFFT restatement of solve_raw)')
     ap.add_argument('--no-tune', action='store_true', help='use the planner default instead of the autotuner')
+    ap.add_argument('--no-whole', action='store_true',
+                    help='skip the whole 2D-in/2D-out solve (cbs_solve_2d) timing')
+    ap.add_argument('--whole-steps', type=int, default=3, help='timed whole 2D-in/2D-out solves')
     ap.add_argument('--copy-probe-mb', type=int, default=0,
                     help='run one device-to-device copy of this many MiB first (calibrates HBM byte counters)')
     ap.add_argument('--workload', default='batched',
@@ -557,7 +565,9 @@ def main():
     RI, _ = spheres(tuple(ri), 8, (0.5, 2.0), (1.36, 1.48), seed=args.seed)
     u = incident_plane_waves(ri, A, 1.2, 0.532, 1.336, [0.1, 0.1, 0.1])
     p = workload_params(ri, M, A)
-    s = FORWARD_SOLVER_CONVERGENT_BORN(p, options=dict(time_kernels=1))
+    # complex64 bulk with FP64 residuals: the per-iteration |f_n|^2 of every rhs is
+    # accumulated in FP64 by the timed kernels; HIP events around every pass
+    s = FORWARD_SOLVER_CONVERGENT_BORN(p, options=dict(time_kernels=1, track_norms=1))
     s.set_RI(RI)
     sim = s.simulation_size
     nvox = int(np.prod(sim))
@@ -621,6 +631,7 @@ def main():
         % (pr_avg_ms, achieved_pr, pk_avg_ms, achieved_pk))
 
     result_gpu = d_out.to_host()
+    nrm = s.norms(A)           # FP64 |f_n|^2 per rhs and iteration, accumulated in the timed solves
     out = {
         'metric': METRIC, 'value': round(value, 3), 'unit': 'Gvoxel-iter/s', 'n_gpus': 1, 'steps': args.steps,
         'warmup': args.warmup, 'ms_per_step': round(ms, 3), 'higher_is_better': True, 'scaling': 'weak',
@@ -633,11 +644,77 @@ def main():
         'latency_ms': {'p50': round(p50, 3), 'p99': round(p99, 3)},
         'roofline': {'bound': 'hbm', 'achieved': round(achieved_pr, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': round(achieved_pr / HBM_PEAK_GBS, 4), 'traffic': measured_traffic(sim, A, plan, bytes_pr / pr_launches),
+                     'traffic_source': TRAFFIC_SOURCE,
                      'kernel': 'pass_real', 'avg_launch_ms': round(pr_avg_ms, 4),
                      'algorithmic_bytes_per_launch': round(bytes_pr / pr_launches),
                      'pass_kspace': {'achieved': round(achieved_pk, 1), 'avg_launch_ms': round(pk_avg_ms, 4),
                                      'algorithmic_bytes_per_launch': bytes_pk_launch}},
+        'fp64_residuals': {'tracked_in_timed_region': True,
+                           'last_term_over_first_max': float(np.sqrt(np.max(nrm[:, -1] / nrm[:, 0])))},
     }
+
+    if not args.no_cpu:
+        # accuracy of the timed result itself (rhs 0 of the last timed solve: the autotuned
+        # plan, all M iterations) and of the same handle at shorter depths, against one
+        # FP64 oracle run that records the series at each depth
+        from oracle.cbs import ForwardSolverConvergentBorn as Oracle
+        cores = os.cpu_count() or 1
+        u0 = u[..., 0]
+        depths = sorted(set([d for d in args.check_depths if 0 < d < M] + [M]))
+        snaps = {d: None for d in depths}
+        t0 = time.perf_counter()
+        o = Oracle(workload_params(ri, M, 1), io_kernels=False, workers=cores)
+        o.set_RI(RI)
+        o.born_loop(u0, snapshots=snaps)
+        t_or = time.perf_counter() - t0
+        errs = {M: _rel(result_gpu[0].T, snaps[M])}
+        for d in depths[:-1]:
+            s.parameters['iterations_number'] = d
+            s.set_RI(RI)                    # same handle: the autotuned plan stays
+            errs[d] = _rel(s.solve_incident(u0), snaps[d])
+        s.parameters['iterations_number'] = M
+        s.set_RI(RI)
+        err = errs[M]
+        out['accuracy'] = {'rel_l2_err_vs_fp64_oracle': err, 'angles': 1, 'iterations': M,
+                           'by_iterations': {str(d): errs[d] for d in depths},
+                           'what': 'rhs 0 of the timed solve (autotuned plan, all %d iterations) and the same '
+                                   'handle at the shorter depths, against the FP64 oracle' % M,
+                           'grid': sim, 'tolerance': 1e-5, 'pass': all(e <= 1e-5 for e in errs.values()),
+                           'oracle_seconds': round(t_or, 2)}
+        log('accuracy rel err %s (oracle %.1f s)' % ({d: '%.2e' % e for d, e in errs.items()}, t_or))
+
+    if not args.no_whole:
+        # the whole 2D-in/2D-out solve on device buffers (cbs_solve_2d: refocus of the 2D
+        # input into the 3D incident fields, the Born loop, the projection onto the 2D
+        # transmitted fields; 3D fields kept), production settings (graph replay)
+        import ctypes as C
+        from inverse_solver_amd._lib import lib, check, CBS_MEM_DEVICE
+        from inverse_solver_amd.phantoms import circle_illumination
+        t0 = time.perf_counter()
+        s.set_options(time_kernels=0)
+        _, src = s.source_2d(circle_illumination(ri[:2], A, 1.2, 0.532, [0.1] * 3))
+        log('I/O kernels and 2D sources %.1f s' % (time.perf_counter() - t0))
+        d_src = dv.DeviceArray.from_host(np.ascontiguousarray(src.T))
+        d_t = dv.DeviceArray(d_src.shape, np.complex64)
+        whole = lambda: check(lib.cbs_solve_2d(s._h, C.c_void_p(d_src.ptr), A, C.c_void_p(d_t.ptr), None,
+                                               C.c_void_p(d_out.ptr), CBS_MEM_DEVICE))
+        whole()
+        wms = []
+        for k in range(args.whole_steps):
+            e0.record(stream)
+            whole()
+            e1.record(stream)
+            e1.synchronize()
+            wms.append(e0.elapsed_ms(e1))
+        wm = float(np.mean(wms))
+        out['whole_solve_2d'] = {
+            'value': round(vox_iters / (wm / 1e3) / 1e9, 3), 'unit': 'Gvoxel-iter/s', 'ms_per_step': round(wm, 3),
+            'steps': args.whole_steps,
+            'what': 'cbs_solve_2d with 2D inputs and outputs in HBM: refocus (A6), the Born loop, the transmitted '
+                    '2D fields (A7) and the 3D fields, %d angles x %d iterations' % (A, M)}
+        log('whole 2D solve %.1f ms -> %.2f Gvoxel-iter/s' % (wm, out['whole_solve_2d']['value']))
+        d_src.free()
+        d_t.free()
     s.close()
 
     if not args.no_plain:
@@ -678,25 +755,6 @@ def main():
             'speedup_of_batched_fused': round(value / rval, 2), 'rel_diff_vs_batched': rdiff}
 
     if not args.no_cpu:
-        from oracle.cbs import ForwardSolverConvergentBorn as Oracle
-        cores = os.cpu_count() or 1
-        u0 = u[..., 0]
-        # accuracy: 1 angle, check_iters iterations, GPU vs FP64 oracle on the same grid
-        pc = workload_params(ri, args.check_iters, 1)
-        t0 = time.perf_counter()
-        o = Oracle(pc, io_kernels=False, workers=cores)
-        o.set_RI(RI)
-        ref = o.born_loop(u0)
-        t_or = time.perf_counter() - t0
-        sg = FORWARD_SOLVER_CONVERGENT_BORN(pc)
-        sg.set_RI(RI)
-        got = sg.solve_incident(u0)
-        sg.close()
-        err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
-        out['accuracy'] = {'rel_l2_err_vs_fp64_oracle': err, 'angles': 1, 'iterations': args.check_iters,
-                           'grid': sim, 'tolerance': 1e-5, 'pass': err <= 1e-5,
-                           'oracle_seconds': round(t_or, 2)}
-        log('accuracy rel err %.3e (%d iterations, oracle %.1f s)' % (err, args.check_iters, t_or))
         # CPU baseline: complex64 op-for-op port of the reference loop on this host
         pcpu = workload_params(ri, args.cpu_iters, 1)
         ow = Oracle(workload_params(ri, 3, 1), dtype=np.complex64, io_kernels=False, workers=cores)

@@ -91,7 +91,9 @@ typedef struct {
   int32_t tile_t;       /* fused form: force the k-space x tile (0 = planner's choice) */
   int32_t max_batch;    /* right-hand sides per launch (0 = all of the call) */
   int32_t track_norms;  /* 1: accumulate FP64 |f_n|^2 per rhs and iteration */
-  int32_t use_graph;    /* 1: capture the iteration loop in a hipGraph and replay it */
+  int32_t use_graph;    /* 1 (the default): capture the iteration loop of the fused and four-pass
+                           forms in a hipGraph on first use and replay it (not while time_kernels
+                           is set); 0: launch every kernel */
   int32_t time_kernels; /* 1: time each pass kind with HIP events (cbs_get_kernel_times) */
   int32_t threads_real;   /* fused form: force threads per block of the real-space pass (0 = planner) */
   int32_t threads_kspace; /* fused form: force threads per block of the k-space pass (0 = planner) */
@@ -219,8 +221,23 @@ int cbs_phase_branch_cuts(int device, const float* residues, int32_t* step1, int
  * called by BACKWARD_SOLVER_RYTOV.m:49.  unwrapped may alias wrapped. */
 int cbs_phase_unwrap(int device, const float* wrapped, float* unwrapped, const uint32_t dims[3], int mem);
 
-/* FP64 per-iteration |f_n|^2 of the last solve (track_norms=1): out[r * n_iter + j]. */
+/* FP64 per-iteration |f_n|^2 of the last solve (track_norms=1, or a convergence test
+ * set): out[r * n_iter + j], summed over the field components. */
 int cbs_get_norms(cbs_handle h, double* out, uint32_t n_rhs, uint32_t n_iter);
+
+/* FP64 convergence test (no counterpart in the reference, whose loop always runs
+ * Bornmax iterations, FORWARD_SOLVER_CONVERGENT_BORN.m:577): with tol > 0, rhs r stops
+ * after the first iteration n >= 3 at which |f_n|_2 <= tol |F_1 + F_2|_2, the norm of
+ * the series term against the first-order field (both summed in FP64 on the device
+ * over the simulation grid and the field components).  The test is made on the device
+ * by the next launches (a stopped rhs leaves Field and f unchanged), so no launch
+ * waits for the host and the iteration loop stays one captured graph.  tol = 0 (the
+ * default) turns it off.  Norms are then tracked as with track_norms. */
+int cbs_set_convergence(cbs_handle h, double tol);
+
+/* Iterations each rhs of the last solve ran (Bornmax unless the convergence test
+ * stopped it earlier): out[r], r < n_rhs, n_rhs equal to the last solve's. */
+int cbs_get_iterations(cbs_handle h, uint32_t* out, uint32_t n_rhs);
 
 /* Per-kind kernel time of the last solve (time_kernels=1):
  * out_ms[k], out_count[k] for k = 0 pass_real, 1 pass_kspace, 2 other. */

@@ -80,7 +80,33 @@ struct StateArgs {
   const uint8_t* parity;   // per-rhs flip parity (0/1)
   long long rhs_stride;    // elements between consecutive rhs in field/fstate/spec
   double* norms;           // optional [nrhs] FP64 |f_n|^2 accumulators (may be null)
+  double* fref;            // optional [nrhs] FP64 |Field|^2 accumulators, filled by the jj = 2 launch
 };
+
+// FP64 convergence test (an option; the reference runs a fixed Bornmax).  Rhs r has
+// stopped after iteration n >= 3 when |f_n|^2 <= tol^2 |F_1 + F_2|^2, both summed in
+// FP64 over the grid and the rhs's P component slots: the increment norm relative to
+// the first-order (jj = 2) Field.  Iterations after the stop leave Field and f alone;
+// an iteration that skips, the increment it adds is zero, so the test stays true and
+// every later launch skips as well.  norms = null: no test.
+struct ConvArgs {
+  const double* norms;     // [M][nslots] |f_n|^2 (row n - 1 holds iteration n)
+  const double* fref;      // [nslots] |F_1 + F_2|^2
+  double tol2;
+  int P;                   // component slots per rhs
+  int nslots;              // slots per norms row
+  int M;                   // Bornmax
+};
+
+__device__ __forceinline__ bool conv_stopped(const ConvArgs& cv, int n, int r) {
+  if (n < 3) return false;
+  double s = 0.0, ref = 0.0;
+  for (int q = 0; q < cv.P; ++q) {
+    s += cv.norms[(long long)(n - 1) * cv.nslots + cv.P * r + q];
+    ref += cv.fref[cv.P * r + q];
+  }
+  return s <= cv.tol2 * ref;
+}
 
 // flip state of iteration jj for rhs r: 1 = Greenp flipped (sigma = -1), 0 = sigma = +1
 __device__ __forceinline__ int flip_of(int jj, const uint8_t* parity, int r) { return (jj + parity[r]) & 1; }

@@ -24,6 +24,7 @@ struct PassRArgs {
   int order;   // block order: 0 = rhs fastest (V rows reused by consecutive blocks),
                // 1 = y2 fastest, then rhs (the b blocks of one (rhs, z) slab run together)
   int nt;      // 1: nontemporal access to the state arrays (they exceed the caches)
+  ConvArgs cv; // convergence test (cv.norms = null: off)
 };
 
 // ROWS = 0: the tile is the y coset y = y1 * b + y2 and the pass does the a-point
@@ -33,8 +34,8 @@ struct PassRArgs {
 // in the natural [z][y][x] layout for the separate y and z passes.
 //
 // STEADY = 1: the launch is known to be a steady-state one (update UPD_DEFER or
-// UPD_PAIR, next p = gamma x, no norm tracking), so the pointwise step compiles to
-// straight-line code; STEADY = 0 takes every update kind at run time.
+// UPD_PAIR, next p = gamma x), so the pointwise step compiles to straight-line code;
+// STEADY = 0 takes every update kind at run time.
 template <int NX, int A, int NT, int ROWS, int STEADY>
 __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
   static_assert(NX % 2 == 0, "pass_real processes x in pairs");
@@ -70,6 +71,32 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
   const int fwd = STEADY ? FWD_GAMMA : args.fwd;
   const float sg_inv = sign_inv(args.jj_inv ? flip_of(args.jj_inv, args.st.parity, r) : 0);
   const float sg_fwd = sign_fwd(args.jj_fwd ? flip_of(args.jj_fwd, args.st.parity, r) : 0);
+  // block-uniform base of this z plane; per-lane offsets inside a plane fit in 32 bits
+  const long long zoff = (long long)z * c.ny * NX;
+  // byte offset inside the z plane of pair i of the tile
+  auto pair_off = [&](int i) {
+    const unsigned e = 2u * (unsigned)i;
+    const unsigned y1 = e / NX, x = e - y1 * NX;
+    return ((ROWS ? (unsigned)y2 * A + y1 : y1 * (unsigned)c.b + (unsigned)y2) * NX + x) * 8u;
+  };
+
+  if (args.cv.norms && args.jj_inv > 0) {
+    // convergence test: this slot's rhs stopped after an earlier iteration
+    const int rr = r / args.cv.P;
+    if (conv_stopped(args.cv, args.jj_inv - 1, rr)) {
+      if (!conv_stopped(args.cv, args.jj_inv - 2, rr) && upd_kind(args.jj_inv - 1, args.cv.M) == UPD_DEFER) {
+        // the last iteration run deferred its increment: Field += f
+        float2* __restrict__ F = args.st.field + roff + zoff;
+        const float2* __restrict__ f = args.st.fstate + roff + zoff;
+        for (int i = tid; i < E2; i += NT) {
+          const unsigned g = pair_off(i);
+          const float4 a = *byte_at<const float4>(F, g), b = *byte_at<const float4>(f, g);
+          *byte_at<float4>(F, g) = make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+        }
+      }
+      return;
+    }
+  }
 
   // bulk load of the spectrum tile (all loads in flight before LDS writes)
   float4 reg[L];
@@ -113,14 +140,13 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
   // pointwise update, CH pairs per thread at a time
   const bool rf = reads_f(upd, fwd), rF = reads_F(upd);
   const bool wF = writes_F(upd), wf = fwd == FWD_GAMMA;
-  // block-uniform bases of this z plane; per-lane offsets inside a plane fit in 32 bits
-  const long long zoff = (long long)z * c.ny * NX;
   const float2* __restrict__ V = args.st.V + zoff;
   float2* __restrict__ F = args.st.field + roff + zoff;
   float2* __restrict__ f = args.st.fstate + roff + zoff;
-  // the steady instantiation never tracks norms (the host launches the general one then)
-  const bool want_norm = !STEADY && args.st.norms != nullptr;
-  double nrm = 0.0;
+  const bool want_norm = args.st.norms != nullptr;
+  // |F_1 + F_2|^2 for the convergence test, formed by the jj = 2 launch
+  const bool want_fref = !STEADY && args.st.fref != nullptr && upd == UPD_VQ_ADD;
+  double nrm = 0.0, nrmF = 0.0;
 #pragma unroll
   for (int k0 = 0; k0 < L; k0 += CH) {
     float4 vv[CH], xx[CH], FF[CH], rx[CH];
@@ -135,7 +161,7 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
       if (ok[k]) {
         const unsigned e = 2u * (unsigned)i;
         const unsigned y1 = e / NX, x = e - y1 * NX;
-        const unsigned g = ((ROWS ? (unsigned)y2 * A + y1 : y1 * (unsigned)c.b + (unsigned)y2) * NX + x) * 8u;
+        const unsigned g = pair_off(i);
         gi[k] = g;
         vv[k] = *byte_at<const float4>(V, g);
         // per-axis x ramp and mask pairs (read through the cache: the LDS holds three tiles per CU)
@@ -162,6 +188,8 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
                                   c.inv_eps, sg_inv, sg_fwd, Fn0, xn0, want_norm, nrm);
       const float2 p1 = born_step(upd, fwd, q1, cmul(hi2(rx[k]), ryz), mx[k].y * myz, hi2(vv[k]), hi2(xo4),
                                   hi2(Fo4), c.inv_eps, sg_inv, sg_fwd, Fn1, xn1, want_norm, nrm);
+      if (want_fref)
+        nrmF += (double)Fn0.x * Fn0.x + (double)Fn0.y * Fn0.y + (double)Fn1.x * Fn1.x + (double)Fn1.y * Fn1.y;
       if (wF) store_stream(byte_at<float4>(F, gi[k]), pack2(Fn0, Fn1), args.nt);
       if (wf) store_stream(byte_at<float4>(f, gi[k]), pack2(xn0, xn1), args.nt);
       if (fwd != FWD_NONE) {
@@ -171,6 +199,7 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
     }
   }
   if (want_norm) flush_norm(nrm, args.st.norms + r);
+  if (want_fref) flush_norm(nrmF, args.st.fref + r);
   if (fwd == FWD_NONE) return;
   __syncthreads();
   line_fft<NX, false, A, NT>(lds, row, s_twx, tid);
@@ -199,6 +228,7 @@ struct PassKArgs {
   int order;   // block order: 0 = rhs fastest, then a-index, then x tile;
                // 1 = x tile fastest, then a-index, then rhs (whole rows read together)
   int nt;      // 1: nontemporal stores of the spectrum (it exceeds the caches)
+  ConvArgs cv; // convergence test (cv.norms = null: off): a stopped rhs is skipped
 };
 
 // P = 1: scalar Green's function.  P = 3: vector run; the block holds the three
@@ -237,6 +267,7 @@ __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
     p1 = t2 % c.a;
     r = t2 / c.a;
   }
+  if (args.cv.norms && conv_stopped(args.cv, args.jj - 1, r)) return;
   const int x0 = xt * T;
   const long long rowstride = (long long)c.a * c.nx;     // between consecutive y2
   float2* __restrict__ spec = args.spec + (long long)(P * r) * args.rhs_stride + (long long)p1 * c.nx + x0;
@@ -407,6 +438,7 @@ void pass_kspace_seq3(PassKArgs args) {
     p1 = t2 % c.a;
     r = t2 / c.a;
   }
+  if (args.cv.norms && conv_stopped(args.cv, args.jj - 1, r)) return;
   const int x0 = xt * T;
   const long long rowstride = (long long)c.a * c.nx;
   float2* __restrict__ spec = args.spec + (long long)(3 * r) * args.rhs_stride + (long long)p1 * c.nx + x0;

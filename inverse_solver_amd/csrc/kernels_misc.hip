@@ -47,12 +47,25 @@ __global__ void green_plain_vec(float2* __restrict__ spec, long long comp_stride
 
 // Plain form: the pointwise part of one iteration on natural-layout arrays
 // (Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:595-642).
-__global__ void update_plain(StateArgs st, Tables tb, Consts c, int upd, int fwd, int jj_inv, int jj_fwd, int rhs) {
-  double nrm = 0.0;
+// cv: the convergence test of pass_real (a stopped rhs is skipped; the first skipped
+// iteration after a deferred one adds the deferred increment to Field).
+__global__ void update_plain(StateArgs st, Tables tb, Consts c, int upd, int fwd, int jj_inv, int jj_fwd, int rhs,
+                             ConvArgs cv) {
+  double nrm = 0.0, nrmF = 0.0;
   const long long ro = (long long)rhs * st.rhs_stride;
   float2* spec = st.spec + ro;
   float2* F = st.field + ro;
   float2* f = st.fstate + ro;
+  if (cv.norms && jj_inv > 0) {
+    const int rr = rhs / cv.P;
+    if (conv_stopped(cv, jj_inv - 1, rr)) {
+      if (!conv_stopped(cv, jj_inv - 2, rr) && upd_kind(jj_inv - 1, cv.M) == UPD_DEFER) {
+        GRID_STRIDE(i, c.nvox) F[i] = cadd(F[i], f[i]);
+      }
+      return;
+    }
+  }
+  const bool want_fref = st.fref != nullptr && upd == UPD_VQ_ADD;
   const float sg_inv = sign_inv(jj_inv ? flip_of(jj_inv, st.parity, rhs) : 0);
   const float sg_fwd = sign_fwd(jj_fwd ? flip_of(jj_fwd, st.parity, rhs) : 0);
   GRID_STRIDE(i, c.nvox) {
@@ -69,10 +82,24 @@ __global__ void update_plain(StateArgs st, Tables tb, Consts c, int upd, int fwd
     float2 Fn, xn;
     const float2 p = born_step(upd, fwd, q, r, m, st.V[i], xo, Fo, c.inv_eps, sg_inv, sg_fwd, Fn, xn, st.norms != nullptr, nrm);
     if (writes_F(upd)) F[i] = Fn;
+    if (want_fref) nrmF += (double)Fn.x * Fn.x + (double)Fn.y * Fn.y;
     if (fwd == FWD_GAMMA) f[i] = xn;
     if (fwd != FWD_NONE) spec[i] = p;
   }
   if (st.norms) flush_norm(nrm, st.norms + rhs);
+  if (want_fref) flush_norm(nrmF, st.fref + rhs);
+}
+
+// Flip parity of each component slot: (flip_state + (rhs0 + v / P) * M) & 1, the
+// persistent alternating-shift state of the reference's handle after the earlier
+// right-hand sides (Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:579-585).
+__global__ void fill_parity(uint8_t* __restrict__ parity, int nslots, int P, int flip_state, int rhs0, int M) {
+  GRID_STRIDE(v, nslots) parity[v] = (uint8_t)((flip_state + (long long)(rhs0 + v / P) * M) & 1);
+}
+
+// Zero fill of n 32-bit words (accumulators, counters), in place of a memset node.
+__global__ void zero_words(uint32_t* __restrict__ p, long long n) {
+  GRID_STRIDE(i, n) p[i] = 0u;
 }
 
 // p0 = (i/eps) * S / 2 with S = (V + i eps) .* u_in on the ROI, zero elsewhere

@@ -11,7 +11,10 @@ struct RoiGeom {
 
 __global__ void green_plain(float2* __restrict__ spec, Tables tb, Consts c, int flip);
 __global__ void green_plain_vec(float2* __restrict__ spec, long long comp_stride, Tables tb, Consts c, int flip);
-__global__ void update_plain(StateArgs st, Tables tb, Consts c, int upd, int fwd, int jj_inv, int jj_fwd, int rhs);
+__global__ void update_plain(StateArgs st, Tables tb, Consts c, int upd, int fwd, int jj_inv, int jj_fwd, int rhs,
+                             ConvArgs cv);
+__global__ void fill_parity(uint8_t* __restrict__ parity, int nslots, int P, int flip_state, int rhs0, int M);
+__global__ void zero_words(uint32_t* __restrict__ p, long long n);
 __global__ void seed_source(const float2* __restrict__ u_in, const float2* __restrict__ V, float2* __restrict__ fstate,
                             Consts c, RoiGeom rg, long long rhs_stride, int nrhs);
 __global__ void finalize_field(const float2* __restrict__ field, const float2* __restrict__ u_in, float2* __restrict__ u_out,

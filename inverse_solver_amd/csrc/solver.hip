@@ -132,12 +132,24 @@ struct Solver {
   Plan plan;
   TableSet tabs;        // tables of the active plan
   int flip_state = 0;   // persistent alternating-shift parity (reference handle semantics)
-  // workspace
+  // workspace: the handle's own buffers; ws points at them (or at memory a caller provides)
   DevBuf<float2> field, fstate, spec, uin, uout;
   DevBuf<uint8_t> parity;
-  DevBuf<double> norms;
+  DevBuf<double> norms, fref;
+  struct Ws {
+    float2* field = nullptr;
+    float2* fstate = nullptr;
+    float2* spec = nullptr;
+    uint8_t* parity = nullptr;
+    double* norms = nullptr;   // [M][slots] when norms are tracked
+    double* fref = nullptr;    // [slots] when the convergence test is on
+  } ws;
   std::vector<double> last_norms;
   int last_norm_rhs = 0, last_norm_iter = 0;
+  // FP64 convergence test (0 = off: every solve runs Bornmax iterations)
+  double conv_tol = 0.0;
+  std::vector<uint32_t> last_iters;   // iterations each rhs of the last solve ran
+  bool tracking() const { return opt.track_norms || conv_tol > 0.0; }
   // kernel timing
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, int>> ev_used;   // (kind, index of start event)
@@ -330,6 +342,7 @@ struct Solver {
 
   // ---- FP64 tables for the active plan
   void build_tables() {
+    drop_graph();   // captured loops hold the table addresses, which may move here
     const int a = plan.a, b = plan.b;
     TableSet& T = tabs;
     T.a = a; T.b = b;
@@ -472,10 +485,26 @@ struct Solver {
   StateArgs state_args(int nb) {
     (void)nb;
     StateArgs s;
-    s.V = V.p; s.field = field.p; s.fstate = fstate.p; s.spec = spec.p; s.parity = parity.p;
+    s.V = V.p; s.field = ws.field; s.fstate = ws.fstate; s.spec = ws.spec; s.parity = ws.parity;
     s.rhs_stride = nvox;
     s.norms = nullptr;
+    s.fref = nullptr;
     return s;
+  }
+  // convergence test arguments for a batch of nv component slots (norms = null: off)
+  ConvArgs conv_args(int nv, int M) const {
+    ConvArgs cv;
+    cv.norms = conv_tol > 0.0 ? ws.norms : nullptr;
+    cv.fref = ws.fref;
+    cv.tol2 = conv_tol * conv_tol;
+    cv.P = P;
+    cv.nslots = nv;
+    cv.M = M;
+    return cv;
+  }
+  // norm accumulators of the launch that completes iteration jj_inv (null: not tracked)
+  double* norm_row(int jj_inv, int nv) const {
+    return (tracking() && jj_inv > 0) ? ws.norms + (size_t)(jj_inv - 1) * nv : nullptr;
   }
 
   static int upd_of(int jj, int M) { return upd_kind(jj, M); }
@@ -492,25 +521,27 @@ struct Solver {
     const Consts c = consts(nv);
     const Consts ck = consts(nb);
     const Tables tb = tabs.view();
+    const ConvArgs cv = conv_args(nv, M);
     auto pass_r = [&](int upd, int fwd, int jj_inv, int jj_fwd) {
       PassRArgs args;
       args.tb = tb; args.c = c; args.st = state_args(nb);
-      if (opt.track_norms && jj_inv > 0) {
-        args.st.norms = norms.p + (size_t)(jj_inv - 1) * nv;
-      }
+      args.st.norms = norm_row(jj_inv, nv);
+      if (conv_tol > 0.0 && upd == UPD_VQ_ADD) args.st.fref = ws.fref;
       args.upd = upd; args.fwd = fwd; args.jj_inv = jj_inv; args.jj_fwd = jj_fwd;
       args.order = opt.block_order & 1;
       args.nt = stream_hint(nv);
+      args.cv = cv;
       const dim3 grid((unsigned)((long long)nv * plan.b * nz));
-      const bool steady = (upd == UPD_DEFER || upd == UPD_PAIR) && fwd == FWD_GAMMA && !args.st.norms;
+      const bool steady = (upd == UPD_DEFER || upd == UPD_PAIR) && fwd == FWD_GAMMA;
       const PassRFn fn = steady ? pr->fn_steady : pr->fn;
       timed(0, [&] { hipLaunchKernelGGL(fn, grid, dim3(pr->nt), 0, stream, args); });
     };
     auto pass_k = [&](int jj) {
       PassKArgs args;
-      args.tb = tb; args.c = ck; args.spec = spec.p; args.parity = parity.p; args.rhs_stride = nvox; args.jj = jj;
+      args.tb = tb; args.c = ck; args.spec = ws.spec; args.parity = ws.parity; args.rhs_stride = nvox; args.jj = jj;
       args.order = (opt.block_order >> 1) & 1;
       args.nt = stream_hint(nv);
+      args.cv = cv;
       const int ntile = (nx + pk->t - 1) / pk->t;
       const dim3 grid((unsigned)((long long)nb * plan.a * ntile));
       timed(1, [&] { hipLaunchKernelGGL(pk->fn, grid, dim3(pk->nt), 0, stream, args); });
@@ -537,23 +568,27 @@ struct Solver {
     ck.a = ny;   // k-space pass: every y is its own row, natural layout
     ck.b = 1;
     const Tables tb = tabs.view();
+    const ConvArgs cv = conv_args(nv, M);
     auto pass_r = [&](int upd, int fwd, int jj_inv, int jj_fwd) {
       PassRArgs args;
       args.tb = tb; args.c = c; args.st = state_args(nb);
-      if (opt.track_norms && jj_inv > 0) args.st.norms = norms.p + (size_t)(jj_inv - 1) * nv;
+      args.st.norms = norm_row(jj_inv, nv);
+      if (conv_tol > 0.0 && upd == UPD_VQ_ADD) args.st.fref = ws.fref;
       args.upd = upd; args.fwd = fwd; args.jj_inv = jj_inv; args.jj_fwd = jj_fwd;
       args.order = 0;
       args.nt = stream_hint(nv);
+      args.cv = cv;
       const dim3 grid((unsigned)((long long)nv * plan.b * nz));
-      const bool steady = (upd == UPD_DEFER || upd == UPD_PAIR) && fwd == FWD_GAMMA && !args.st.norms;
+      const bool steady = (upd == UPD_DEFER || upd == UPD_PAIR) && fwd == FWD_GAMMA;
       const PassRFn fn = steady ? pr->fn_steady : pr->fn;
       timed(0, [&] { hipLaunchKernelGGL(fn, grid, dim3(pr->nt), 0, stream, args); });
     };
     auto pass_k = [&](int jj) {
       PassKArgs args;
-      args.tb = tb; args.c = ck; args.spec = spec.p; args.parity = parity.p; args.rhs_stride = nvox; args.jj = jj;
+      args.tb = tb; args.c = ck; args.spec = ws.spec; args.parity = ws.parity; args.rhs_stride = nvox; args.jj = jj;
       args.order = 0;
       args.nt = stream_hint(nv);
+      args.cv = cv;
       const int ntile = (nx + pk->t - 1) / pk->t;
       const dim3 grid((unsigned)((long long)nb * ny * ntile));
       timed(1, [&] { hipLaunchKernelGGL(pk->fn, grid, dim3(pk->nt), 0, stream, args); });
@@ -563,7 +598,7 @@ struct Solver {
       const int nt = (nx + sy->t - 1) / sy->t;
       const long long nouter = (long long)nz * nv;
       const dim3 g((unsigned)(nt * nouter));
-      timed(2, [&] { hipLaunchKernelGGL(sy->fn, g, dim3(sy->nt), 0, stream, spec.p, nx, (long long)nx,
+      timed(2, [&] { hipLaunchKernelGGL(sy->fn, g, dim3(sy->nt), 0, stream, ws.spec, nx, (long long)nx,
                                         (long long)nx * ny, (int)nouter, tabs.tw_y.p, inverse); });
     };
     pass_r(UPD_NONE, FWD_P0, 0, 1);
@@ -583,12 +618,14 @@ struct Solver {
     const Tables tb = tabs.view();
     const int nv = nb * P;
     const unsigned pw_grid = grid_for(nvox);
+    const ConvArgs cv = conv_args(nv, M);
     auto update = [&](int upd, int fwd, int jj_inv, int jj_fwd) {
       StateArgs s = state_args(1);
-      s.norms = (opt.track_norms && jj_inv > 0) ? norms.p + (size_t)(jj_inv - 1) * nv : nullptr;
+      s.norms = norm_row(jj_inv, nv);
+      if (conv_tol > 0.0 && upd == UPD_VQ_ADD) s.fref = ws.fref;
       for (int q = 0; q < P; ++q)
         timed(2, [&] { hipLaunchKernelGGL(update_plain, dim3(pw_grid), dim3(256), 0, stream, s, tb, c, upd, fwd,
-                                          jj_inv, jj_fwd, P * r + q); });
+                                          jj_inv, jj_fwd, P * r + q, cv); });
     };
     auto fft3 = [&](float2* sp, int inverse) {
       const RowsInfo* rw = plan.rw;
@@ -614,7 +651,7 @@ struct Solver {
       if (!inverse) { fx(); fy(); fz(); }
       else { fz(); fy(); fx(); }
     };
-    float2* sp0 = spec.p + (long long)(P * r) * nvox;
+    float2* sp0 = ws.spec + (long long)(P * r) * nvox;
     update(UPD_NONE, FWD_P0, 0, 1);
     for (int jj = 1; jj <= M; ++jj) {
       for (int q = 0; q < P; ++q) fft3(sp0 + (long long)q * nvox, 0);
@@ -632,6 +669,7 @@ struct Solver {
   }
   std::vector<uint8_t> host_parity;
 
+  // the handle's own state buffers for batches of nb rhs, and ws pointing at them
   void ensure_workspace(int nb, bool host_io, int host_bufs = 1) {
     const size_t nv = (size_t)nb * P;
     const size_t n = nv * nvox;
@@ -639,10 +677,18 @@ struct Solver {
     fstate.ensure(n);
     spec.ensure(n);
     parity.ensure(nv);
+    if (tracking()) norms.ensure(std::max<size_t>(1, (size_t)prm.Bornmax) * nv);
+    if (conv_tol > 0.0) fref.ensure(nv);
     if (host_io) {
       uin.ensure((size_t)host_bufs * nv * nri);
       uout.ensure((size_t)host_bufs * nv * nri);
     }
+    ws.field = field.p;
+    ws.fstate = fstate.p;
+    ws.spec = spec.p;
+    ws.parity = parity.p;
+    ws.norms = tracking() ? norms.p : nullptr;
+    ws.fref = conv_tol > 0.0 ? fref.p : nullptr;
   }
 
   uint64_t footprint(uint32_t n_rhs) const {
@@ -661,35 +707,48 @@ struct Solver {
   }
 
   // Born loop for a batch whose incident fields sit at d_uin (device, RI grid).
-  void born_batch(const float2* d_uin, float2* d_uout, int nb, int rhs_index0) {
+  // Born loop for a batch whose incident fields sit at d_uin (device, RI grid), on the
+  // state buffers ws points at.  no_alloc: no graph capture and no kernel timing (both
+  // create runtime objects), for callers that must not allocate.
+  void born_batch(const float2* d_uin, float2* d_uout, int nb, int rhs_index0, bool no_alloc = false) {
     const int M = (int)prm.Bornmax;
-    // the previous batch's kernels may still read the device parity array
-    HIP_CHECK(hipStreamSynchronize(stream));
     const int nv = nb * P;
+    if (last_iters.size() < (size_t)(rhs_index0 + nb)) last_iters.resize((size_t)(rhs_index0 + nb), 0u);
+    if (tracking() && last_norms.size() < (size_t)(rhs_index0 + nb) * M)
+      last_norms.resize((size_t)(rhs_index0 + nb) * M, 0.0);
     host_parity.assign((size_t)nv, 0);
     for (int v = 0; v < nv; ++v)
       host_parity[v] = (uint8_t)((flip_state + (long long)(rhs_index0 + v / P) * M) & 1);
-    HIP_CHECK(hipMemcpy(parity.p, host_parity.data(), (size_t)nv, hipMemcpyHostToDevice));
+    // the device copy is formed in stream order, after the previous batch's kernels
+    hipLaunchKernelGGL(fill_parity, dim3(grid_for(nv)), dim3(256), 0, stream, ws.parity, nv, P, flip_state,
+                       rhs_index0, M);
     const Consts c = consts(nv);
     const RoiGeom g = roi();
-    if (opt.track_norms && M > 0) {
-      norms.ensure((size_t)M * nv);
-      HIP_CHECK(hipMemsetAsync(norms.p, 0, (size_t)M * nv * sizeof(double), stream));
+    if (tracking() && M > 0) {
+      const long long words = 2ll * M * nv;   // FP64 accumulators as 32-bit words
+      hipLaunchKernelGGL(zero_words, dim3(grid_for(words)), dim3(256), 0, stream,
+                         reinterpret_cast<uint32_t*>(ws.norms), words);
+      if (conv_tol > 0.0)
+        hipLaunchKernelGGL(zero_words, dim3(grid_for(2ll * nv)), dim3(256), 0, stream,
+                           reinterpret_cast<uint32_t*>(ws.fref), 2ll * nv);
     }
     {
       const long long tot = (long long)nv * nvox;
       timed(2, [&] { hipLaunchKernelGGL(seed_source, dim3(grid_for(tot)), dim3(256), 0, stream, d_uin,
-                                        V.p, fstate.p, c, g, nvox, nv); });
+                                        V.p, ws.fstate, c, g, nvox, nv); });
     }
     if (M == 0) {
-      HIP_CHECK(hipMemsetAsync(field.p, 0, (size_t)nv * nvox * sizeof(float2), stream));
+      const long long words = 2ll * nv * nvox;
+      hipLaunchKernelGGL(zero_words, dim3(grid_for(words)), dim3(256), 0, stream,
+                         reinterpret_cast<uint32_t*>(ws.field), words);
     } else if (plan.form == CBS_FORM_FUSED || plan.form == CBS_FORM_ROWS) {
-      if (opt.use_graph && !opt.time_kernels) {
+      if (opt.use_graph && !opt.time_kernels && !no_alloc) {
         // replay a captured iteration loop (the kernel arguments, including the
         // potential's scalars, are baked in; the cache is dropped whenever the
         // potential, the options or the plan change)
-        const std::vector<long long> key = {nb, M, (long long)field.p, (long long)fstate.p, (long long)spec.p,
-                                            (long long)parity.p, (long long)(opt.track_norms ? norms.p : nullptr),
+        const std::vector<long long> key = {nb, M, (long long)ws.field, (long long)ws.fstate, (long long)ws.spec,
+                                            (long long)ws.parity, (long long)ws.norms, (long long)ws.fref,
+                                            (long long)(conv_tol > 0.0),
                                             (long long)plan.pr, (long long)plan.pk, (long long)opt.block_order};
         GraphSlot* gs = nullptr;
         for (auto& g : gslot)
@@ -722,19 +781,31 @@ struct Solver {
     {
       const long long tot = (long long)nv * nri;
       timed(2, [&] { hipLaunchKernelGGL(finalize_field, dim3(grid_for(tot)), dim3(256), 0, stream,
-                                        field.p, d_uin, d_uout, c, g, nvox, nv); });
+                                        ws.field, d_uin, d_uout, c, g, nvox, nv); });
     }
     HIP_CHECK(hipGetLastError());
-    if (opt.track_norms && M > 0) {
-      std::vector<double> tmp((size_t)M * nv);
-      HIP_CHECK(hipMemcpyAsync(tmp.data(), norms.p, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (tracking() && M > 0) {
+      std::vector<double> tmp((size_t)M * nv), ref((size_t)nv, 0.0);
+      HIP_CHECK(hipMemcpyAsync(tmp.data(), ws.norms, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+      if (conv_tol > 0.0)
+        HIP_CHECK(hipMemcpyAsync(ref.data(), ws.fref, ref.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
       HIP_CHECK(hipStreamSynchronize(stream));
-      for (int r = 0; r < nb; ++r)
+      for (int r = 0; r < nb; ++r) {
+        double fr = 0.0;
+        for (int q = 0; q < P; ++q) fr += ref[(size_t)P * r + q];
+        uint32_t stop = (uint32_t)M;
         for (int j = 0; j < M; ++j) {
           double acc = 0.0;   // |f_n|^2 summed over the field components
           for (int q = 0; q < P; ++q) acc += tmp[(size_t)j * nv + (size_t)P * r + q];
           last_norms[(size_t)(rhs_index0 + r) * M + j] = acc;
+          // the device's test (conv_stopped) on the same FP64 values
+          if (conv_tol > 0.0 && stop == (uint32_t)M && j + 1 >= 3 && acc <= conv_tol * conv_tol * fr)
+            stop = (uint32_t)(j + 1);
         }
+        last_iters[(size_t)(rhs_index0 + r)] = stop;
+      }
+    } else {
+      for (int r = 0; r < nb; ++r) last_iters[(size_t)(rhs_index0 + r)] = (uint32_t)M;
     }
   }
 
@@ -803,11 +874,7 @@ struct Solver {
     }
     ensure_workspace(nbw, mem == CBS_MEM_HOST, pipe ? 2 : 1);
     const int M = (int)prm.Bornmax;
-    if (opt.track_norms) {
-      last_norms.assign((size_t)n_rhs * M, 0.0);
-      last_norm_rhs = (int)n_rhs;
-      last_norm_iter = M;
-    }
+    begin_solve(n_rhs, M);
     ev_used.clear();
     if (pipe) {
       solve_3d_host_pipelined(u_in, n_rhs, u_out, nbw);
@@ -837,6 +904,15 @@ struct Solver {
     HIP_CHECK(hipStreamSynchronize(stream));
     if (opt.time_kernels) resolve_times();
     if (prm.acyclic) flip_state = (int)((flip_state + (long long)n_rhs * M) & 1);
+  }
+
+  // per-solve records: norms (when tracked) and the iterations each rhs ran
+  void begin_solve(uint32_t n_rhs, int M) {
+    last_iters.assign((size_t)n_rhs, (uint32_t)M);
+    last_norms.clear();
+    if (tracking()) last_norms.assign((size_t)n_rhs * M, 0.0);
+    last_norm_rhs = tracking() ? (int)n_rhs : 0;
+    last_norm_iter = M;
   }
 
   static unsigned grid_for(long long n) {
@@ -928,11 +1004,7 @@ struct Solver {
       if (refl) io_rf.ensure(nvw * nf);
     }
     const int M = (int)prm.Bornmax;
-    if (opt.track_norms) {
-      last_norms.assign((size_t)n_rhs * M, 0.0);
-      last_norm_rhs = (int)n_rhs;
-      last_norm_iter = M;
-    }
+    begin_solve(n_rhs, M);
     ev_used.clear();
     const float sc = (float)(1.0 / (double)n2);
     const Consts c = consts(1);
@@ -958,7 +1030,7 @@ struct Solver {
       born_batch(io_uin.p, u3, nb, (int)r0);
       if (trans || refl) {
         // emitter -> fft2 per z -> sum_z E K (FP64) -> ifft2 -> crop  (:444-511)
-        timed(2, [&] { hipLaunchKernelGGL(io_emitter, dim3(grid_for(nv * n2 * rz)), dim3(256), 0, stream, field.p,
+        timed(2, [&] { hipLaunchKernelGGL(io_emitter, dim3(grid_for(nv * n2 * rz)), dim3(256), 0, stream, ws.field,
                                           io_uin.p, V.p, io_buf3.p, iog, c, rg.ox, rg.oy, rg.oz, nvox, nv,
                                           (float)prm.dV); });
         fft2d(io_buf3.p, (long long)nv * rz, 0);
@@ -1031,8 +1103,11 @@ struct Solver {
     const uint32_t saved_iters = prm.Bornmax;
     const int saved_flip = flip_state;
     const int saved_time = opt.time_kernels, saved_norm = opt.track_norms;
+    const double saved_tol = conv_tol;
     opt.time_kernels = 0;
     opt.track_norms = 0;
+    opt.use_graph = 0;      // every trial plan runs eagerly (its tables are rebuilt per trial)
+    conv_tol = 0.0;
     prm.Bornmax = iters;
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
@@ -1086,6 +1161,7 @@ struct Solver {
     } catch (...) {
       prm.Bornmax = saved_iters;
       flip_state = saved_flip;
+      conv_tol = saved_tol;
       opt = saved;
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
@@ -1106,6 +1182,7 @@ struct Solver {
     opt.block_order = best_order;
     opt.time_kernels = saved_time;
     opt.track_norms = saved_norm;
+    conv_tol = saved_tol;
     plan = best_plan;
     build_tables();
     HIP_CHECK(hipStreamSynchronize(stream));
@@ -1557,6 +1634,7 @@ int cbs_create(const cbs_geometry* geo, cbs_handle* out) {
     s->nvox = (long long)s->nx * s->ny * s->nz;
     s->nri = (long long)s->rx * s->ry * s->rz;
     s->opt.form = CBS_FORM_FUSED;
+    s->opt.use_graph = 1;   // the fused and four-pass loops replay a captured graph unless kernels are timed
     HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     Solver* raw = s.release();
     std::lock_guard<std::mutex> lk(g_mu);
@@ -1670,11 +1748,31 @@ int cbs_field_gradient(cbs_handle h, const void* u, const void* w, uint32_t n_rh
   });
 }
 
+int cbs_set_convergence(cbs_handle h, double tol) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    require(tol >= 0.0 && std::isfinite(tol), CBS_ERR_ARG, "convergence tolerance must be finite and >= 0");
+    DeviceGuard dg(s->device);
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+    s->conv_tol = tol;
+    s->drop_graph();
+  });
+}
+
+int cbs_get_iterations(cbs_handle h, uint32_t* out, uint32_t n_rhs) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    require(out != nullptr, CBS_ERR_ARG, "null output");
+    require(n_rhs == s->last_iters.size(), CBS_ERR_ARG, "n_rhs does not match the last solve");
+    std::memcpy(out, s->last_iters.data(), s->last_iters.size() * sizeof(uint32_t));
+  });
+}
+
 int cbs_get_norms(cbs_handle h, double* out, uint32_t n_rhs, uint32_t n_iter) {
   return guarded([&] {
     Solver* s = lookup(h);
     require(out != nullptr, CBS_ERR_ARG, "null output");
-    require(s->opt.track_norms, CBS_ERR_STATE, "track_norms is off");
+    require(s->tracking(), CBS_ERR_STATE, "track_norms is off (and no convergence test is set)");
     require((int)n_rhs == s->last_norm_rhs && (int)n_iter == s->last_norm_iter, CBS_ERR_ARG,
             "n_rhs/n_iter do not match the last solve");
     std::memcpy(out, s->last_norms.data(), s->last_norms.size() * sizeof(double));

@@ -325,7 +325,10 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
             raise ValueError('Input field must be either a scalar or a 2D vector')
         return inp
 
-    def _solve(self, input_field, field_3d_dev, return_2d=True):
+    def source_2d(self, input_field):
+        """The host half of solve before the library call (:415-417, _CUDA.m:122): the
+        input spectrum U after transform_field_3D, and the filtered real-space source
+        (x, y, P, A) complex64 that cbs_solve_2d takes.  Also uploads the I/O kernels."""
         p = self.parameters
         inp = self._inputs(input_field)
         if list(inp.shape[:2]) != [int(p['size'][0]), int(p['size'][1])]:
@@ -333,10 +336,15 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
         if not hasattr(self, 'Bornmax'):
             raise RuntimeError('set_RI must be called before solve')
         self._ensure_io()
-        A = inp.shape[3]
-        P = self.pole_num
         U = self.transform_field_3D(centered_fft2(inp.astype(np.complex128)))          # :415-417
         src = np.asfortranarray(centered_ifft2(U).astype(np.complex64))                # _CUDA.m:122
+        return U, src
+
+    def _solve(self, input_field, field_3d_dev, return_2d=True):
+        p = self.parameters
+        U, src = self.source_2d(input_field)
+        A = src.shape[3]
+        P = self.pole_num
         fx, fy = int(p['size'][0]), int(p['size'][1])
         rx, ry, rz = self.expected_RI_size
         want_t = bool(p['return_transmission']) and return_2d
@@ -403,6 +411,17 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
         out = np.zeros(n_rhs * self.Bornmax, dtype=np.float64)
         check(lib.cbs_get_norms(self._h, out.ctypes.data_as(C.POINTER(C.c_double)), n_rhs, self.Bornmax))
         return out.reshape(n_rhs, self.Bornmax)
+
+    def set_convergence(self, tol):
+        """FP64 convergence test (0 = off, the reference's fixed Bornmax): each rhs stops
+        after the first iteration n >= 3 with |f_n| <= tol |F_1 + F_2| (cbs_set_convergence)."""
+        check(lib.cbs_set_convergence(self._h, float(tol)))
+
+    def iterations(self, n_rhs):
+        """Iterations each rhs of the last solve ran."""
+        out = np.zeros(n_rhs, dtype=np.uint32)
+        check(lib.cbs_get_iterations(self._h, out.ctypes.data_as(C.POINTER(C.c_uint32)), n_rhs))
+        return out
 
     def kernel_times(self):
         ms = (C.c_double * 3)()

@@ -402,7 +402,7 @@ class ForwardSolverConvergentBorn:
         u_in = self.incident_field(source)
         return self.born_loop(u_in)
 
-    def born_loop(self, incident_field, increments=None):
+    def born_loop(self, incident_field, increments=None, tol=None, stats=None, snapshots=None):
         """:564-647: the two-chain, alternating-shift Born series.
 
         Scalar: incident_field (x, y, z).  Vector (pole_num = 3): (x, y, z, 3), with
@@ -418,6 +418,16 @@ class ForwardSolverConvergentBorn:
         ``increments``: an optional list that receives, per iteration, the FP64 sum
         over the simulation grid (and the components) of |Field_n|^2 at the moment
         Field_n is added to Field (:638), i.e. the norm of each series term.
+
+        ``tol``: the build's FP64 convergence test, which the reference does not have
+        (its loop always runs Bornmax iterations): the loop stops after the first
+        iteration n >= 3 at which |Field_n|^2 <= tol^2 |F_1 + F_2|^2 (the norm of the
+        series term against the first-order field, both over the simulation grid).
+        The flipped Greenp state still advances by Bornmax, as if every iteration had
+        run.  ``stats`` (a dict) receives 'iterations' and 'fref' (|F_1 + F_2|^2).
+
+        ``snapshots``: a dict whose keys are iteration counts n; each value is set to
+        Field[ROI] + u_in after iteration n, the result a Bornmax = n series gives.
         """
         p = self.parameters
         dt = self.dtype
@@ -443,6 +453,8 @@ class ForwardSolverConvergentBorn:
         ax = (0, 1, 2)
         w = self.workers
         temp = None
+        fref = None
+        ran = self.Bornmax
         for jj in range(1, self.Bornmax + 1):                                  # :577
             if p['acyclic']:                                                   # :579-585
                 Greenp = fft_flip(Greenp, [1, 1, 1], False)
@@ -474,10 +486,29 @@ class ForwardSolverConvergentBorn:
             if jj == 2:
                 temp = Field.copy()                                            # :634-637
             Field = Field + Field_n                                            # :638
+            nrm = None
+            if increments is not None or tol is not None:
+                nrm = float(np.sum(np.abs(Field_n.astype(np.complex128)) ** 2))
             if increments is not None:
-                increments.append(float(np.sum(np.abs(Field_n.astype(np.complex128)) ** 2)))
+                increments.append(nrm)
+            if jj == 2 and tol is not None:
+                fref = float(np.sum(np.abs(Field.astype(np.complex128)) ** 2))
             if jj == 3:
                 Field_n = Field_n + temp                                       # :639-642
+            if snapshots is not None and jj in snapshots:
+                snapshots[jj] = Field[roi] + inc
+            if tol is not None and jj >= 3 and nrm <= tol * tol * fref:
+                ran = jj
+                if (self.Bornmax - jj) % 2 and p['acyclic']:
+                    # the flips of the iterations not run
+                    Greenp = fft_flip(Greenp, [1, 1, 1], False)
+                    if vec:
+                        rads = fft_flip(rads, [1, 1, 1], False)
+                    ramp = np.conj(ramp)
+                break
+        if stats is not None:
+            stats['iterations'] = ran
+            stats['fref'] = fref
         if p['acyclic'] and self.Bornmax % 2:
             # persist the flipped state like the reference's handle properties
             self.Greenp = Greenp.astype(np.complex128)

@@ -151,7 +151,7 @@ def test_graph_replay_equals_eager():
     p = params(ri, iterations=14)
     RI = sphere_ri(ri, 0.5)
     fields = np.stack([random_field(ri, 20 + s) for s in range(2)], -1)
-    s = _gpu()(p)
+    s = _gpu()(p, options=dict(use_graph=0))
     s.set_RI(RI)
     eager = s.solve_incident(fields)
     g = _gpu()(p, options=dict(use_graph=1))
@@ -168,8 +168,8 @@ def test_graph_replay_equals_eager():
 def test_increment_norms_match_oracle(form, P):
     """track_norms: the FP64 |f_n|^2 of every series term over the simulation grid
     (summed over the components of a vector run) equals the oracle's, and tracking
-    them leaves the fields unchanged (the steady-state kernels do not track norms,
-    so those launches fall back to the general ones)."""
+    them leaves the fields unchanged (the steady-state kernels accumulate them as
+    the general ones do)."""
     ri = (16, 16, 16)
     M = 11
     p = params(ri, iterations=M, vector_simulation=(P == 3))

@@ -152,3 +152,63 @@ def test_eps_ramp_and_bornmax_closed_forms():
             ii[j] = dd
             line.append(o.phase_ramp[tuple(min(ii[q], o.phase_ramp.shape[q] - 1) for q in range(3))])
         np.testing.assert_allclose(np.array(line), expect, atol=1e-14)
+
+
+def test_convergence_stop_truncates_the_series():
+    """The oracle's FP64 convergence test (an option the reference does not have):
+    with tol, the loop stops after the first iteration n >= 3 whose term norm is at
+    most tol times the first-order field's norm, and returns exactly the fixed-length
+    series of n iterations; the flip state then stands where a full Bornmax run leaves
+    it (the next angle starts on the same shift as without the test).  No boundary
+    (ROI = the whole grid), so the first-order field's norm is pinned by the returned
+    2-iteration series."""
+    ri = (8, 7, 9)
+    M = 15
+    p = params(ri, bt_px_target=0, iterations=M)
+    RI = sphere_ri(ri, 0.25, n_s=1.40)
+    u = random_field(ri, 3)
+    o = Oracle(p, io_kernels=False)
+    o.set_RI(RI)
+    assert list(o.V.shape) == list(ri)
+    inc, st = [], {}
+    o.born_loop(u, increments=inc, tol=0.0, stats=st)
+    assert st['iterations'] == M and len(inc) == M
+    o2 = Oracle(dict(p, iterations_number=2), io_kernels=False)
+    o2.set_RI(RI)
+    F12 = o2.born_loop(u) - u
+    np.testing.assert_allclose(st['fref'], np.sum(np.abs(F12) ** 2), rtol=1e-12)
+    ratio = np.sqrt(np.array(inc) / st['fref'])
+    for tol_n in (6, 9):
+        tol = float(np.sqrt(ratio[tol_n - 1] * ratio[tol_n - 2]))
+        want = next(n for n in range(3, M + 1) if ratio[n - 1] <= tol)
+        o = Oracle(p, io_kernels=False)
+        o.set_RI(RI)
+        st2 = {}
+        got = o.born_loop(u, tol=tol, stats=st2)
+        assert st2['iterations'] == want < M
+        on = Oracle(dict(p, iterations_number=want), io_kernels=False)
+        on.set_RI(RI)
+        np.testing.assert_array_equal(got, on.born_loop(u))
+        # the next angle: the same as after a full-length first angle
+        ofull = Oracle(p, io_kernels=False)
+        ofull.set_RI(RI)
+        ofull.born_loop(u)
+        np.testing.assert_array_equal(o.born_loop(u), ofull.born_loop(u))
+
+
+def test_snapshots_equal_shorter_series():
+    """born_loop(snapshots={n: ...}) records after iteration n exactly what a series of
+    Bornmax = n returns (used to state the error at several depths from one run)."""
+    ri = (8, 7, 9)
+    p = params(ri, bt_px_target=2, iterations=12)
+    RI = sphere_ri(ri, 0.25, n_s=1.40)
+    u = random_field(ri, 4)
+    o = Oracle(p, io_kernels=False)
+    o.set_RI(RI)
+    snaps = {4: None, 7: None, 12: None}
+    full = o.born_loop(u, snapshots=snaps)
+    np.testing.assert_array_equal(snaps[12], full)
+    for n in (4, 7):
+        on = Oracle(dict(p, iterations_number=n), io_kernels=False)
+        on.set_RI(RI)
+        np.testing.assert_array_equal(snaps[n], on.born_loop(u))
