@@ -150,6 +150,14 @@ int cbs_solve_3d(cbs_handle h, const void* u_in, uint32_t n_rhs, void* u_out, in
  * available).  Replaces the green octants of CudaBorn action 0 (main.cpp:93-94). */
 int cbs_set_io_kernels(cbs_handle h, const void* refocus, const void* kernel_trans, const void* kernel_ref, int mem);
 
+/* The same from the first-octant kernels that CudaBorn action 0 receives
+ * (main.cpp:93-94; FORWARD_SOLVER_CONVERGENT_BORN_CUDA.m:28-38 passes
+ * refocusing_util(1:gx, 1:gy, 1:gz) and kernel_trans(1:gx, 1:gy, 1:gz), gx = floor(conv_x/2)+1,
+ * gy = floor(conv_y/2)+1, gz = floor(RI_z/2)+1).  They are expanded with the symmetry the
+ * reference's field_in/field_out kernels apply (execution2.cu:195-275); the reflection
+ * kernel is the conjugated transmission kernel, as there. */
+int cbs_set_io_kernels_octant(cbs_handle h, const void* refocus_octant, const void* trans_octant, int mem);
+
 /* Full 2D-in / 2D-out solve for n_rhs angles.  source_2d: field grid, real
  * space, already NA/Abbe filtered (what FORWARD_SOLVER_CONVERGENT_BORN_CUDA.m:122
  * passes).  Outputs (any may be null): trans_2d and ref_2d on the field grid

@@ -19,7 +19,8 @@ CBS_FORM_FUSED, CBS_FORM_PLAIN, CBS_FORM_ROWS = 0, 1, 2
 EXPORTED = [
     'cbs_create', 'cbs_destroy', 'cbs_set_options', 'cbs_get_options', 'cbs_set_potential', 'cbs_potential_max',
     'cbs_set_potential_ri', 'cbs_solve_3d',
-    'cbs_set_io_kernels', 'cbs_solve_2d', 'cbs_get_norms', 'cbs_get_kernel_times', 'cbs_get_stream',
+    'cbs_set_io_kernels', 'cbs_set_io_kernels_octant', 'cbs_solve_2d', 'cbs_get_norms', 'cbs_get_kernel_times',
+    'cbs_get_stream',
     'cbs_footprint', 'cbs_get_plan', 'cbs_autotune', 'cbs_tv_prox', 'cbs_tv_prox_complex', 'cbs_field_gradient',
     'cbs_phase_residues', 'cbs_phase_integrate', 'cbs_phase_branch_cuts', 'cbs_phase_unwrap',
     'cbs_set_convergence', 'cbs_get_iterations',
@@ -68,6 +69,7 @@ def _load():
         'cbs_set_potential_ri': ([vp, C.POINTER(PotentialParams), vp, i32, i32, C.c_double, C.c_double, i32], i32),
         'cbs_solve_3d': ([vp, vp, u32, vp, i32], i32),
         'cbs_set_io_kernels': ([vp, vp, vp, vp, i32], i32),
+        'cbs_set_io_kernels_octant': ([vp, vp, vp, i32], i32),
         'cbs_solve_2d': ([vp, vp, u32, vp, vp, vp, i32], i32),
         'cbs_get_norms': ([vp, C.POINTER(C.c_double), u32, u32], i32),
         'cbs_get_kernel_times': ([vp, C.POINTER(C.c_double), C.POINTER(C.c_uint64), u32], i32),

@@ -87,7 +87,14 @@ __device__ __forceinline__ float2 twiddle_ct(float2 v, int e) {
 // ---------------------------------------------------------------- small DFTs
 constexpr int first_factor(int r) {
   return (r % 4 == 0 && r != 4) ? 4 : (r % 2 == 0 && r != 2) ? 2 : (r % 3 == 0 && r != 3) ? 3
-       : (r % 5 == 0 && r != 5) ? 5 : r;
+       : (r % 5 == 0 && r != 5) ? 5 : (r % 7 == 0 && r != 7) ? 7 : (r % 11 == 0 && r != 11) ? 11
+       : (r % 13 == 0 && r != 13) ? 13 : r;
+}
+constexpr bool is_prime_above5(int r) {
+  if (r <= 5) return false;
+  for (int d = 2; d * d <= r; ++d)
+    if (r % d == 0) return false;
+  return true;
 }
 
 template <int R, bool INV>
@@ -158,32 +165,71 @@ struct Dft<5, INV> {
   }
 };
 
+// Odd prime R > 5 (7, 11, 13 for the run-time plans): with a_j = x_j + x_{R-j} and
+// b_j = x_j - x_{R-j}, j = 1..(R-1)/2, and t = 2 pi j k / R,
+//   X_k = x_0 + sum_j a_j cos t - i sum_j b_j sin t,  X_{R-k} = the same with + i,
+// and the signs of i swapped for the inverse.
+template <int R, bool INV>
+struct DftPrime {
+  __device__ __forceinline__ static void run(float2* v) {
+    constexpr int H = (R - 1) / 2;
+    float2 a[H], b[H];
+    const float2 x0 = v[0];
+    float2 sum = x0;
+#pragma unroll
+    for (int j = 1; j <= H; ++j) {
+      a[j - 1] = cadd(v[j], v[R - j]);
+      b[j - 1] = csub(v[j], v[R - j]);
+      sum = cadd(sum, a[j - 1]);
+    }
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+      float2 A = x0, B = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int j = 1; j <= H; ++j) {
+        const int e = (j * k) % R;
+        const float c = kRoots<R>.c[e], sn = kRoots<R>.s[e];
+        A = make_float2(fmaf(a[j - 1].x, c, A.x), fmaf(a[j - 1].y, c, A.y));
+        B = make_float2(fmaf(b[j - 1].x, sn, B.x), fmaf(b[j - 1].y, sn, B.y));
+      }
+      const float2 iB = make_float2(-B.y, B.x);
+      v[k] = INV ? cadd(A, iB) : csub(A, iB);
+      v[R - k] = INV ? csub(A, iB) : cadd(A, iB);
+    }
+    v[0] = sum;
+  }
+};
+
 // Composite R = F * M, four-step in registers:
 //   n = M*n1 + n2, k = k1 + F*k2;  X[k] = sum_n2 W_R^{n2 k1} W_M^{n2 k2} sum_n1 x[M n1 + n2] W_F^{n1 k1}
 template <int R, bool INV>
 struct Dft {
-  static constexpr int F = first_factor(R);
-  static constexpr int M = R / F;
-  static_assert(F > 1 && F < R, "radix not supported (needs factors 2, 3, 5)");
+  static constexpr int F = is_prime_above5(R) ? 1 : first_factor(R);
+  static constexpr int M = is_prime_above5(R) ? R : R / F;
+  static_assert(is_prime_above5(R) || (F > 1 && F < R), "radix not supported (factors 2, 3, 5, 7, 11, 13)");
   __device__ __forceinline__ static void run(float2* v) {
-    float2 w[R];
+    if constexpr (is_prime_above5(R)) {
+      DftPrime<R, INV>::run(v);
+    } else {
+      float2 w[R];
 #pragma unroll
-    for (int n2 = 0; n2 < M; ++n2) {
-      float2 u[F];
+      for (int n2 = 0; n2 < M; ++n2) {
+        float2 u[F];
 #pragma unroll
-      for (int n1 = 0; n1 < F; ++n1) u[n1] = v[M * n1 + n2];
-      Dft<F, INV>::run(u);
+        for (int n1 = 0; n1 < F; ++n1) u[n1] = v[M * n1 + n2];
+        Dft<F, INV>::run(u);
 #pragma unroll
-      for (int k1 = 0; k1 < F; ++k1) w[k1 * M + n2] = twiddle_ct<R, INV>(u[k1], n2 * k1);
-    }
+        for (int k1 = 0; k1 < F; ++k1) w[k1 * M + n2] = twiddle_ct<R, INV>(u[k1], n2 * k1);
+      }
 #pragma unroll
-    for (int k1 = 0; k1 < F; ++k1) {
-      float2 u[M];
+      for (int k1 = 0; k1 < F; ++k1) {
+        float2 u[M];
 #pragma unroll
-      for (int n2 = 0; n2 < M; ++n2) u[n2] = w[k1 * M + n2];
-      Dft<M, INV>::run(u);
+        for (int n2 = 0; n2 < M; ++n2) u[n2] = w[k1 * M + n2];
+        Dft<M, INV>::run(u);
 #pragma unroll
-      for (int k2 = 0; k2 < M; ++k2) v[k1 + F * k2] = u[k2];
+        for (int k2 = 0; k2 < M; ++k2) v[k1 + F * k2] = u[k2];
+      }
     }
   }
 };

@@ -1,8 +1,8 @@
 // Explicit instantiations of the templated passes and a lookup registry.
 // The shape lists cover the simulation grids of the supported workloads
 // (sim 96^3, 160^3, 128x128x135, 128^3, 192^2x128, 288^2x160, 576^2x320) and the small grids
-// used by the tests; a grid whose line lengths are absent here is rejected by
-// the planner with a clear error.
+// used by the tests; a grid whose line lengths are absent here runs the per-axis
+// form with run-time-planned line FFTs (fft_rt.hip).
 #include "cbs_passes.hpp"
 #include "kernels_misc.hpp"
 

@@ -15,6 +15,7 @@
 
 #include "../../include/cbs_hip.h"
 #include "cbs_passes.hpp"
+#include "fft_rt.hpp"
 #include "goldstein.hpp"
 #include "io_kernels.hpp"
 #include "kernels_misc.hpp"
@@ -113,7 +114,93 @@ struct Plan {
   const RowsInfo* rw = nullptr;
   const StridedInfo* sy = nullptr;
   const StridedInfo* sz = nullptr;
+  bool rt[3] = {false, false, false};   // per-axis form: axis j runs the run-time-planned FFT
 };
+
+// ------------------------------------------------------------------ run-time-planned line FFTs
+struct RtPlan {
+  RtFft k{};
+  DevBuf<float2> tw, chirp, bspec;
+};
+
+// Stage radices of a Stockham transform of length L (16, 8, 4, 2 for the power of two,
+// then 9, 3, 5, 7, 11, 13); false when a larger prime factor is left.
+static bool rt_factor(int L, std::vector<int>& rs) {
+  rs.clear();
+  int m = L;
+  while (m % 16 == 0) { rs.push_back(16); m /= 16; }
+  for (int r : {8, 4, 2})
+    if (m % r == 0) { rs.push_back(r); m /= r; }
+  for (int r : {9, 3, 5, 7, 11, 13})
+    while (m % r == 0) { rs.push_back(r); m /= r; }
+  return m == 1;
+}
+
+// smallest m' >= m whose prime factors are 2, 3, 5 and 7
+static int smooth_at_least(int m) {
+  for (;; ++m) {
+    int k = m;
+    for (int q : {2, 3, 5, 7})
+      while (k % q == 0) k /= q;
+    if (k == 1) return m;
+  }
+}
+
+// Plan of the length-n line transform: Stockham when n factors into the radices,
+// else Bluestein with the chirp c_k = exp(i pi k^2 / n) and the DFT of the chirp
+// filter over a length L >= 2n - 1 (FP64 on the host, rounded to complex64).
+static void build_rt(RtPlan& P, int n) {
+  require(n >= 1, CBS_ERR_ARG, "line length must be >= 1");
+  std::vector<int> rs;
+  int L = n, blue = 0;
+  if (!rt_factor(n, rs)) {
+    blue = 1;
+    L = smooth_at_least(2 * n - 1);
+    rt_factor(L, rs);
+  }
+  require(2 * L <= kRtLdsElems && (int)rs.size() <= kRtMaxStages, CBS_ERR_UNSUPPORTED,
+          "line length " + std::to_string(n) + " is beyond the run-time FFT (transform length " + std::to_string(L) +
+              " > " + std::to_string(kRtLdsElems / 2) + ")");
+  P.k = RtFft{};
+  P.k.n = n;
+  P.k.L = L;
+  P.k.blue = blue;
+  P.k.nst = (int)rs.size();
+  for (size_t i = 0; i < rs.size(); ++i) P.k.radix[i] = rs[i];
+  std::vector<float2> tw(L);
+  std::vector<std::complex<double>> w(L);
+  for (int e = 0; e < L; ++e) {
+    const double ang = -2.0 * M_PI * (double)e / (double)L;
+    w[e] = std::complex<double>(std::cos(ang), std::sin(ang));
+    tw[e] = make_float2((float)w[e].real(), (float)w[e].imag());
+  }
+  P.tw.upload(tw);
+  P.k.tw = P.tw.p;
+  P.k.chirp = P.k.bspec = nullptr;
+  if (blue) {
+    std::vector<std::complex<double>> c(n), bv(L, 0.0);
+    std::vector<float2> cf(n), bf(L);
+    for (int k = 0; k < n; ++k) {
+      const long long q = ((long long)k * k) % (2ll * n);   // k^2 mod 2n: the exact phase
+      const double ang = M_PI * (double)q / (double)n;
+      c[k] = std::complex<double>(std::cos(ang), std::sin(ang));
+      cf[k] = make_float2((float)c[k].real(), (float)c[k].imag());
+    }
+    for (int m = 0; m < n; ++m) bv[m] = c[m];
+    for (int m = 1; m < n; ++m) bv[L - m] = c[m];
+    for (int k = 0; k < L; ++k) {
+      std::complex<double> acc = 0.0;
+      for (int m = 0; m < L; ++m)
+        if (bv[m] != 0.0) acc += bv[m] * w[(size_t)(((long long)k * m) % L)];
+      acc /= (double)L;
+      bf[k] = make_float2((float)acc.real(), (float)acc.imag());
+    }
+    P.chirp.upload(cf);
+    P.bspec.upload(bf);
+    P.k.chirp = P.chirp.p;
+    P.k.bspec = P.bspec.p;
+  }
+}
 
 // ------------------------------------------------------------------ solver
 struct Solver {
@@ -176,6 +263,27 @@ struct Solver {
   DevBuf<float2> io_buf3, io_spec2, io_out_t, io_out_r, io_uin, io_src, io_u3d, io_tr, io_rf;
   const RowsInfo* io_rows = nullptr;
   const StridedInfo* io_cols = nullptr;
+  // run-time-planned line FFTs: the per-axis form's axes without a compiled kernel, and
+  // the convolution grid's x / y transforms (used when io_rows / io_cols are null)
+  RtPlan rtp[3], io_rtx, io_rty;
+
+  void rt_rows(const RtPlan& P, float2* data, long long nrows, int inverse) {
+    const int lines = rt_lines(P.k.L, 64);
+    const long long nblk = (nrows + lines - 1) / lines;
+    require(nblk < (1ll << 31), CBS_ERR_UNSUPPORTED, "run-time FFT batch too large for one launch");
+    const size_t lds = 2ull * lines * P.k.L * sizeof(float2);
+    timed(2, [&] { hipLaunchKernelGGL(fft_rows_rt, dim3((unsigned)nblk), dim3(256), lds, stream, data, nrows, P.k,
+                                      lines, inverse); });
+  }
+  void rt_strided(const RtPlan& P, float2* data, int ncols, long long es, long long os, long long nouter,
+                  int inverse) {
+    const int T = rt_lines(P.k.L, 16);
+    const long long nblk = (long long)((ncols + T - 1) / T) * nouter;
+    require(nblk < (1ll << 31), CBS_ERR_UNSUPPORTED, "run-time FFT batch too large for one launch");
+    const size_t lds = 2ull * T * P.k.L * sizeof(float2);
+    timed(2, [&] { hipLaunchKernelGGL(fft_strided_rt, dim3((unsigned)nblk), dim3(256), lds, stream, data, ncols,
+                                      es, os, (int)nouter, P.k, T, inverse); });
+  }
 
   void drop_graph() {
     for (auto& g : gslot) {
@@ -277,15 +385,20 @@ struct Solver {
       if (make_rows_plan(p)) return;   // then the four-pass form
       p.form = CBS_FORM_PLAIN;         // grids without either run the per-axis form
     }
+    // per-axis form: compiled line FFTs where the length has one, run-time-planned
+    // (Stockham, or Bluestein for prime factors above 13) for every other length
     p.rw = find_fft_rows(nx);
     p.sy = find_fft_strided(ny, 0);
     p.sz = find_fft_strided(nz, 0);
-    if (!p.rw || !p.sy || !p.sz)
-      throw CbsError(CBS_ERR_UNSUPPORTED, "no FFT kernels for simulation grid " + shape_str() +
-                                              " (line lengths must be instantiated 2,3,5-smooth sizes)");
+    p.rt[0] = !p.rw;
+    p.rt[1] = !p.sy;
+    p.rt[2] = !p.sz;
+    const int dims[3] = {nx, ny, nz};
+    for (int j = 0; j < 3; ++j)
+      if (p.rt[j]) build_rt(rtp[j], dims[j]);
     p.a = ny;
     p.b = 1;
-    p.t = p.sy->t;
+    p.t = p.sy ? p.sy->t : 0;
     plan = p;
   }
   // four-pass form: the real-space pass on blocks of contiguous rows, strided y
@@ -411,36 +524,39 @@ struct Solver {
       return ((double)m + sh) * prm.fourier_res[j] / prm.k0_nm;
     };
     auto rfl = [&](int j, int n, int flip, int d) { return flip ? r0(j, n, (n - d) % n) : r0(j, n, d); };
+    // frequency at spectrum position p of axis j: the compiled line FFTs leave a transposed
+    // order, the run-time-planned ones the natural order
+    auto p2f = [&](int j, int n, int p) { return plan.rt[j] ? p : pos2freq(n, p); };
     {
       std::vector<float> gx(2 * (size_t)nx + 64, 0.f);
       for (int f = 0; f < 2; ++f)
-        for (int p = 0; p < nx; ++p) gx[(size_t)f * nx + p] = (float)gfl(0, nx, f, pos2freq(nx, p));
+        for (int p = 0; p < nx; ++p) gx[(size_t)f * nx + p] = (float)gfl(0, nx, f, p2f(0, nx, p));
       T.green_x.upload(gx);
       std::vector<float> gy(2 * (size_t)ny, 0.f);
       for (int f = 0; f < 2; ++f)
         for (int k2 = 0; k2 < b; ++k2)
           for (int p1 = 0; p1 < a; ++p1) {
             const int idx = k2 * a + p1;   // plain and four-pass forms: position of the strided y transform
-            const int ky = (plan.form == CBS_FORM_FUSED) ? pos2freq(a, p1) + a * k2 : pos2freq(ny, idx);
+            const int ky = (plan.form == CBS_FORM_FUSED) ? pos2freq(a, p1) + a * k2 : p2f(1, ny, idx);
             gy[(size_t)f * ny + idx] = (float)gfl(1, ny, f, ky);
           }
       T.green_y.upload(gy);
       std::vector<float> gz(2 * (size_t)nz, 0.f);
       for (int f = 0; f < 2; ++f)
-        for (int p = 0; p < nz; ++p) gz[(size_t)f * nz + p] = (float)gfl(2, nz, f, pos2freq(nz, p));
+        for (int p = 0; p < nz; ++p) gz[(size_t)f * nz + p] = (float)gfl(2, nz, f, p2f(2, nz, p));
       T.green_z.upload(gz);
     }
     if (P == 3) {
       std::vector<float> qx(2 * (size_t)nx + 64, 0.f), qy(2 * (size_t)ny, 0.f), qz(2 * (size_t)nz, 0.f);
       for (int f = 0; f < 2; ++f) {
-        for (int p = 0; p < nx; ++p) qx[(size_t)f * nx + p] = (float)rfl(0, nx, f, pos2freq(nx, p));
+        for (int p = 0; p < nx; ++p) qx[(size_t)f * nx + p] = (float)rfl(0, nx, f, p2f(0, nx, p));
         for (int k2 = 0; k2 < b; ++k2)
           for (int p1 = 0; p1 < a; ++p1) {
             const int idx = k2 * a + p1;
-            const int ky = (plan.form == CBS_FORM_FUSED) ? pos2freq(a, p1) + a * k2 : pos2freq(ny, idx);
+            const int ky = (plan.form == CBS_FORM_FUSED) ? pos2freq(a, p1) + a * k2 : p2f(1, ny, idx);
             qy[(size_t)f * ny + idx] = (float)rfl(1, ny, f, ky);
           }
-        for (int p = 0; p < nz; ++p) qz[(size_t)f * nz + p] = (float)rfl(2, nz, f, pos2freq(nz, p));
+        for (int p = 0; p < nz; ++p) qz[(size_t)f * nz + p] = (float)rfl(2, nz, f, p2f(2, nz, p));
       }
       T.rad_x.upload(qx);
       T.rad_y.upload(qy);
@@ -633,16 +749,19 @@ struct Solver {
       const StridedInfo* sz = plan.sz;
       auto fx = [&] {
         const long long nrows = (long long)ny * nz;
+        if (plan.rt[0]) return rt_rows(rtp[0], sp, nrows, inverse);
         const dim3 g((unsigned)((nrows + rw->rows - 1) / rw->rows));
         timed(2, [&] { hipLaunchKernelGGL(rw->fn, g, dim3(rw->nt), 0, stream, sp, nrows, tabs.tw_x.p, inverse); });
       };
       auto fy = [&] {
+        if (plan.rt[1]) return rt_strided(rtp[1], sp, nx, (long long)nx, (long long)nx * ny, nz, inverse);
         const int nt = (nx + sy->t - 1) / sy->t;
         const dim3 g((unsigned)(nt * nz));
         timed(2, [&] { hipLaunchKernelGGL(sy->fn, g, dim3(sy->nt), 0, stream, sp, nx, (long long)nx,
                                           (long long)nx * ny, nz, tabs.tw_a.p, inverse); });
       };
       auto fz = [&] {
+        if (plan.rt[2]) return rt_strided(rtp[2], sp, nx, (long long)nx * ny, (long long)nx, ny, inverse);
         const int nt = (nx + sz->t - 1) / sz->t;
         const dim3 g((unsigned)(nt * ny));
         timed(2, [&] { hipLaunchKernelGGL(sz->fn, g, dim3(sz->nt), 0, stream, sp, nx, (long long)nx * ny,
@@ -933,8 +1052,8 @@ struct Solver {
             "RI_center other than [0 0]: the reference's crop_field2RI indexes outside the field grid");
     io_rows = find_fft_rows(cx);
     io_cols = find_fft_strided(cy, 0);
-    require(io_rows && io_cols, CBS_ERR_UNSUPPORTED,
-            "no FFT kernels for the convolution grid " + std::to_string(cx) + "x" + std::to_string(cy));
+    if (!io_rows) build_rt(io_rtx, cx);    // lengths without a compiled kernel: run-time plans
+    if (!io_cols) build_rt(io_rty, cy);
     const size_t n = (size_t)cx * cy * rz;
     const hipMemcpyKind kind = mem == CBS_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
     io_refocus_k.ensure(n);
@@ -947,8 +1066,8 @@ struct Solver {
       HIP_CHECK(hipMemcpy(io_kref.p, kref, n * sizeof(float2), kind));
     }
     std::vector<int> kx(cx), ky(cy);
-    for (int p = 0; p < cx; ++p) kx[p] = (pos2freq(cx, p) + cx / 2) % cx;
-    for (int p = 0; p < cy; ++p) ky[p] = (pos2freq(cy, p) + cy / 2) % cy;
+    for (int p = 0; p < cx; ++p) kx[p] = ((io_rows ? pos2freq(cx, p) : p) + cx / 2) % cx;
+    for (int p = 0; p < cy; ++p) ky[p] = ((io_cols ? pos2freq(cy, p) : p) + cy / 2) % cy;
     io_kcx.upload(kx);
     io_kcy.upload(ky);
     io_tw_cx.upload(twiddles(cx));
@@ -961,16 +1080,62 @@ struct Solver {
     io.ready = true;
   }
 
+  // First-octant kernels of CudaBorn action 0 (FORWARD_SOLVER_CONVERGENT_BORN_CUDA.m:28-35:
+  // refocusing_util and kernel_trans over floor(conv/2)+1 x floor(conv/2)+1 x floor(RI_z/2)+1,
+  // the centred (kx, ky) quarter from the most negative frequency to zero, z from the first
+  // plane to the middle one).  Expanded to the whole (conv_x, conv_y, RI_z) kernels with the
+  // symmetry the reference's field_in/field_out kernels read them with (execution2.cu:195-275):
+  // centred index c takes octant index conv/2 - |c - conv/2| on each axis; z takes
+  // RI_z/2 - |z - RI_z/2|, conjugated for the refocusing and negated and conjugated for the
+  // transmission from the middle plane on; the reflection kernel is the conjugated
+  // transmission kernel, as field_out_kernel forms it.  (The y index uses conv_y; the
+  // reference's conv_x there, quirk Q1, is the same for the square grids it supports.)
+  void set_io_kernels_octant(const float2* r_oct, const float2* t_oct, int mem) {
+    require(r_oct != nullptr && t_oct != nullptr, CBS_ERR_ARG, "refocus and kernel_trans octants are required");
+    require(mem == CBS_MEM_HOST || mem == CBS_MEM_DEVICE, CBS_ERR_ARG, "mem must be CBS_MEM_HOST or CBS_MEM_DEVICE");
+    const int gx = cx / 2 + 1, gy = cy / 2 + 1, gz = rz / 2 + 1;
+    const size_t no = (size_t)gx * gy * gz;
+    std::vector<float2> ro(no), to(no);
+    const hipMemcpyKind kind = mem == CBS_MEM_HOST ? hipMemcpyHostToHost : hipMemcpyDeviceToHost;
+    HIP_CHECK(hipMemcpy(ro.data(), r_oct, no * sizeof(float2), kind));
+    HIP_CHECK(hipMemcpy(to.data(), t_oct, no * sizeof(float2), kind));
+    const size_t n = (size_t)cx * cy * rz;
+    std::vector<float2> R(n), T(n), K(n);
+    for (int z = 0; z < rz; ++z) {
+      const int g3 = rz / 2 - std::abs(z - rz / 2);
+      const bool upper = z >= rz / 2;
+      for (int c2 = 0; c2 < cy; ++c2) {
+        const int g2 = cy / 2 - std::abs(c2 - cy / 2);
+        for (int c1 = 0; c1 < cx; ++c1) {
+          const int g1 = cx / 2 - std::abs(c1 - cx / 2);
+          const size_t o = (size_t)g1 + (size_t)gx * ((size_t)g2 + (size_t)gy * g3);
+          const size_t i = (size_t)c1 + (size_t)cx * ((size_t)c2 + (size_t)cy * z);
+          float2 r = ro[o], t = to[o];
+          if (upper) {
+            r = make_float2(r.x, -r.y);
+            t = make_float2(-t.x, t.y);
+          }
+          R[i] = r;
+          T[i] = t;
+          K[i] = make_float2(t.x, -t.y);
+        }
+      }
+    }
+    set_io_kernels(R.data(), T.data(), K.data(), CBS_MEM_HOST);
+  }
+
   // batched 2D FFT over nslices (cx x cy) planes
   void fft2d(float2* buf, long long nslices, int inverse) {
     const RowsInfo* rw = io_rows;
     const StridedInfo* st = io_cols;
     auto fx_ = [&] {
       const long long nrows = nslices * cy;
+      if (!rw) return rt_rows(io_rtx, buf, nrows, inverse);
       const dim3 g((unsigned)((nrows + rw->rows - 1) / rw->rows));
       timed(2, [&] { hipLaunchKernelGGL(rw->fn, g, dim3(rw->nt), 0, stream, buf, nrows, io_tw_cx.p, inverse); });
     };
     auto fy_ = [&] {
+      if (!st) return rt_strided(io_rty, buf, cx, (long long)cx, (long long)cx * cy, nslices, inverse);
       const int nt = (cx + st->t - 1) / st->t;
       require((long long)nt * nslices < (1ll << 31), CBS_ERR_UNSUPPORTED, "2D FFT batch too large for one launch");
       const dim3 g((unsigned)(nt * nslices));
@@ -1832,6 +1997,15 @@ int cbs_set_io_kernels(cbs_handle h, const void* refocus, const void* kernel_tra
     DeviceGuard dg(s->device);
     s->set_io_kernels(reinterpret_cast<const float2*>(refocus), reinterpret_cast<const float2*>(kernel_trans),
                       reinterpret_cast<const float2*>(kernel_ref), mem);
+  });
+}
+
+int cbs_set_io_kernels_octant(cbs_handle h, const void* refocus_octant, const void* trans_octant, int mem) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    DeviceGuard dg(s->device);
+    s->set_io_kernels_octant(reinterpret_cast<const float2*>(refocus_octant),
+                             reinterpret_cast<const float2*>(trans_octant), mem);
   });
 }
 

@@ -8,9 +8,12 @@
 //   u  = CbsMi355x(uint32(2), id, u_in)                    (3D incident field, RI grid, n_rhs on dim 4)
 //        CbsMi355x(uint32(3), id)
 //   [t, r, u3] = CbsMi355x(uint32(4), id, source_2d)       (filtered real-space 2D fields, n_rhs on dim 3)
-// Differences from CudaBorn: action 0 takes the whole (conv_x, conv_y, RI_z)
-// kernels instead of first octants, and one handle runs on one device
-// (geometry.used_gpu(1)); all angles of a call are solved as one batch.
+// Action 0 takes either CudaBorn's first octants (floor(conv/2)+1 x floor(conv/2)+1 x
+// floor(RI_z/2)+1 of refocusing_util and kernel_trans, as FORWARD_SOLVER_CONVERGENT_BORN_CUDA.m:28-38
+// passes them; expanded by cbs_set_io_kernels_octant with the reference's symmetry) or the
+// whole (conv_x, conv_y, RI_z) kernels plus kernel_ref.  Differences from CudaBorn: one
+// handle runs on one device (geometry.used_gpu(1)), and all angles of a call are solved as
+// one batch.
 //
 // Build (with MATLAB R2018a+):  mex -R2018a CbsMi355x.cpp -I../../include -L.. -lcbs_hip
 #include <cstdint>
@@ -107,9 +110,15 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
       e->geo.pole_num = (pn && mxGetNumberOfElements(pn)) ? (int32_t)num_at(pn, 0) : 1;
       check(cbs_create(&e->geo, &e->h));
       const size_t nk = (size_t)e->geo.conv_size[0] * e->geo.conv_size[1] * e->geo.RI_size[2];
-      const void* kref = nrhs > 5 ? (const void*)complex_single(prhs[5], nk, "kernel_ref") : nullptr;
-      check(cbs_set_io_kernels(e->h, complex_single(prhs[3], nk, "refocusing_util"),
-                               complex_single(prhs[4], nk, "kernel_trans"), kref, CBS_MEM_HOST));
+      const size_t no = (size_t)(e->geo.conv_size[0] / 2 + 1) * (e->geo.conv_size[1] / 2 + 1) * (e->geo.RI_size[2] / 2 + 1);
+      if (nrhs == 5 && mxGetNumberOfElements(prhs[3]) == no && no != nk) {
+        check(cbs_set_io_kernels_octant(e->h, complex_single(prhs[3], no, "green_for_cuda"),
+                                        complex_single(prhs[4], no, "green_for_cuda_2"), CBS_MEM_HOST));
+      } else {
+        const void* kref = nrhs > 5 ? (const void*)complex_single(prhs[5], nk, "kernel_ref") : nullptr;
+        check(cbs_set_io_kernels(e->h, complex_single(prhs[3], nk, "refocusing_util"),
+                                 complex_single(prhs[4], nk, "kernel_trans"), kref, CBS_MEM_HOST));
+      }
       g_handles.push_back(e);
       if (nlhs > 0) {
         plhs[0] = mxCreateNumericMatrix(1, 1, mxUINT32_CLASS, mxREAL);

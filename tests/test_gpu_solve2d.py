@@ -78,3 +78,32 @@ def test_ri_center_refused_like_the_reference(rc):
     with pytest.raises((CbsError, ValueError)):
         s.solve(u)
     s.close()
+
+
+@pytest.mark.parametrize('ri', [(16, 16, 16), (16, 16, 17)])
+def test_octant_kernels_give_the_full_kernel_solve(ri):
+    """CudaBorn action 0 hands over first octants (FORWARD_SOLVER_CONVERGENT_BORN_CUDA.m:28-38);
+    cbs_set_io_kernels_octant expands them (the symmetry is pinned on the oracle's kernels in
+    test_io_kernels.py).  The 2D solve then equals the one from the whole kernels."""
+    import ctypes as C
+    from inverse_solver_amd import FORWARD_SOLVER_CONVERGENT_BORN
+    from inverse_solver_amd._lib import lib, check, CBS_MEM_HOST
+    p, RI, u = _setup(ri, A=2)
+    full = FORWARD_SOLVER_CONVERGENT_BORN(p)
+    full.set_RI(RI)
+    t0, r0, f0 = full.solve(u)
+    octs = FORWARD_SOLVER_CONVERGENT_BORN(p)
+    octs.set_RI(RI)
+    octs._ensure_io()                               # builds the whole kernels on the host
+    R, T = octs._io_arrays[0], octs._io_arrays[1]
+    cx, cy, rz = R.shape
+    ro = np.asfortranarray(R[:cx // 2 + 1, :cy // 2 + 1, :rz // 2 + 1])
+    to = np.asfortranarray(T[:cx // 2 + 1, :cy // 2 + 1, :rz // 2 + 1])
+    check(lib.cbs_set_io_kernels_octant(octs._h, ro.ctypes.data_as(C.c_void_p), to.ctypes.data_as(C.c_void_p),
+                                        CBS_MEM_HOST))
+    t1, r1, f1 = octs.solve(u)
+    e = (rel(t1, t0), rel(r1, r0), rel(f1, f0))
+    print(ri, 'octant vs whole kernels: trans %.2e ref %.2e 3d %.2e' % e, flush=True)
+    assert e[0] < 1e-6 and e[2] < 1e-6 and e[1] < 1e-5
+    full.close()
+    octs.close()
