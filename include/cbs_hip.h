@@ -143,6 +143,20 @@ int cbs_set_potential_ri(cbs_handle h, const cbs_potential_params* prm, const vo
  * the copies of the neighbouring chunks run on a second stream while a chunk solves. */
 int cbs_solve_3d(cbs_handle h, const void* u_in, uint32_t n_rhs, void* u_out, int mem);
 
+/* cbs_solve_3d on device buffers without allocating: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The batch's state (Field, f, the
+ * spectrum, flip parities, FP64 norm accumulators) lives in a fixed workspace of
+ * CBS_NOALLOC_WORKSPACE_BYTES that the library holds on every GPU it runs on until the
+ * process ends; a batch that needs more (about 3 * 8 B * simulation voxels * field
+ * components per rhs, per batch of max_batch rhs) is refused with CBS_ERR_MEMORY.  The
+ * workspace is shared by every handle on a GPU: calls that overlap on one GPU race.
+ * The kernel plan, V and the tables are set up beforehand (cbs_set_potential*,
+ * cbs_autotune); this call creates no graph and no timing events (use_graph and
+ * time_kernels are ignored).  u_in and u_out are device pointers; they may alias. */
+#define CBS_NOALLOC_WORKSPACE_BYTES (64ull << 20)
+int cbs_solve_3d_noalloc(cbs_handle h, const void* u_in, uint32_t n_rhs, void* u_out);
+
 /* Uploads the refocusing and transmission kernels for the 2D input/output
  * projections (each conv_x * conv_y * RI_z complex64, centred (kx, ky) order,
  * z in real space, as FORWARD_SOLVER_CONVERGENT_BORN's refocusing_util and

@@ -23,7 +23,7 @@ EXPORTED = [
     'cbs_get_stream',
     'cbs_footprint', 'cbs_get_plan', 'cbs_autotune', 'cbs_tv_prox', 'cbs_tv_prox_complex', 'cbs_field_gradient',
     'cbs_phase_residues', 'cbs_phase_integrate', 'cbs_phase_branch_cuts', 'cbs_phase_unwrap',
-    'cbs_set_convergence', 'cbs_get_iterations',
+    'cbs_set_convergence', 'cbs_get_iterations', 'cbs_solve_3d_noalloc',
     'cbs_last_error', 'cbs_abi_version',
 ]
 
@@ -88,6 +88,7 @@ def _load():
         'cbs_phase_branch_cuts': ([i32, vp, vp, vp, C.POINTER(C.c_uint32), i32], i32),
         'cbs_phase_unwrap': ([i32, vp, vp, C.POINTER(C.c_uint32), i32], i32),
         'cbs_set_convergence': ([vp, C.c_double], i32),
+        'cbs_solve_3d_noalloc': ([vp, vp, u32, vp], i32),
         'cbs_get_iterations': ([vp, C.POINTER(C.c_uint32), u32], i32),
         'cbs_last_error': ([], C.c_char_p),
         'cbs_abi_version': ([], i32),

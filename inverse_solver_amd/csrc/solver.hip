@@ -202,6 +202,36 @@ static void build_rt(RtPlan& P, int n) {
   }
 }
 
+// ------------------------------------------------------------------ no-allocation workspace
+// Device memory of cbs_solve_3d_noalloc: the state arrays (Field, f, the spectrum), the
+// per-slot flip parities and the FP64 norm accumulators of one batch.  The module holds
+// one copy on every GPU it runs on, for the life of the process; every such call on a
+// GPU shares that GPU's copy.  In host code the array's name is not a device address:
+// scratch_base() gives the address of the current device's copy.
+constexpr size_t kScratchBytes = CBS_NOALLOC_WORKSPACE_BYTES;
+__attribute__((used)) static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// Address of `scratch` on device dev, which must be the current device; resolved once
+// per device (hipGetSymbolAddress) and kept.
+static unsigned char* scratch_base(int dev) {
+  require(dev >= 0 && dev < kMaxDevices, CBS_ERR_ARG, "device ordinal out of range for the no-allocation workspace");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr) {
+      (void)hipGetLastError();
+      throw CbsError(CBS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
+
 // ------------------------------------------------------------------ solver
 struct Solver {
   int device = 0;
@@ -1025,6 +1055,60 @@ struct Solver {
     if (prm.acyclic) flip_state = (int)((flip_state + (long long)n_rhs * M) & 1);
   }
 
+  // cbs_solve_3d on device buffers with the state in the no-allocation workspace scr
+  // (kScratchBytes): no device allocation, no graph capture, no timing events.
+  void solve_3d_noalloc(const float2* u_in, uint32_t n_rhs, float2* u_out, unsigned char* scr) {
+    require(have_potential, CBS_ERR_STATE, "cbs_set_potential must be called before solving");
+    require(n_rhs >= 1, CBS_ERR_ARG, "n_rhs must be >= 1");
+    require(u_in && u_out, CBS_ERR_ARG, "null field pointer");
+    const int M = (int)prm.Bornmax;
+    const int nbw = batch_width(n_rhs);
+    const size_t nv = (size_t)nbw * P;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    size_t off = 0;
+    const size_t o_par = off;
+    off += up(nv);
+    const size_t o_nrm = off;
+    if (tracking()) off += up((size_t)std::max(M, 1) * nv * sizeof(double));
+    const size_t o_ref = off;
+    if (conv_tol > 0.0) off += up(nv * sizeof(double));
+    const size_t state = up(nv * (size_t)nvox * sizeof(float2));
+    const size_t o_state = off;
+    off += 3 * state;
+    if (off > kScratchBytes) {
+      const size_t per_rhs = 3 * (size_t)P * (size_t)nvox * sizeof(float2);
+      throw CbsError(CBS_ERR_MEMORY, "cbs_solve_3d_noalloc: a batch of " + std::to_string(nbw) + " rhs on the " +
+                                         shape_str() + " grid needs " + std::to_string(off) + " bytes, over the " +
+                                         std::to_string(kScratchBytes) + "-byte no-allocation workspace (about " +
+                                         std::to_string(kScratchBytes / per_rhs) +
+                                         " rhs of this grid fit; set max_batch or call cbs_solve_3d)");
+    }
+    ws.parity = scr + o_par;
+    ws.norms = tracking() ? reinterpret_cast<double*>(scr + o_nrm) : nullptr;
+    ws.fref = conv_tol > 0.0 ? reinterpret_cast<double*>(scr + o_ref) : nullptr;
+    ws.field = reinterpret_cast<float2*>(scr + o_state);
+    ws.fstate = reinterpret_cast<float2*>(scr + o_state + state);
+    ws.spec = reinterpret_cast<float2*>(scr + o_state + 2 * state);
+    const int saved_time = opt.time_kernels;
+    opt.time_kernels = 0;
+    try {
+      begin_solve(n_rhs, M);
+      for (uint32_t r0 = 0; r0 < n_rhs; r0 += (uint32_t)nbw) {
+        const int nb = (int)std::min<uint32_t>((uint32_t)nbw, n_rhs - r0);
+        const size_t o = (size_t)r0 * P * nri;
+        born_batch(u_in + o, u_out + o, nb, (int)r0, /*no_alloc=*/true);
+      }
+      HIP_CHECK(hipStreamSynchronize(stream));
+    } catch (...) {
+      opt.time_kernels = saved_time;
+      ws = Ws();
+      throw;
+    }
+    opt.time_kernels = saved_time;
+    ws = Ws();     // the workspace belongs to the GPU, not to this handle
+    if (prm.acyclic) flip_state = (int)((flip_state + (long long)n_rhs * M) & 1);
+  }
+
   // per-solve records: norms (when tracked) and the iterations each rhs ran
   void begin_solve(uint32_t n_rhs, int M) {
     last_iters.assign((size_t)n_rhs, (uint32_t)M);
@@ -1801,6 +1885,12 @@ int cbs_create(const cbs_geometry* geo, cbs_handle* out) {
     s->opt.form = CBS_FORM_FUSED;
     s->opt.use_graph = 1;   // the fused and four-pass loops replay a captured graph unless kernels are timed
     HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    // set-up of the no-allocation workspace on this device (a failure is reported by
+    // cbs_solve_3d_noalloc, which resolves it again)
+    try {
+      (void)scratch_base(s->device);
+    } catch (const CbsError&) {
+    }
     Solver* raw = s.release();
     std::lock_guard<std::mutex> lk(g_mu);
     g_live.insert(raw);
@@ -1883,6 +1973,15 @@ int cbs_solve_3d(cbs_handle h, const void* u_in, uint32_t n_rhs, void* u_out, in
     Solver* s = lookup(h);
     DeviceGuard dg(s->device);
     s->solve_3d(reinterpret_cast<const float2*>(u_in), n_rhs, reinterpret_cast<float2*>(u_out), mem);
+  });
+}
+
+int cbs_solve_3d_noalloc(cbs_handle h, const void* u_in, uint32_t n_rhs, void* u_out) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    DeviceGuard dg(s->device);
+    unsigned char* scr = scratch_base(s->device);
+    s->solve_3d_noalloc(reinterpret_cast<const float2*>(u_in), n_rhs, reinterpret_cast<float2*>(u_out), scr);
   });
 }
 
