@@ -261,7 +261,8 @@ class SolverCache:
                 s.set_RI(r.RI)
                 return s.solve(r.fields)
             if r.kind == 'adjoint':
-                s.set_RI(np.conj(np.flip(r.RI, axis=2)))
+                # the adjoint potential conj(flip_z(RI)) (BACKWARD_SOLVER_MULTI.m:48), formed on the device
+                s.set_RI(r.RI, flip_conj=True)
                 t, rf, u = s.solve(np.conj(r.fields))
                 return t, rf, (None if u is None else np.flip(np.conj(u), axis=2))
             raise ValueError('unknown request kind %r' % r.kind)

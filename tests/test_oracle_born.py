@@ -212,3 +212,31 @@ def test_snapshots_equal_shorter_series():
         on = Oracle(dict(p, iterations_number=n), io_kernels=False)
         on.set_RI(RI)
         np.testing.assert_array_equal(snaps[n], on.born_loop(u))
+
+
+def test_reciprocity_dot_product():
+    """Adjoint (dot-product) pin of the series behind the MULTI gradient
+    (Codes/BACKWARD_SOLVER_MULTI.m:30-54): the solution operator (I - G V)^-1 G of the
+    Helmholtz equation is symmetric because G is, so the scattered field S x =
+    Field[ROI] of an incident x obeys y^T Vt (S x) = (S y)^T Vt x (bilinear, no
+    conjugate).  The converged series keeps it to its truncation and boundary
+    error; the sesquilinear pairing (S y)^H Vt x, which an adjoint built with a
+    conjugate too many would satisfy, does not hold."""
+    ri = (10, 9, 8)
+    p = params(ri, bt_px_target=0, iterations=300)
+    RI = sphere_ri(ri, 0.3, n_s=1.40)
+    x, y = random_field(ri, 1), random_field(ri, 2)
+    outs = []
+    for f in (x, y):
+        o = Oracle(p, io_kernels=False)
+        o.set_RI(RI)
+        outs.append(o.born_loop(f) - f)
+    R = o.ROI
+    roi = tuple(slice(R[2 * j] - 1, R[2 * j + 1]) for j in range(3))
+    Vt = o.V[roi] + 1j * o.eps_imag
+    Sx, Sy = outs
+    a = np.sum(y * Vt * Sx)
+    b = np.sum(Sy * Vt * x)
+    c = np.sum(np.conj(Sy) * Vt * x)
+    assert abs(a - b) < 2e-3 * abs(a)
+    assert abs(a - c) > 0.3 * abs(a)
