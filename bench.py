@@ -422,6 +422,37 @@ def reference_form_gpu(s, u0, M, log_fn):
     return ms, out.cpu().numpy()
 
 
+def workload_retrieval(args):
+    """Experimental input pipeline (FIELD_EXPERIMENTAL_RETRIEVAL.get_fields) on synthetic
+    off-axis holograms: 49 background + 49 sample images of 512^2 pixels (a phase bump of
+    7 rad behind tilted plane waves, reference carrier at a quarter of the sampling rate),
+    sideband filtering on the host, unwrapping on the GPU (cbs_phase_unwrap)."""
+    from inverse_solver_amd.field_retrieval import FIELD_EXPERIMENTAL_RETRIEVAL, get_default_parameters
+    n, A = 512, 49
+    i = np.arange(n)
+    x = i - n // 2
+    phi = 7.0 * np.exp(-(x[:, None] ** 2 + (x[None, :] + 5) ** 2) / (2 * 44.0 ** 2))
+    plane = lambda k: np.exp(2j * np.pi * (k[0] * i[:, None] + k[1] * i[None, :]) / n)   # noqa: E731
+    R = plane((n // 4, n // 4))
+    rng = np.random.default_rng(4)
+    tilts = [(0, 0)] + [tuple(rng.integers(-20, 21, 2)) for _ in range(A - 1)]
+    bg = np.stack([100 * np.abs(R + 0.5 * plane(t)) ** 2 for t in tilts], -1)
+    sp = np.stack([100 * np.abs(R + 0.5 * plane(t) * np.exp(1j * phi)) ** 2 for t in tilts], -1)
+    p = get_default_parameters(dict(size=[n, n, 1], resolution=[0.1] * 3, resolution_image=[0.1, 0.1]))
+    FIELD_EXPERIMENTAL_RETRIEVAL(dict(p)).get_fields(bg[:, :, :2], sp[:, :, :2])      # warm-up
+    times = []
+    for _ in range(max(args.steps, 1)):
+        t0 = time.perf_counter()
+        inp, out, _, k0s, _, _ = FIELD_EXPERIMENTAL_RETRIEVAL(dict(p)).get_fields(bg, sp)
+        times.append(time.perf_counter() - t0)
+    sec = float(np.median(times))
+    ratio = out[:, :, 0, :] / inp[:, :, 0, :]
+    err = _rel(ratio, np.broadcast_to(np.exp(1j * phi)[2:-2, 2:-2, None], ratio.shape))
+    return _line('holograms per second (off-axis field retrieval)', round(A / sec, 2), 'holograms/s',
+                 'FIELD_EXPERIMENTAL_RETRIEVAL.get_fields: %d background + %d sample holograms of %d^2 px' % (A, A, n),
+                 sec * 1e3, args, {'accuracy': {'rel_l2_err_ratio_vs_object': err}}, dtype='complex128')
+
+
 def workload_vector(args):
     """Vector (dyadic Green's function) Born loop: 128^3 RI + 16 px = 160^3 sim, 8 spheres
     (seed 2), p-polarised plane waves for 16 circle angles, 200 iterations; inputs
@@ -528,7 +559,7 @@ def main():
     ap.add_argument('--copy-probe-mb', type=int, default=0,
                     help='run one device-to-device copy of this many MiB first (calibrates HBM byte counters)')
     ap.add_argument('--workload', default='batched',
-                    choices=['batched', 'c1', 'inverse', 'serving', 'large', 'vector'],
+                    choices=['batched', 'c1', 'inverse', 'serving', 'large', 'vector', 'retrieval'],
                     help='batched: the headline 49-angle solve (default); the others are the secondary '
                          'configurations, each printing its own JSON line')
     ap.add_argument('--requests', type=int, default=200, help='serving: number of requests')
@@ -546,7 +577,8 @@ def main():
 
     if args.workload != 'batched':
         out = {'c1': workload_c1, 'inverse': workload_inverse, 'serving': workload_serving,
-               'large': workload_large, 'vector': workload_vector}[args.workload](args)
+               'large': workload_large, 'vector': workload_vector,
+               'retrieval': workload_retrieval}[args.workload](args)
         print(json.dumps(out), flush=True)
         return
 
@@ -716,6 +748,25 @@ def main():
         d_src.free()
         d_t.free()
     s.close()
+
+    # practical HBM ceiling on this box: device-to-device copy of 2 GiB (bytes read +
+    # written), best of five
+    ca, cb = dv.DeviceArray((1 << 28,), np.complex64), dv.DeviceArray((1 << 28,), np.complex64)
+    cb.copy_from(ca)
+    dv.synchronize()
+    best = 1e30
+    for _ in range(5):
+        t0 = time.perf_counter()
+        cb.copy_from(ca)
+        dv.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    copy_gbs = 2.0 * ca.nbytes / best / 1e9
+    out['hbm_copy_ceiling'] = {'GB_s': round(copy_gbs, 1), 'frac_of_peak': round(copy_gbs / HBM_PEAK_GBS, 4),
+                               'what': 'hipMemcpy device-to-device of 2 GiB, read + written bytes, best of 5 '
+                                       '(host timer around the synchronous copy)'}
+    log('copy ceiling %.0f GB/s' % copy_gbs)
+    ca.free()
+    cb.free()
 
     if not args.no_plain:
         # plain single-angle form (unfused per-axis kernels, batch 1) on the same inputs
