@@ -339,9 +339,9 @@ __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
     const float* gy = tb.green_y + flip * c.ny + p1;
     const float km2 = c.km2, eps2 = c.eps * c.eps, inv_n = c.inv_n, eps = c.eps;
     if constexpr (P == 1) {
-      auto green = [&](int l, int pz, float2 v) {
+      auto green = [&](int l, int pz0, int pz1, bool hi, float2 v) {
         const int k2 = l / T, t = l - k2 * T;
-        const float d = gx[t < tvalid ? t : 0] + gy[k2 * c.a] + gz[pz] - km2;
+        const float d = gx[t < tvalid ? t : 0] + gy[k2 * c.a] + pick2(gz, pz0, pz1, hi) - km2;
         const float s = inv_n * __builtin_amdgcn_rcpf(fmaf(d, d, eps2));   // hardware reciprocal (1 ulp)
         return cmul(v, make_float2(d * s, eps * s));
       };

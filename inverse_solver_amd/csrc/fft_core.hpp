@@ -343,10 +343,37 @@ __device__ __forceinline__ int group_of(int t) {
   else return t / NL;
 }
 
+// Line groups of 32 lines in 64-lane waves (NT a multiple of 64): a wave holds group g0
+// in lanes 0-31 and g0 + 1 in lanes 32-63, g0 wave-uniform, so a per-group constant comes
+// from two scalar loads and a select instead of one vector load per lane.
+template <int NL, int NT>
+constexpr bool half_wave_groups() { return NL == 32 && NT % 64 == 0; }
+
+// Group of task t: g0 (wave-uniform when the groups allow) plus hi.
+template <int NL, int NT>
+__device__ __forceinline__ int group_pair(int t, bool& hi) {
+  if constexpr (half_wave_groups<NL, NT>()) {
+    hi = (t & 32) != 0;
+    return __builtin_amdgcn_readfirstlane(t >> 6) * 2;
+  } else {
+    hi = false;
+    return group_of<NL, NT>(t);
+  }
+}
+
+// a[hi ? i1 : i0] with both indices loaded (uniform indices become scalar loads)
+template <class T>
+__device__ __forceinline__ T pick2(const T* __restrict__ a, int i0, int i1, bool hi) {
+  const T x = a[i0];
+  const T y = a[i1];
+  return hi ? y : x;
+}
+
 // Forward FFT, pointwise multiply, inverse FFT on NL lines in LDS, with the
 // last forward step, the multiply and the first inverse step fused in
-// registers (they act on the same R2 values of one task).  mul(l, p, v)
-// returns v times the factor for line l at position p (forward output order).
+// registers (they act on the same R2 values of one task).  mul(l, p0, p1, hi, v)
+// returns v times the factor for line l at position p = hi ? p1 : p0 (forward output
+// order; p0 and p1 are wave-uniform when the line groups allow).
 // Result: the circular convolution, unnormalised (N times the usual inverse).
 template <int N, int NL, int NT, class Addr, class Mul>
 __device__ __forceinline__ void line_fft_mul_ifft(float2* __restrict__ lds, const Addr& addr,
@@ -360,7 +387,7 @@ __device__ __forceinline__ void line_fft_mul_ifft(float2* __restrict__ lds, cons
       for (int e = 0; e < N; ++e) v[e] = lds[addr(l, e)];
       Dft<N, false>::run(v);
 #pragma unroll
-      for (int e = 0; e < N; ++e) v[e] = mul(l, e, v[e]);
+      for (int e = 0; e < N; ++e) v[e] = mul(l, e, e, false, v[e]);
       Dft<N, true>::run(v);
 #pragma unroll
       for (int e = 0; e < N; ++e) lds[addr(l, e)] = v[e];
@@ -368,37 +395,40 @@ __device__ __forceinline__ void line_fft_mul_ifft(float2* __restrict__ lds, cons
     __syncthreads();
   } else {
     for (int t = tid; t < NL * R2; t += NT) {           // forward step A
-      const int l = t % NL, n2 = group_of<NL, NT>(t);
+      bool hi;
+      const int l = t % NL, g0 = group_pair<NL, NT>(t, hi), n2 = g0 + hi;
       float2 v[R1];
 #pragma unroll
       for (int n1 = 0; n1 < R1; ++n1) v[n1] = lds[addr(l, R2 * n1 + n2)];
       Dft<R1, false>::run(v);
 #pragma unroll
-      for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], tw[n2 * k1]);
+      for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], pick2(tw, g0 * k1, (g0 + 1) * k1, hi));
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1) lds[addr(l, R2 * k1 + n2)] = v[k1];
     }
     __syncthreads();
     for (int t = tid; t < NL * R1; t += NT) {           // step B, multiply, inverse step B
-      const int l = t % NL, k1 = group_of<NL, NT>(t);
+      bool hi;
+      const int l = t % NL, g0 = group_pair<NL, NT>(t, hi), k1 = g0 + hi;
       float2 v[R2];
 #pragma unroll
       for (int n2 = 0; n2 < R2; ++n2) v[n2] = lds[addr(l, R2 * k1 + n2)];
       Dft<R2, false>::run(v);
 #pragma unroll
-      for (int k2 = 0; k2 < R2; ++k2) v[k2] = mul(l, R2 * k1 + k2, v[k2]);
+      for (int k2 = 0; k2 < R2; ++k2) v[k2] = mul(l, R2 * g0 + k2, R2 * (g0 + 1) + k2, hi, v[k2]);
       Dft<R2, true>::run(v);
 #pragma unroll
       for (int n2 = 0; n2 < R2; ++n2) lds[addr(l, R2 * k1 + n2)] = v[n2];
     }
     __syncthreads();
     for (int t = tid; t < NL * R2; t += NT) {           // inverse step A
-      const int l = t % NL, n2 = group_of<NL, NT>(t);
+      bool hi;
+      const int l = t % NL, g0 = group_pair<NL, NT>(t, hi), n2 = g0 + hi;
       float2 v[R1];
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1) v[k1] = lds[addr(l, R2 * k1 + n2)];
 #pragma unroll
-      for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmulc(v[k1], tw[n2 * k1]);
+      for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmulc(v[k1], pick2(tw, g0 * k1, (g0 + 1) * k1, hi));
       Dft<R1, true>::run(v);
 #pragma unroll
       for (int n1 = 0; n1 < R1; ++n1) lds[addr(l, R2 * n1 + n2)] = v[n1];
