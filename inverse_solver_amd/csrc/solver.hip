@@ -425,7 +425,7 @@ struct Solver {
     p.rt[2] = !p.sz;
     const int dims[3] = {nx, ny, nz};
     for (int j = 0; j < 3; ++j)
-      if (p.rt[j]) build_rt(rtp[j], dims[j]);
+      if (p.rt[j] && rtp[j].k.n != dims[j]) build_rt(rtp[j], dims[j]);   // once per handle
     p.a = ny;
     p.b = 1;
     p.t = p.sy ? p.sy->t : 0;
@@ -484,8 +484,20 @@ struct Solver {
   }
 
   // ---- FP64 tables for the active plan
+  // what build_tables() reads: the plan and the potential's geometry scalars
+  std::vector<double> tab_key;
+  std::vector<double> table_key() const {
+    std::vector<double> k = {(double)plan.form, (double)plan.a, (double)plan.b, (double)plan.t,
+                             (double)(uintptr_t)plan.pr, (double)(uintptr_t)plan.pk, (double)(uintptr_t)plan.sy,
+                             (double)plan.rt[0], (double)plan.rt[1], (double)plan.rt[2], prm.k0_nm,
+                             prm.fourier_res[0], prm.fourier_res[1], prm.fourier_res[2], prm.boundary_sharpness,
+                             (double)prm.acyclic, (double)prm.shift_axes, (double)prm.ramp_axes};
+    for (int j = 0; j < 6; ++j) k.push_back((double)prm.ROI[j]);
+    return k;
+  }
   void build_tables() {
     drop_graph();   // captured loops hold the table addresses, which may move here
+    tab_key.clear();
     const int a = plan.a, b = plan.b;
     TableSet& T = tabs;
     T.a = a; T.b = b;
@@ -592,6 +604,7 @@ struct Solver {
       T.rad_y.upload(qy);
       T.rad_z.upload(qz);
     }
+    tab_key = table_key();
   }
 
   // ---- kernel timing
@@ -1590,7 +1603,10 @@ struct Solver {
 
   void finish_potential() {
     make_plan();
-    build_tables();
+    // the tables depend on the plan and on the geometry scalars, not on V or eps: a new
+    // potential on the same geometry (every set_RI of a reconstruction or a serving loop)
+    // keeps them
+    if (table_key() != tab_key) build_tables();
     HIP_CHECK(hipStreamSynchronize(stream));
     flip_state = 0;
     have_potential = true;

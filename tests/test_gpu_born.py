@@ -239,3 +239,27 @@ def test_host_buffer_pipeline_equals_device_path():
         assert rel(got3[..., a], refs[a]) < TOL, a
     host.close()
     dev.close()
+
+
+@pytest.mark.parametrize('form', [0, 1, 2])
+def test_repeat_set_ri_on_one_handle(form):
+    """A new potential on the same geometry keeps the handle's tables (they depend on the
+    plan and the geometry scalars only); solves after each set_RI equal fresh handles', and
+    a changed boundary sharpness rebuilds them."""
+    ri = (16, 16, 16)
+    p = params(ri, iterations=9)
+    RIs = [sphere_ri(ri, 0.5, n_s=1.40), sphere_ri(ri, 0.35, n_s=1.46, center=(6, 9, 8))]
+    u = plane_wave(ri, 0.1, 0.2)
+    s = _gpu()(p, options=dict(form=form))
+    for k in range(3):
+        RI = RIs[k % 2]
+        if k == 2:
+            s.parameters['boundary_sharpness'] = 0.5
+        s.set_RI(RI)
+        got = s.solve_incident(u)
+        pf = dict(p, boundary_sharpness=0.5) if k == 2 else p
+        f = _gpu()(pf, options=dict(form=form))
+        f.set_RI(RI)
+        np.testing.assert_array_equal(got, f.solve_incident(u))
+        f.close()
+    s.close()
