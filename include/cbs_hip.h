@@ -86,7 +86,9 @@ typedef struct {
   int32_t form;         /* CBS_FORM_FUSED (two passes per iteration; grids without such a plan fall
                            back to CBS_FORM_ROWS, then CBS_FORM_PLAIN), CBS_FORM_ROWS (four passes:
                            row blocks in real space, y transforms, z transforms with the Green's
-                           multiply) or CBS_FORM_PLAIN (per-axis transforms, one rhs at a time) */
+                           multiply) or CBS_FORM_PLAIN (per-axis transforms, 8 launches per
+                           iteration for the whole batch; grids whose line lengths have no compiled
+                           kernel run it on run-time-planned line FFTs) */
   int32_t split_a;      /* fused form: force ny = a*b split (0 = planner's choice) */
   int32_t tile_t;       /* fused form: force the k-space x tile (0 = planner's choice) */
   int32_t max_batch;    /* right-hand sides per launch (0 = all of the call) */

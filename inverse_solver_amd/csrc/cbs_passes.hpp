@@ -631,11 +631,12 @@ __global__ __launch_bounds__(NT) void fft_rows(float2* data, long long nrows, co
 
 // FFT along a strided axis of length N (stride `es` elements).  A block takes
 // T consecutive x positions of one line set: element e of line t at
-// base + t + e*es.  Blocks enumerate (x tile, outer index) with
-// base = outer * os + xt * T.
+// base + t + e*es.  Blocks enumerate (x tile, outer index o < nouter) with
+// base = (o / inner) * os2 + (o % inner) * os + xt * T (two outer levels: e.g. the
+// y positions of a z transform and the right-hand sides).
 template <int N, int T, int NT>
 __global__ __launch_bounds__(NT) void fft_strided(float2* data, int nx, long long es, long long os, int nouter,
-                                                  const float2* tw, int inverse) {
+                                                  const float2* tw, int inverse, int inner, long long os2) {
   __shared__ float2 lds[N * T];
   __shared__ float2 s_tw[N];   // twiddles: a task's R1 - 1 of them differ from lane to lane
   for (int i = threadIdx.x; i < N; i += NT) s_tw[i] = tw[i];
@@ -643,11 +644,11 @@ __global__ __launch_bounds__(NT) void fft_strided(float2* data, int nx, long lon
   const int xt = blockIdx.x % ntile;
   const int o = blockIdx.x / ntile;
   const int tvalid = nx - xt * T < T ? nx - xt * T : T;
-  float2* base = data + (long long)o * os + (long long)xt * T;
+  float2* base = data + (long long)(o / inner) * os2 + (long long)(o % inner) * os + (long long)xt * T;
   auto ad = [](int l, int e) { return e * T + l; };
   // even x extent (and so even strides and tile origins): 16-byte pair accesses,
   // all of a thread's loads issued before the LDS writes
-  const bool pairs = (T % 2 == 0) && (nx % 2 == 0) && (es % 2 == 0) && (os % 2 == 0);
+  const bool pairs = (T % 2 == 0) && (nx % 2 == 0) && (es % 2 == 0) && (os % 2 == 0) && (os2 % 2 == 0);
   constexpr int TP = T / 2;
   constexpr int LP = (N * TP + NT - 1) / NT;
   if (pairs) {

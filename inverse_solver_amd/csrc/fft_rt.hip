@@ -114,7 +114,7 @@ __global__ void fft_rows_rt(float2* data, long long nrows, RtFft p, int lines_pe
 }
 
 __global__ void fft_strided_rt(float2* data, int nx, long long es, long long os, int nouter, RtFft p, int T,
-                               int inverse) {
+                               int inverse, int inner, long long os2) {
   extern __shared__ float2 rt_lds[];
   const int L = p.L, n = p.n;
   float2* a = rt_lds;
@@ -124,7 +124,7 @@ __global__ void fft_strided_rt(float2* data, int nx, long long es, long long os,
   const int o = blockIdx.x / ntile;
   if (o >= nouter) return;
   const int tv = nx - xt * T < T ? nx - xt * T : T;
-  float2* base = data + (long long)o * os + (long long)xt * T;
+  float2* base = data + (long long)(o / inner) * os2 + (long long)(o % inner) * os + (long long)xt * T;
   for (int i = threadIdx.x; i < n * tv; i += blockDim.x) {
     const int e = i / tv, t = i - e * tv;
     a[t * L + e] = base[e * es + t];

@@ -38,9 +38,10 @@ __host__ __device__ inline int rt_lines(int L, int cap) {
 __global__ void fft_rows_rt(float2* data, long long nrows, RtFft p, int lines_per_block, int inverse);
 
 // Strided lines: element e of line t in the block's tile at base + t + e * es, the tile
-// being T consecutive x positions (x tile xt of ceil(nx / T)) of outer set o:
-// base = o * os + xt * T.  T = rt_lines(p.L, 16); dynamic LDS = 2 * T * p.L * 8 bytes.
+// being T consecutive x positions (x tile xt of ceil(nx / T)) of outer set o < nouter:
+// base = (o / inner) * os2 + (o % inner) * os + xt * T.  T = rt_lines(p.L, 16); dynamic
+// LDS = 2 * T * p.L * 8 bytes.
 __global__ void fft_strided_rt(float2* data, int nx, long long es, long long os, int nouter, RtFft p, int T,
-                               int inverse);
+                               int inverse, int inner, long long os2);
 
 }  // namespace cbs

@@ -10,8 +10,14 @@ namespace cbs {
 
 // Plain form: spectrum in natural [z][y][x] layout (each axis in LineFft
 // position order).  Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:609-613 (Greenp .* fftn)
-// with the 1/N of the following ifftn folded in.
-__global__ void green_plain(float2* __restrict__ spec, Tables tb, Consts c, int flip) {
+// with the 1/N of the following ifftn folded in.  blockIdx.y is the rhs; its flip
+// state of iteration jj follows from its parity; a stopped rhs (cv) is skipped.
+__global__ void green_plain(float2* __restrict__ spec, Tables tb, Consts c, const uint8_t* __restrict__ parity, int jj,
+                            ConvArgs cv) {
+  const int r = blockIdx.y;
+  if (cv.norms && conv_stopped(cv, jj - 1, r)) return;
+  const int flip = flip_of(jj, parity, r);
+  spec += (long long)r * c.nvox;
   GRID_STRIDE(i, c.nvox) {
     const int x = (int)(i % c.nx);
     const long long yz = i / c.nx;
@@ -24,8 +30,13 @@ __global__ void green_plain(float2* __restrict__ spec, Tables tb, Consts c, int 
 }
 
 // Plain form, vector runs: the dyadic multiply G (I - c r r^T) on the three
-// component spectra of one rhs (comp_stride elements apart); :607-615.
-__global__ void green_plain_vec(float2* __restrict__ spec, long long comp_stride, Tables tb, Consts c, int flip) {
+// component spectra of one rhs (comp_stride elements apart; rhs blockIdx.y); :607-615.
+__global__ void green_plain_vec(float2* __restrict__ spec, long long comp_stride, Tables tb, Consts c,
+                                const uint8_t* __restrict__ parity, int jj, ConvArgs cv) {
+  const int rr = blockIdx.y;
+  if (cv.norms && conv_stopped(cv, jj - 1, rr)) return;
+  const int flip = flip_of(jj, parity, 3 * rr);
+  spec += (long long)3 * rr * comp_stride;
   const float2 cc = make_float2(c.c_re, c.c_im);
   GRID_STRIDE(i, c.nvox) {
     const int x = (int)(i % c.nx);
@@ -47,10 +58,11 @@ __global__ void green_plain_vec(float2* __restrict__ spec, long long comp_stride
 
 // Plain form: the pointwise part of one iteration on natural-layout arrays
 // (Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:595-642).
-// cv: the convergence test of pass_real (a stopped rhs is skipped; the first skipped
-// iteration after a deferred one adds the deferred increment to Field).
-__global__ void update_plain(StateArgs st, Tables tb, Consts c, int upd, int fwd, int jj_inv, int jj_fwd, int rhs,
-                             ConvArgs cv) {
+// Component slot blockIdx.y.  cv: the convergence test of pass_real (a stopped rhs is
+// skipped; the first skipped iteration after a deferred one adds the deferred increment
+// to Field).
+__global__ void update_plain(StateArgs st, Tables tb, Consts c, int upd, int fwd, int jj_inv, int jj_fwd, ConvArgs cv) {
+  const int rhs = blockIdx.y;
   double nrm = 0.0, nrmF = 0.0;
   const long long ro = (long long)rhs * st.rhs_stride;
   float2* spec = st.spec + ro;

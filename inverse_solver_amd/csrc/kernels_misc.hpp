@@ -9,10 +9,11 @@ struct RoiGeom {
   long long nvox;
 };
 
-__global__ void green_plain(float2* __restrict__ spec, Tables tb, Consts c, int flip);
-__global__ void green_plain_vec(float2* __restrict__ spec, long long comp_stride, Tables tb, Consts c, int flip);
-__global__ void update_plain(StateArgs st, Tables tb, Consts c, int upd, int fwd, int jj_inv, int jj_fwd, int rhs,
-                             ConvArgs cv);
+__global__ void green_plain(float2* __restrict__ spec, Tables tb, Consts c, const uint8_t* __restrict__ parity, int jj,
+                            ConvArgs cv);
+__global__ void green_plain_vec(float2* __restrict__ spec, long long comp_stride, Tables tb, Consts c,
+                                const uint8_t* __restrict__ parity, int jj, ConvArgs cv);
+__global__ void update_plain(StateArgs st, Tables tb, Consts c, int upd, int fwd, int jj_inv, int jj_fwd, ConvArgs cv);
 __global__ void fill_parity(uint8_t* __restrict__ parity, int nslots, int P, int flip_state, int rhs0, int M);
 __global__ void zero_words(uint32_t* __restrict__ p, long long n);
 __global__ void seed_source(const float2* __restrict__ u_in, const float2* __restrict__ V, float2* __restrict__ fstate,
@@ -36,7 +37,7 @@ __global__ void potential_fill(RiGrid g, int flip_conj, RoiGeom rg, int snx, int
 using PassRFn = void (*)(struct PassRArgs);
 using PassKFn = void (*)(struct PassKArgs);
 using RowsFn = void (*)(float2*, long long, const float2*, int);
-using StridedFn = void (*)(float2*, int, long long, long long, int, const float2*, int);
+using StridedFn = void (*)(float2*, int, long long, long long, int, const float2*, int, int, long long);
 
 // fn takes every update kind; fn_steady only UPD_DEFER / UPD_PAIR with FWD_GAMMA
 struct PassRInfo { PassRFn fn, fn_steady; int nx, a, nt, lds, rows; };

@@ -305,14 +305,16 @@ struct Solver {
     timed(2, [&] { hipLaunchKernelGGL(fft_rows_rt, dim3((unsigned)nblk), dim3(256), lds, stream, data, nrows, P.k,
                                       lines, inverse); });
   }
+  // outer set o < nouter at (o / inner) * os2 + (o % inner) * os (inner = 0: one level)
   void rt_strided(const RtPlan& P, float2* data, int ncols, long long es, long long os, long long nouter,
-                  int inverse) {
+                  int inverse, long long inner = 0, long long os2 = 0) {
     const int T = rt_lines(P.k.L, 16);
     const long long nblk = (long long)((ncols + T - 1) / T) * nouter;
     require(nblk < (1ll << 31), CBS_ERR_UNSUPPORTED, "run-time FFT batch too large for one launch");
     const size_t lds = 2ull * T * P.k.L * sizeof(float2);
+    const int in = (int)(inner > 0 ? inner : nouter);
     timed(2, [&] { hipLaunchKernelGGL(fft_strided_rt, dim3((unsigned)nblk), dim3(256), lds, stream, data, ncols,
-                                      es, os, (int)nouter, P.k, T, inverse); });
+                                      es, os, (int)nouter, P.k, T, inverse, in, os2); });
   }
 
   void drop_graph() {
@@ -758,7 +760,7 @@ struct Solver {
       const long long nouter = (long long)nz * nv;
       const dim3 g((unsigned)(nt * nouter));
       timed(2, [&] { hipLaunchKernelGGL(sy->fn, g, dim3(sy->nt), 0, stream, ws.spec, nx, (long long)nx,
-                                        (long long)nx * ny, (int)nouter, tabs.tw_y.p, inverse); });
+                                        (long long)nx * ny, (int)nouter, tabs.tw_y.p, inverse, (int)nouter, 0ll); });
     };
     pass_r(UPD_NONE, FWD_P0, 0, 1);
     for (int jj = 1; jj <= M; ++jj) {
@@ -771,65 +773,72 @@ struct Solver {
     HIP_CHECK(hipGetLastError());
   }
 
-  // plain schedule for rhs r of a batch of nb (state arrays indexed by component slot P r + q)
-  void run_plain(int r, int nb, int M) {
+  // per-axis schedule for a batch of nb rhs (nv = nb P component slots): every launch covers
+  // the whole batch (the rhs or slot on the grid's y dimension, or as an outer index of the
+  // line transforms)
+  void run_plain(int nb, int M) {
+    const int nv = nb * P;
     const Consts c = consts(1);
     const Tables tb = tabs.view();
-    const int nv = nb * P;
-    const unsigned pw_grid = grid_for(nvox);
+    const unsigned pw_grid = std::min<unsigned>(grid_for(nvox), 65535u);
     const ConvArgs cv = conv_args(nv, M);
     auto update = [&](int upd, int fwd, int jj_inv, int jj_fwd) {
       StateArgs s = state_args(1);
       s.norms = norm_row(jj_inv, nv);
       if (conv_tol > 0.0 && upd == UPD_VQ_ADD) s.fref = ws.fref;
-      for (int q = 0; q < P; ++q)
-        timed(2, [&] { hipLaunchKernelGGL(update_plain, dim3(pw_grid), dim3(256), 0, stream, s, tb, c, upd, fwd,
-                                          jj_inv, jj_fwd, P * r + q, cv); });
+      timed(2, [&] { hipLaunchKernelGGL(update_plain, dim3(pw_grid, (unsigned)nv), dim3(256), 0, stream, s, tb, c, upd,
+                                        fwd, jj_inv, jj_fwd, cv); });
     };
-    auto fft3 = [&](float2* sp, int inverse) {
+    auto fft3 = [&](int inverse) {
+      float2* sp = ws.spec;
       const RowsInfo* rw = plan.rw;
       const StridedInfo* sy = plan.sy;
       const StridedInfo* sz = plan.sz;
       auto fx = [&] {
-        const long long nrows = (long long)ny * nz;
+        const long long nrows = (long long)ny * nz * nv;
         if (plan.rt[0]) return rt_rows(rtp[0], sp, nrows, inverse);
-        const dim3 g((unsigned)((nrows + rw->rows - 1) / rw->rows));
-        timed(2, [&] { hipLaunchKernelGGL(rw->fn, g, dim3(rw->nt), 0, stream, sp, nrows, tabs.tw_x.p, inverse); });
+        const long long nblk = (nrows + rw->rows - 1) / rw->rows;
+        require(nblk < (1ll << 31), CBS_ERR_UNSUPPORTED, "x transform batch too large for one launch");
+        timed(2, [&] { hipLaunchKernelGGL(rw->fn, dim3((unsigned)nblk), dim3(rw->nt), 0, stream, sp, nrows,
+                                          tabs.tw_x.p, inverse); });
       };
-      auto fy = [&] {
-        if (plan.rt[1]) return rt_strided(rtp[1], sp, nx, (long long)nx, (long long)nx * ny, nz, inverse);
-        const int nt = (nx + sy->t - 1) / sy->t;
-        const dim3 g((unsigned)(nt * nz));
-        timed(2, [&] { hipLaunchKernelGGL(sy->fn, g, dim3(sy->nt), 0, stream, sp, nx, (long long)nx,
-                                          (long long)nx * ny, nz, tabs.tw_a.p, inverse); });
+      auto fy = [&] {   // outer: the z planes of every slot, nx * ny apart
+        const long long nouter = (long long)nz * nv;
+        if (plan.rt[1]) return rt_strided(rtp[1], sp, nx, (long long)nx, (long long)nx * ny, nouter, inverse);
+        const long long nblk = (long long)((nx + sy->t - 1) / sy->t) * nouter;
+        require(nblk < (1ll << 31), CBS_ERR_UNSUPPORTED, "y transform batch too large for one launch");
+        timed(2, [&] { hipLaunchKernelGGL(sy->fn, dim3((unsigned)nblk), dim3(sy->nt), 0, stream, sp, nx,
+                                          (long long)nx, (long long)nx * ny, (int)nouter, tabs.tw_a.p, inverse,
+                                          (int)nouter, 0ll); });
       };
-      auto fz = [&] {
-        if (plan.rt[2]) return rt_strided(rtp[2], sp, nx, (long long)nx * ny, (long long)nx, ny, inverse);
-        const int nt = (nx + sz->t - 1) / sz->t;
-        const dim3 g((unsigned)(nt * ny));
-        timed(2, [&] { hipLaunchKernelGGL(sz->fn, g, dim3(sz->nt), 0, stream, sp, nx, (long long)nx * ny,
-                                          (long long)nx, ny, tabs.tw_z.p, inverse); });
+      auto fz = [&] {   // outer: y (nx apart) within a slot, then the slots (nvox apart)
+        const long long nouter = (long long)ny * nv;
+        if (plan.rt[2])
+          return rt_strided(rtp[2], sp, nx, (long long)nx * ny, (long long)nx, nouter, inverse, ny, nvox);
+        const long long nblk = (long long)((nx + sz->t - 1) / sz->t) * nouter;
+        require(nblk < (1ll << 31), CBS_ERR_UNSUPPORTED, "z transform batch too large for one launch");
+        timed(2, [&] { hipLaunchKernelGGL(sz->fn, dim3((unsigned)nblk), dim3(sz->nt), 0, stream, sp, nx,
+                                          (long long)nx * ny, (long long)nx, (int)nouter, tabs.tw_z.p, inverse, ny,
+                                          (long long)nvox); });
       };
       if (!inverse) { fx(); fy(); fz(); }
       else { fz(); fy(); fx(); }
     };
-    float2* sp0 = ws.spec + (long long)(P * r) * nvox;
     update(UPD_NONE, FWD_P0, 0, 1);
     for (int jj = 1; jj <= M; ++jj) {
-      for (int q = 0; q < P; ++q) fft3(sp0 + (long long)q * nvox, 0);
-      const int flip = (jj + (int)host_parity[P * r]) & 1;
+      fft3(0);
       if (P == 1)
-        timed(2, [&] { hipLaunchKernelGGL(green_plain, dim3(pw_grid), dim3(256), 0, stream, sp0, tb, c, flip); });
+        timed(2, [&] { hipLaunchKernelGGL(green_plain, dim3(pw_grid, (unsigned)nb), dim3(256), 0, stream, ws.spec, tb,
+                                          c, ws.parity, jj, cv); });
       else
-        timed(2, [&] { hipLaunchKernelGGL(green_plain_vec, dim3(pw_grid), dim3(256), 0, stream, sp0, nvox, tb, c,
-                                          flip); });
-      for (int q = 0; q < P; ++q) fft3(sp0 + (long long)q * nvox, 1);
+        timed(2, [&] { hipLaunchKernelGGL(green_plain_vec, dim3(pw_grid, (unsigned)nb), dim3(256), 0, stream, ws.spec,
+                                          nvox, tb, c, ws.parity, jj, cv); });
+      fft3(1);
       const int fwd = fwd_of(jj, M);
       update(upd_of(jj, M), fwd, jj, fwd == FWD_NONE ? 0 : jj + 1);
     }
     HIP_CHECK(hipGetLastError());
   }
-  std::vector<uint8_t> host_parity;
 
   // the handle's own state buffers for batches of nb rhs, and ws pointing at them
   void ensure_workspace(int nb, bool host_io, int host_bufs = 1) {
@@ -862,7 +871,6 @@ struct Solver {
     return b;
   }
   int batch_width(uint32_t n_rhs) const {
-    if (opt.form == CBS_FORM_PLAIN || plan.form == CBS_FORM_PLAIN) return 1 > (int)n_rhs ? (int)n_rhs : 1;
     int nb = (int)n_rhs;
     if (opt.max_batch > 0 && opt.max_batch < nb) nb = opt.max_batch;
     return nb;
@@ -878,9 +886,6 @@ struct Solver {
     if (last_iters.size() < (size_t)(rhs_index0 + nb)) last_iters.resize((size_t)(rhs_index0 + nb), 0u);
     if (tracking() && last_norms.size() < (size_t)(rhs_index0 + nb) * M)
       last_norms.resize((size_t)(rhs_index0 + nb) * M, 0.0);
-    host_parity.assign((size_t)nv, 0);
-    for (int v = 0; v < nv; ++v)
-      host_parity[v] = (uint8_t)((flip_state + (long long)(rhs_index0 + v / P) * M) & 1);
     // the device copy is formed in stream order, after the previous batch's kernels
     hipLaunchKernelGGL(fill_parity, dim3(grid_for(nv)), dim3(256), 0, stream, ws.parity, nv, P, flip_state,
                        rhs_index0, M);
@@ -903,14 +908,14 @@ struct Solver {
       const long long words = 2ll * nv * nvox;
       hipLaunchKernelGGL(zero_words, dim3(grid_for(words)), dim3(256), 0, stream,
                          reinterpret_cast<uint32_t*>(ws.field), words);
-    } else if (plan.form == CBS_FORM_FUSED || plan.form == CBS_FORM_ROWS) {
+    } else {
       if (opt.use_graph && !opt.time_kernels && !no_alloc) {
         // replay a captured iteration loop (the kernel arguments, including the
         // potential's scalars, are baked in; the cache is dropped whenever the
         // potential, the options or the plan change)
         const std::vector<long long> key = {nb, M, (long long)ws.field, (long long)ws.fstate, (long long)ws.spec,
                                             (long long)ws.parity, (long long)ws.norms, (long long)ws.fref,
-                                            (long long)(conv_tol > 0.0),
+                                            (long long)(conv_tol > 0.0), (long long)plan.form,
                                             (long long)plan.pr, (long long)plan.pk, (long long)opt.block_order};
         GraphSlot* gs = nullptr;
         for (auto& g : gslot)
@@ -922,8 +927,7 @@ struct Solver {
           if (gs->graph) (void)hipGraphDestroy(gs->graph);
           *gs = GraphSlot();
           HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-          if (plan.form == CBS_FORM_ROWS) run_rows(nb, M);
-          else run_fused(nb, M);
+          run_loop(nb, M);
           hipGraph_t captured = nullptr;
           hipError_t e = hipStreamEndCapture(stream, &captured);
           if (e != hipSuccess) throw CbsError(CBS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -932,13 +936,9 @@ struct Solver {
           gs->key = key;
         }
         HIP_CHECK(hipGraphLaunch(gs->exec, stream));
-      } else if (plan.form == CBS_FORM_ROWS) {
-        run_rows(nb, M);
       } else {
-        run_fused(nb, M);
+        run_loop(nb, M);
       }
-    } else {
-      for (int r = 0; r < nb; ++r) run_plain(r, nb, M);
     }
     {
       const long long tot = (long long)nv * nri;
@@ -1122,6 +1122,12 @@ struct Solver {
     if (prm.acyclic) flip_state = (int)((flip_state + (long long)n_rhs * M) & 1);
   }
 
+  void run_loop(int nb, int M) {
+    if (plan.form == CBS_FORM_ROWS) run_rows(nb, M);
+    else if (plan.form == CBS_FORM_PLAIN) run_plain(nb, M);
+    else run_fused(nb, M);
+  }
+
   // per-solve records: norms (when tracked) and the iterations each rhs ran
   void begin_solve(uint32_t n_rhs, int M) {
     last_iters.assign((size_t)n_rhs, (uint32_t)M);
@@ -1237,7 +1243,7 @@ struct Solver {
       require((long long)nt * nslices < (1ll << 31), CBS_ERR_UNSUPPORTED, "2D FFT batch too large for one launch");
       const dim3 g((unsigned)(nt * nslices));
       timed(2, [&] { hipLaunchKernelGGL(st->fn, g, dim3(st->nt), 0, stream, buf, cx, (long long)cx,
-                                        (long long)cx * cy, (int)nslices, io_tw_cy.p, inverse); });
+                                        (long long)cx * cy, (int)nslices, io_tw_cy.p, inverse, (int)nslices, 0ll); });
     };
     if (!inverse) { fx_(); fy_(); }
     else { fy_(); fx_(); }

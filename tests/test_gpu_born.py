@@ -93,13 +93,14 @@ def test_block_orders_give_identical_results():
         np.testing.assert_array_equal(o, outs[0])
 
 
-def test_batch_equals_individual_and_oracle():
+@pytest.mark.parametrize('form', [0, 1, 2])
+def test_batch_equals_individual_and_oracle(form):
     ri = (16, 16, 16)
     p = params(ri, iterations=16)
     RI = sphere_ri(ri, 0.6, n_s=1.40)
     fields = [random_field(ri, s) for s in range(3)]
     ref, _ = _oracle_loop(p, RI, fields)
-    s = _gpu()(p)
+    s = _gpu()(p, options=dict(form=form))
     s.set_RI(RI)
     batch = s.solve_incident(np.stack(fields, -1))
     for j in range(3):
@@ -110,7 +111,8 @@ def test_batch_equals_individual_and_oracle():
     s.close()
 
 
-def test_odd_bornmax_alternation_persists_across_rhs():
+@pytest.mark.parametrize('form', [0, 1, 2])
+def test_odd_bornmax_alternation_persists_across_rhs(form):
     """With odd Bornmax the reference leaves Greenp flipped after each angle
     (FORWARD_SOLVER_CONVERGENT_BORN.m:579-585), so consecutive angles start on
     opposite shifts.  The oracle keeps that state; the GPU must match it."""
@@ -119,7 +121,7 @@ def test_odd_bornmax_alternation_persists_across_rhs():
     RI = sphere_ri(ri, 0.6)
     fields = [random_field(ri, 10 + s) for s in range(3)]
     ref, _ = _oracle_loop(p, RI, fields)
-    s = _gpu()(p)
+    s = _gpu()(p, options=dict(form=form))
     s.set_RI(RI)
     got = s.solve_incident(np.stack(fields, -1))
     for j in range(3):
