@@ -44,4 +44,26 @@ __global__ void fft_rows_rt(float2* data, long long nrows, RtFft p, int lines_pe
 __global__ void fft_strided_rt(float2* data, int nx, long long es, long long os, int nouter, RtFft p, int T,
                                int inverse, int inner, long long os2);
 
+// Bluestein's transform on a compiled convolution length L >= 2n - 1 (fft_blue.hip):
+// the chirp products are fused into the tile load and store, and the convolution is
+// one compiled forward FFT, multiply and inverse FFT of length L (line_fft_mul_ifft),
+// so no stage runs on run-time radices.  The inverse is conj(DFT(conj x)).
+struct BlueArgs {
+  int n;                    // line length
+  const float2* chirp;      // [n] c_k = exp(i pi k^2 / n)
+  const float2* bpos;       // [L] DFT_L of the chirp filter over L, in the forward output order of L
+  const float2* tw;         // [L] exp(-2 pi i e / L)
+};
+// rows: NL contiguous lines of n per block; strided: as fft_strided_rt with T = t lines
+using BlueRowsFn = void (*)(float2*, long long, BlueArgs, int);
+using BlueStridedFn = void (*)(float2*, int, long long, long long, int, BlueArgs, int, int, long long);
+struct BlueInfo {
+  BlueRowsFn rows;
+  BlueStridedFn strided;
+  int L, nl, t, nt;
+  int r2;                   // forward output position p holds frequency p / r2 + (L / r2) (p % r2)
+};
+// the compiled length L >= m with the fewest points (null when m is above every one)
+const BlueInfo* find_blue(int m);
+
 }  // namespace cbs

@@ -121,6 +121,8 @@ struct Plan {
 struct RtPlan {
   RtFft k{};
   DevBuf<float2> tw, chirp, bspec;
+  const BlueInfo* bi = nullptr;   // Bluestein on a compiled convolution length (else k)
+  BlueArgs ba{};
 };
 
 // Stage radices of a Stockham transform of length L (16, 8, 4, 2 for the power of two,
@@ -148,14 +150,18 @@ static int smooth_at_least(int m) {
 
 // Plan of the length-n line transform: Stockham when n factors into the radices,
 // else Bluestein with the chirp c_k = exp(i pi k^2 / n) and the DFT of the chirp
-// filter over a length L >= 2n - 1 (FP64 on the host, rounded to complex64).
+// filter over a length L >= 2n - 1 (FP64 on the host, rounded to complex64) -- on the
+// shortest compiled convolution length when one is long enough (fft_blue.hip), else
+// on a run-time Stockham transform.
 static void build_rt(RtPlan& P, int n) {
   require(n >= 1, CBS_ERR_ARG, "line length must be >= 1");
   std::vector<int> rs;
   int L = n, blue = 0;
+  P.bi = nullptr;
   if (!rt_factor(n, rs)) {
     blue = 1;
-    L = smooth_at_least(2 * n - 1);
+    P.bi = find_blue(2 * n - 1);
+    L = P.bi ? P.bi->L : smooth_at_least(2 * n - 1);
     rt_factor(L, rs);
   }
   require(2 * L <= kRtLdsElems && (int)rs.size() <= kRtMaxStages, CBS_ERR_UNSUPPORTED,
@@ -195,10 +201,19 @@ static void build_rt(RtPlan& P, int n) {
       acc /= (double)L;
       bf[k] = make_float2((float)acc.real(), (float)acc.imag());
     }
+    if (P.bi) {
+      // the compiled convolution multiplies position p of its forward output, which
+      // holds frequency p / r2 + (L / r2)(p % r2)
+      const int r2 = P.bi->r2, r1 = L / r2;
+      std::vector<float2> bp(L);
+      for (int q = 0; q < L; ++q) bp[q] = bf[r2 == 1 ? q : q / r2 + r1 * (q % r2)];
+      bf.swap(bp);
+    }
     P.chirp.upload(cf);
     P.bspec.upload(bf);
     P.k.chirp = P.chirp.p;
     P.k.bspec = P.bspec.p;
+    P.ba = BlueArgs{n, P.chirp.p, P.bspec.p, P.tw.p};
   }
 }
 
@@ -298,6 +313,13 @@ struct Solver {
   RtPlan rtp[3], io_rtx, io_rty;
 
   void rt_rows(const RtPlan& P, float2* data, long long nrows, int inverse) {
+    if (P.bi) {
+      const long long nblk = (nrows + P.bi->nl - 1) / P.bi->nl;
+      require(nblk < (1ll << 31), CBS_ERR_UNSUPPORTED, "run-time FFT batch too large for one launch");
+      timed(2, [&] { hipLaunchKernelGGL(P.bi->rows, dim3((unsigned)nblk), dim3(P.bi->nt), 0, stream, data, nrows,
+                                        P.ba, inverse); });
+      return;
+    }
     const int lines = rt_lines(P.k.L, 64);
     const long long nblk = (nrows + lines - 1) / lines;
     require(nblk < (1ll << 31), CBS_ERR_UNSUPPORTED, "run-time FFT batch too large for one launch");
@@ -308,11 +330,19 @@ struct Solver {
   // outer set o < nouter at (o / inner) * os2 + (o % inner) * os (inner = 0: one level)
   void rt_strided(const RtPlan& P, float2* data, int ncols, long long es, long long os, long long nouter,
                   int inverse, long long inner = 0, long long os2 = 0) {
+    const int in = (int)(inner > 0 ? inner : nouter);
+    if (P.bi) {
+      const int T = P.bi->t;
+      const long long nblk = (long long)((ncols + T - 1) / T) * nouter;
+      require(nblk < (1ll << 31), CBS_ERR_UNSUPPORTED, "run-time FFT batch too large for one launch");
+      timed(2, [&] { hipLaunchKernelGGL(P.bi->strided, dim3((unsigned)nblk), dim3(P.bi->nt), 0, stream, data, ncols,
+                                        es, os, (int)nouter, P.ba, inverse, in, os2); });
+      return;
+    }
     const int T = rt_lines(P.k.L, 16);
     const long long nblk = (long long)((ncols + T - 1) / T) * nouter;
     require(nblk < (1ll << 31), CBS_ERR_UNSUPPORTED, "run-time FFT batch too large for one launch");
     const size_t lds = 2ull * T * P.k.L * sizeof(float2);
-    const int in = (int)(inner > 0 ? inner : nouter);
     timed(2, [&] { hipLaunchKernelGGL(fft_strided_rt, dim3((unsigned)nblk), dim3(256), lds, stream, data, ncols,
                                       es, os, (int)nouter, P.k, T, inverse, in, os2); });
   }

@@ -5,9 +5,11 @@ The reference plans its FFTs on whatever simulation size it gets
 rounding of the size (FORWARD_SOLVER_CONVERGENT_BORN.m:120).  Such grids run the
 per-axis form with run-time-planned line FFTs: Stockham stages of radix 2, 3, 4, 5, 7,
 8, 9, 11, 13 and 16, and Bluestein's chirp-z transform for lengths with a larger prime
-factor.  Checked at reduced iteration counts (<= 1e-5 relative L2 against the oracle):
-  * Born loops on grids with factors 7, 11, 13, with prime lengths (97, 83) and with
-    lengths 2 x a prime;
+factor (on a compiled convolution length up to 1296, so lines up to 648; on run-time
+Stockham stages above that).  Checked at reduced iteration counts (<= 1e-5 relative L2
+against the oracle):
+  * Born loops on grids with factors 7, 11, 13, with prime lengths (97, 83, 19 and the
+    661 of the run-time Bluestein) and with lengths 2 x a prime;
   * the reference demo's forward geometry (SIMULATION_BEAD.m:11-19: 161 x 161 x 61 at
     0.532 / 4 / 1.2 um, default boundary_thickness 6 -> 183 x 183 x 83, convolution grid
     322 x 322), the whole 2D-in/2D-out solve;
@@ -49,6 +51,7 @@ def _loop_case(ri, bt_px, iters, nrhs=2, **kw):
     ((81, 60, 41), 8),       # sim 97 x 76 x 57: the prime 97, 4 x 19, 3 x 19 (Bluestein)
     ((30, 22, 20), 3),       # sim 36 x 28 x 26: 4 x 7, 2 x 13 with a compiled x length
     ((100, 120, 56), 8),     # sim 116 x 136 x 72: 4 x 29, 8 x 17 (Bluestein), 72 (Stockham)
+    ((651, 9, 9), 5),        # sim 661 x 19 x 19: 661 (Bluestein on run-time stages), 19 (compiled, L = 48)
 ])
 def test_born_loop_any_grid(ri, bt_px):
     sim, plan, errs = _loop_case(ri, bt_px, iters=6)
