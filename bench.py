@@ -329,6 +329,63 @@ def workload_large(args):
                                    'accuracy': {'rel_l2_err_vs_fp64_oracle': err, 'angles': 1, 'iterations': 6}})
 
 
+def hbm_ceiling(stream, log, gib=2, reps=10):
+    """Practical HBM ceilings on this device, in the same run: the nontemporal 16-byte
+    streaming probe (inverse_solver_amd/csrc/stream_probe.hip, libcbs_probe.so) over
+    arrays of `gib` GiB -- a copy (1 read + 1 write stream) and a 3-read/2-write mix --
+    best of three runs of `reps` launches, HIP events on the solver's stream; and the
+    runtime's own hipMemcpy device-to-device (host timer around the synchronous copy).
+    Rates count bytes read + written."""
+    import ctypes as C
+    from inverse_solver_amd import device as dv
+    lib = C.CDLL(os.path.join(ROOT, 'inverse_solver_amd', 'libcbs_probe.so'))
+    f = lib.cbs_probe_stream
+    f.restype = C.c_int
+    f.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_int, C.c_void_p,
+                  C.POINTER(C.c_float)]
+    nbytes = gib << 30
+    arrs = [dv.DeviceArray((nbytes // 8,), np.complex64) for _ in range(3)]
+    for d in arrs:
+        d.upload(np.zeros(nbytes // 8, np.complex64))
+    res = {}
+    for kind, name, streams in ((0, 'stream_copy', 2), (1, 'stream_3r2w', 5)):
+        best = 1e30
+        for _ in range(3):
+            ms = C.c_float(0.0)
+            rc = f(kind, arrs[0].ptr, arrs[1].ptr, arrs[2].ptr, nbytes, reps, C.c_void_p(stream), C.byref(ms))
+            if rc != 0:
+                raise RuntimeError('cbs_probe_stream(%d) failed: %d' % (kind, rc))
+            best = min(best, ms.value)
+        gbs = streams * nbytes / (best / 1e3) / 1e9
+        res[name] = {'GB_s': round(gbs, 1), 'frac_of_peak': round(gbs / HBM_PEAK_GBS, 4), 'ms': round(best, 4)}
+    arrs[1].copy_from(arrs[0])
+    dv.synchronize()
+    best = 1e30
+    for _ in range(5):
+        t0 = time.perf_counter()
+        arrs[1].copy_from(arrs[0])
+        dv.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    gbs = 2.0 * nbytes / best / 1e9
+    res['hipmemcpy_d2d'] = {'GB_s': round(gbs, 1), 'frac_of_peak': round(gbs / HBM_PEAK_GBS, 4)}
+    for d in arrs:
+        d.free()
+    res['what'] = ('%d GiB arrays; stream_*: nontemporal 16-byte loads/stores, 256-thread blocks, best of 3 x %d '
+                   'launches by HIP events; hipmemcpy_d2d: best of 5 synchronous copies' % (gib, reps))
+    log('HBM ceilings: stream copy %.0f, stream 3R2W %.0f, hipMemcpy %.0f GB/s'
+        % (res['stream_copy']['GB_s'], res['stream_3r2w']['GB_s'], gbs))
+    return res
+
+
+def workload_ceiling(args):
+    """The practical HBM ceilings alone (hbm_ceiling on the null stream): what the
+    passes' achieved GB/s are compared against."""
+    c = hbm_ceiling(None, lambda *a: print(*a, file=sys.stderr, flush=True))
+    return _line('streaming HBM ceiling (3 read + 2 write streams)', c['stream_3r2w']['GB_s'], 'GB/s',
+                 'nontemporal 16-byte streaming probe over 2 GiB arrays', c['stream_3r2w']['ms'], args,
+                 {'hbm_ceiling': c}, dtype='f32')
+
+
 def reference_form_gpu(s, u0, M, log_fn):
     """The reference's own GPU loop, op for op: FORWARD_SOLVER_CONVERGENT_BORN.solve_raw
     (:577-647) as MATLAB runs it on gpuArrays for one angle -- one device kernel per
@@ -559,7 +616,7 @@ def main():
     ap.add_argument('--copy-probe-mb', type=int, default=0,
                     help='run one device-to-device copy of this many MiB first (calibrates HBM byte counters)')
     ap.add_argument('--workload', default='batched',
-                    choices=['batched', 'c1', 'inverse', 'serving', 'large', 'vector', 'retrieval'],
+                    choices=['batched', 'c1', 'inverse', 'serving', 'large', 'vector', 'retrieval', 'ceiling'],
                     help='batched: the headline 49-angle solve (default); the others are the secondary '
                          'configurations, each printing its own JSON line')
     ap.add_argument('--requests', type=int, default=200, help='serving: number of requests')
@@ -578,7 +635,7 @@ def main():
     if args.workload != 'batched':
         out = {'c1': workload_c1, 'inverse': workload_inverse, 'serving': workload_serving,
                'large': workload_large, 'vector': workload_vector,
-               'retrieval': workload_retrieval}[args.workload](args)
+               'retrieval': workload_retrieval, 'ceiling': workload_ceiling}[args.workload](args)
         print(json.dumps(out), flush=True)
         return
 
@@ -747,26 +804,12 @@ def main():
         log('whole 2D solve %.1f ms -> %.2f Gvoxel-iter/s' % (wm, out['whole_solve_2d']['value']))
         d_src.free()
         d_t.free()
+    # the ceilings on the solver's stream, before the handle (and its stream) is closed
+    out['hbm_ceiling'] = hbm_ceiling(stream, log)
+    c = out['hbm_ceiling']['stream_3r2w']['GB_s']
+    out['roofline']['frac_of_stream_ceiling'] = round(achieved_pr / c, 4)
+    out['roofline']['pass_kspace']['frac_of_stream_ceiling'] = round(achieved_pk / c, 4)
     s.close()
-
-    # practical HBM ceiling on this box: device-to-device copy of 2 GiB (bytes read +
-    # written), best of five
-    ca, cb = dv.DeviceArray((1 << 28,), np.complex64), dv.DeviceArray((1 << 28,), np.complex64)
-    cb.copy_from(ca)
-    dv.synchronize()
-    best = 1e30
-    for _ in range(5):
-        t0 = time.perf_counter()
-        cb.copy_from(ca)
-        dv.synchronize()
-        best = min(best, time.perf_counter() - t0)
-    copy_gbs = 2.0 * ca.nbytes / best / 1e9
-    out['hbm_copy_ceiling'] = {'GB_s': round(copy_gbs, 1), 'frac_of_peak': round(copy_gbs / HBM_PEAK_GBS, 4),
-                               'what': 'hipMemcpy device-to-device of 2 GiB, read + written bytes, best of 5 '
-                                       '(host timer around the synchronous copy)'}
-    log('copy ceiling %.0f GB/s' % copy_gbs)
-    ca.free()
-    cb.free()
 
     if not args.no_plain:
         # plain single-angle form (unfused per-axis kernels, batch 1) on the same inputs
