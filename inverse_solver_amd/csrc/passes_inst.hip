@@ -1,14 +1,13 @@
 // Explicit instantiations of the templated passes and a lookup registry.
 // The shape lists cover the simulation grids of the supported workloads
 // (sim 96^3, 160^3, 128x128x135, 128^3, 192^2x128, 288^2x160, 576^2x320) and the small grids
-// used by the tests; a grid whose line lengths are absent here runs the per-axis
-// form with run-time-planned line FFTs (fft_rt.hip).
+// used by the tests; any other grid runs the per-axis form on the line FFTs of
+// lines_inst.hip (every 5-smooth length up to 1296) or the run-time-planned ones
+// (fft_rt.hip, fft_blue.hip).
 #include "cbs_passes.hpp"
 #include "kernels_misc.hpp"
 
 namespace cbs {
-
-constexpr int kNT = 256;
 
 // (NX, A, NT): x length, rows of the y split, threads per block
 #define PR_LIST(X) \
@@ -48,19 +47,6 @@ constexpr int kNT = 256;
 // vector runs, one component at a time through LDS (pass_kspace_seq3): (B, NZ, T, NT)
 #define PKS_LIST(X) X(4, 160, 16, 1024) X(4, 160, 8, 512)
 
-#define RW_LIST(X) \
-  X(16) X(18) X(20) X(24) X(25) X(27) X(30) X(32) X(36) X(40) X(48) X(64) X(96) X(128) X(135) X(160) X(192) \
-  X(256) X(288) X(320) X(512) X(576) X(1024)
-
-#define ST_LIST(X) \
-  X(16) X(18) X(20) X(24) X(25) X(27) X(30) X(32) X(36) X(40) X(48) X(64) X(96) X(128) X(135) X(160) X(192) \
-  X(256) X(288) X(320) X(512) X(576) X(1024)
-
-constexpr int rows_for(int n) { return n <= 64 ? 16 : (n <= 256 ? 8 : (n <= 576 ? 4 : 2)); }
-// 16-wide column tiles (128-byte rows) up to n = 576 (78 KiB with the twiddles: two blocks
-// per CU); at 576 this took the y transforms of the 576^2 x 320 grid from 12.1 to 9.8 ms
-constexpr int strided_t(int n) { return n <= 576 ? 16 : 8; }
-
 // LDS of pass_real: the tile plus its per-axis tables (x: twiddle; rows: twiddle, ramp, mask)
 constexpr int pr_lds(int nx, int a) { return a * row_pitch(nx) * 8 + nx * 8 + a * 20; }
 #define PR_ENTRY(NX, A, NT) {&pass_real<NX, A, NT, 0, 0>, &pass_real<NX, A, NT, 0, 1>, NX, A, NT, pr_lds(NX, A), 0},
@@ -70,13 +56,9 @@ constexpr int pr_lds(int nx, int a) { return a * row_pitch(nx) * 8 + nx * 8 + a 
   {&pass_kspace<B, NZ, T, NT, 3>, B, NZ, T, NT, (int)((3 * NZ * B * T + 2 * NZ) * sizeof(float2)), 3},
 #define PKS_ENTRY(B, NZ, T, NT) \
   {&pass_kspace_seq3<B, NZ, T, NT>, B, NZ, T, NT, (int)(NZ * B * T * sizeof(float2)), 3},
-#define RW_ENTRY(N) {&fft_rows<N, rows_for(N), kNT>, N, rows_for(N), kNT},
-#define ST_ENTRY(N) {&fft_strided<N, strided_t(N), kNT>, N, strided_t(N), kNT},
 
 static const PassRInfo kPR[] = {PR_LIST(PR_ENTRY) PRR_LIST(PRR_ENTRY)};
 static const PassKInfo kPK[] = {PK_LIST(PK_ENTRY) PKV_LIST(PKV_ENTRY) PKS_LIST(PKS_ENTRY)};
-static const RowsInfo kRW[] = {RW_LIST(RW_ENTRY)};
-static const StridedInfo kST[] = {ST_LIST(ST_ENTRY)};
 
 const PassRInfo* find_pass_real(int nx, int a) {
   for (const auto& e : kPR)
@@ -86,16 +68,6 @@ const PassRInfo* find_pass_real(int nx, int a) {
 const PassKInfo* find_pass_kspace(int b, int nz, int t, int pole) {
   for (const auto& e : kPK)
     if (e.b == b && e.nz == nz && e.t == t && e.pole == pole) return &e;
-  return nullptr;
-}
-const RowsInfo* find_fft_rows(int n) {
-  for (const auto& e : kRW)
-    if (e.n == n) return &e;
-  return nullptr;
-}
-const StridedInfo* find_fft_strided(int n, int t) {
-  for (const auto& e : kST)
-    if (e.n == n && (t <= 0 || e.t == t)) return &e;
   return nullptr;
 }
 int list_pass_real(const PassRInfo** out) {

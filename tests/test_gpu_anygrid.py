@@ -3,13 +3,15 @@
 The reference plans its FFTs on whatever simulation size it gets
 (Codes/CUDA_CODE/execution.cu:11-34, execution2.cu:11-35; MATLAB fftn), with no
 rounding of the size (FORWARD_SOLVER_CONVERGENT_BORN.m:120).  Such grids run the
-per-axis form with run-time-planned line FFTs: Stockham stages of radix 2, 3, 4, 5, 7,
-8, 9, 11, 13 and 16, and Bluestein's chirp-z transform for lengths with a larger prime
-factor (on a compiled convolution length up to 1296, so lines up to 648; on run-time
-Stockham stages above that).  Checked at reduced iteration counts (<= 1e-5 relative L2
-against the oracle):
+per-axis form: compiled line FFTs for every 5-smooth length up to 1296 (but 1000, 1125,
+1200, 1215, 1250, 1280), run-time-planned Stockham stages of radix 2, 3, 4, 5, 7, 8, 9,
+11, 13 and 16 for the other lengths that factor into those, and Bluestein's chirp-z
+transform for lengths with a larger prime factor (on a compiled convolution length up
+to 1296, so lines up to 648; on run-time Stockham stages above that).  Checked at
+reduced iteration counts (<= 1e-5 relative L2 against the oracle):
   * Born loops on grids with factors 7, 11, 13, with prime lengths (97, 83, 19 and the
-    661 of the run-time Bluestein) and with lengths 2 x a prime;
+    661 of the run-time Bluestein), with lengths 2 x a prime, and with 5-smooth lengths
+    that have no fused kernel (100, 120, 75);
   * the reference demo's forward geometry (SIMULATION_BEAD.m:11-19: 161 x 161 x 61 at
     0.532 / 4 / 1.2 um, default boundary_thickness 6 -> 183 x 183 x 83, convolution grid
     322 x 322), the whole 2D-in/2D-out solve;
@@ -50,7 +52,8 @@ def _loop_case(ri, bt_px, iters, nrhs=2, **kw):
     ((34, 36, 44), 4),       # sim 42 x 44 x 52: factors 7, 11, 13 (Stockham)
     ((81, 60, 41), 8),       # sim 97 x 76 x 57: the prime 97, 4 x 19, 3 x 19 (Bluestein)
     ((30, 22, 20), 3),       # sim 36 x 28 x 26: 4 x 7, 2 x 13 with a compiled x length
-    ((100, 120, 56), 8),     # sim 116 x 136 x 72: 4 x 29, 8 x 17 (Bluestein), 72 (Stockham)
+    ((100, 120, 56), 8),     # sim 116 x 136 x 72: 4 x 29, 8 x 17 (Bluestein), 72 (compiled)
+    ((84, 104, 59), 8),      # sim 100 x 120 x 75: compiled line FFTs on every axis
     ((651, 9, 9), 5),        # sim 661 x 19 x 19: 661 (Bluestein on run-time stages), 19 (compiled, L = 48)
 ])
 def test_born_loop_any_grid(ri, bt_px):
