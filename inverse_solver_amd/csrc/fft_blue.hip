@@ -8,6 +8,7 @@
 // filter's spectrum and the inverse FFT (line_fft_mul_ifft), and multiplies by conj(c)
 // on the way out.  The filter spectrum carries 1/L, so the result is the plain DFT.
 #include "fft_rt.hpp"
+#include "cbs_kernels.hpp"
 
 namespace cbs {
 
@@ -162,6 +163,237 @@ __global__ __launch_bounds__(NT) void blue_strided(float2* data, int nx, long lo
   }
 }
 
+// ------------------------------------------------------------------ per-axis form, fused
+// Per-axis form of a single-component run whose x lines are Bluestein-transformed: the
+// inverse x transform of the spectrum rows, the pointwise update of update_plain
+// (born_step) and the forward x transform of the next p in one launch.  Block (bx, slot)
+// takes rows bx NL .. of component slot `slot` (ny nz rows of nx per slot, contiguous).
+// upd = UPD_NONE skips the inverse transform (the first launch: p from the source term);
+// fwd = FWD_NONE skips the forward one (the last).  A slot that the convergence test has
+// stopped leaves Field, f and the spectrum alone (after the flush of a deferred
+// increment), as update_plain does.
+template <int L, int NL, int NT>
+__global__ __launch_bounds__(NT) void blue_rows_update(StateArgs st, Tables tb, Consts c, BlueArgs p, int upd,
+                                                       int fwd, int jj_inv, int jj_fwd, ConvArgs cv) {
+  constexpr int NMAX = (L + 1) / 2;
+  constexpr int K = (NL * NMAX + NT - 1) / NT;
+  constexpr int KZ = (NL * (L - NMAX) + NT - 1) / NT;
+  constexpr int CH = 4;                                  // pointwise elements in flight per thread
+  __shared__ float2 lds[L * (NL + 1)];
+  __shared__ float2 s_tw[L];
+  __shared__ float2 s_b[L];
+  const int n = p.n;
+  const int slot = blockIdx.y;
+  const long long rows = (long long)c.ny * c.nz;
+  const long long r0 = (long long)blockIdx.x * NL;
+  const int nr = rows - r0 < NL ? (int)(rows - r0) : NL;
+  const long long ro = (long long)slot * st.rhs_stride + r0 * n;
+  float2* __restrict__ spec = st.spec + ro;
+  float2* __restrict__ F = st.field + ro;
+  float2* __restrict__ f = st.fstate + ro;
+  const float2* __restrict__ V = st.V + r0 * n;
+  if (cv.norms && jj_inv > 0) {
+    const int rr = slot / cv.P;
+    if (conv_stopped(cv, jj_inv - 1, rr)) {
+      if (!conv_stopped(cv, jj_inv - 2, rr) && upd_kind(jj_inv - 1, cv.M) == UPD_DEFER)
+        for (int i = threadIdx.x; i < nr * n; i += NT) F[i] = cadd(F[i], f[i]);
+      return;
+    }
+  }
+  const BlueTile<NL> ad;
+  blue_tables<L, NT>(p, s_tw, s_b);
+  if (upd != UPD_NONE) {
+    // inverse x transform: conj(DFT(conj q))
+    float2 xv[K], cv2[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = threadIdx.x + k * NT;
+      const int l = i / NMAX, e = i - l * NMAX;
+      xv[k] = cv2[k] = make_float2(0.f, 0.f);
+      if (i < NL * NMAX && e < n && l < nr) {
+        xv[k] = spec[(long long)l * n + e];
+        cv2[k] = p.chirp[e];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = threadIdx.x + k * NT;
+      const int l = i / NMAX, e = i - l * NMAX;
+      if (i < NL * NMAX) lds[ad(l, e)] = cmulc(cconj(xv[k]), cv2[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < KZ; ++k) {
+      const int i = threadIdx.x + k * NT;
+      const int l = i / (L - NMAX), e = NMAX + i - l * (L - NMAX);
+      if (i < NL * (L - NMAX)) lds[ad(l, e)] = make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    blue_convolve<L, NL, NT>(lds, s_tw, s_b);
+  } else {
+    __syncthreads();   // the tables
+  }
+  const bool rf = reads_f(upd, fwd), rF = reads_F(upd);
+  const bool wF = writes_F(upd), wf = fwd == FWD_GAMMA;
+  const bool want_norm = st.norms != nullptr;
+  const bool want_fref = st.fref != nullptr && upd == UPD_VQ_ADD;
+  const float sg_inv = sign_inv(jj_inv ? flip_of(jj_inv, st.parity, slot) : 0);
+  const float sg_fwd = sign_fwd(jj_fwd ? flip_of(jj_fwd, st.parity, slot) : 0);
+  double nrm = 0.0, nrmF = 0.0;
+  // pointwise: element (l, e) of the tile is voxel r0 + l rows, x = e of this slot; the
+  // inverse result is c_e conj(w_e), the forward input p conj(c_e) (zero above n)
+#pragma unroll
+  for (int k0 = 0; k0 < K; k0 += CH) {
+    float2 ch[CH], vv[CH], xo[CH], Fo[CH], w[CH];
+    bool ok[CH];
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      const int i = threadIdx.x + (k0 + k) * NT;
+      const int l = i / NMAX, e = i - l * NMAX;
+      ok[k] = k0 + k < K && i < NL * NMAX && e < n && l < nr;
+      if (ok[k]) {
+        const long long g = (long long)l * n + e;
+        ch[k] = p.chirp[e];
+        vv[k] = V[g];
+        xo[k] = rf ? f[g] : make_float2(0.f, 0.f);
+        Fo[k] = rF ? F[g] : make_float2(0.f, 0.f);
+        w[k] = upd != UPD_NONE ? lds[ad(l, e)] : make_float2(0.f, 0.f);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      if (k0 + k >= K) continue;
+      const int i = threadIdx.x + (k0 + k) * NT;
+      const int l = i / NMAX, e = i - l * NMAX;
+      if (i >= NL * NMAX) continue;
+      if (!ok[k]) {
+        lds[ad(l, e)] = make_float2(0.f, 0.f);
+        continue;
+      }
+      const long long g = (long long)l * n + e;
+      const long long row = r0 + l;
+      const int y = (int)(row % c.ny), z = (int)(row / c.ny);
+      const float2 r = cmul(cmul(tb.ramp_x[e], tb.ramp_y[y]), tb.ramp_z[z]);
+      const float m = tb.mask_x[e] * tb.mask_y[y] * tb.mask_z[z];
+      const float2 q = cmul(ch[k], cconj(w[k]));
+      float2 Fn, xn;
+      const float2 pn = born_step(upd, fwd, q, r, m, vv[k], xo[k], Fo[k], c.inv_eps, sg_inv, sg_fwd, Fn, xn,
+                                  want_norm, nrm);
+      if (wF) F[g] = Fn;
+      if (want_fref) nrmF += (double)Fn.x * Fn.x + (double)Fn.y * Fn.y;
+      if (wf) f[g] = xn;
+      lds[ad(l, e)] = cmulc(pn, ch[k]);
+    }
+  }
+  if (want_norm) flush_norm(nrm, st.norms + slot);
+  if (want_fref) flush_norm(nrmF, st.fref + slot);
+  if (fwd == FWD_NONE) return;
+#pragma unroll
+  for (int k = 0; k < KZ; ++k) {                         // zero padding above NMAX
+    const int i = threadIdx.x + k * NT;
+    const int l = i / (L - NMAX), e = NMAX + i - l * (L - NMAX);
+    if (i < NL * (L - NMAX)) lds[ad(l, e)] = make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  blue_convolve<L, NL, NT>(lds, s_tw, s_b);
+  float2 cv3[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = threadIdx.x + k * NT;
+    const int l = i / NMAX, e = i - l * NMAX;
+    if (i < NL * NMAX && e < n && l < nr) cv3[k] = p.chirp[e];
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = threadIdx.x + k * NT;
+    const int l = i / NMAX, e = i - l * NMAX;
+    if (i < NL * NMAX && e < n && l < nr) spec[(long long)l * n + e] = cmulc(lds[ad(l, e)], cv3[k]);
+  }
+}
+
+// Per-axis form of a single-component run whose z lines are Bluestein-transformed: the
+// forward z transform, the Green's multiply of green_plain and the inverse z transform in
+// one launch.  Tiles as blue_strided with es = nx ny and outer set o = y + ny slot.
+template <int L, int T, int NT>
+__global__ __launch_bounds__(NT) void blue_strided_green(float2* data, Tables tb, Consts c, BlueArgs p,
+                                                         const uint8_t* __restrict__ parity, int jj, ConvArgs cv) {
+  constexpr int NMAX = (L + 1) / 2;
+  constexpr int K = (NMAX * T + NT - 1) / NT;
+  constexpr int KZ = ((L - NMAX) * T + NT - 1) / NT;
+  __shared__ float2 lds[L * (T + 1)];
+  __shared__ float2 s_tw[L];
+  __shared__ float2 s_b[L];
+  const int n = p.n;
+  const int ntile = (c.nx + T - 1) / T;
+  const int xt = blockIdx.x % ntile;
+  const int o = blockIdx.x / ntile;
+  const int y = o % c.ny, slot = o / c.ny;      // the grid holds ceil(nx / T) ny slots blocks
+  if (cv.norms && conv_stopped(cv, jj - 1, slot)) return;
+  const int flip = flip_of(jj, parity, slot);
+  const int x0 = xt * T;
+  const int tv = c.nx - x0 < T ? c.nx - x0 : T;
+  const long long es = (long long)c.nx * c.ny;
+  float2* base = data + (long long)slot * c.nvox + (long long)y * c.nx + x0;
+  const BlueTile<T> ad;
+  float2 xv[K], chv[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = threadIdx.x + k * NT;
+    const int e = i / T, t = i - e * T;
+    xv[k] = chv[k] = make_float2(0.f, 0.f);
+    if (i < NMAX * T && e < n && t < tv) {
+      xv[k] = base[e * es + t];
+      chv[k] = p.chirp[e];
+    }
+  }
+  blue_tables<L, NT>(p, s_tw, s_b);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = threadIdx.x + k * NT;
+    const int e = i / T, t = i - e * T;
+    if (i < NMAX * T) lds[ad(t, e)] = cmulc(xv[k], chv[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < KZ; ++k) {
+    const int i = threadIdx.x + k * NT;
+    const int e = NMAX + i / T, t = i % T;
+    if (i < (L - NMAX) * T) lds[ad(t, e)] = make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  blue_convolve<L, T, NT>(lds, s_tw, s_b);
+  // X_k = conj(c_k) w_k; Y_k = G_k X_k; the inverse transform's input conj(Y_k) conj(c_k)
+  const float gy = tb.green_y[flip * c.ny + y] - c.km2;
+  const float eps2 = c.eps * c.eps;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = threadIdx.x + k * NT;
+    const int e = i / T, t = i - e * T;
+    if (i >= NMAX * T) continue;
+    float2 v = make_float2(0.f, 0.f);
+    if (e < n && t < tv) {
+      const float d = tb.green_x[flip * c.nx + x0 + t] + gy + tb.green_z[flip * c.nz + e];
+      const float s = c.inv_n * __builtin_amdgcn_rcpf(fmaf(d, d, eps2));   // hardware reciprocal (1 ulp)
+      const float2 X = cmulc(lds[ad(t, e)], chv[k]);
+      const float2 Y = cmul(X, make_float2(d * s, c.eps * s));
+      v = cmulc(cconj(Y), chv[k]);
+    }
+    lds[ad(t, e)] = v;
+  }
+#pragma unroll
+  for (int k = 0; k < KZ; ++k) {
+    const int i = threadIdx.x + k * NT;
+    const int e = NMAX + i / T, t = i % T;
+    if (i < (L - NMAX) * T) lds[ad(t, e)] = make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  blue_convolve<L, T, NT>(lds, s_tw, s_b);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = threadIdx.x + k * NT;
+    const int e = i / T, t = i - e * T;
+    if (i < NMAX * T && e < n && t < tv) base[e * es + t] = cmul(chv[k], cconj(lds[ad(t, e)]));
+  }
+}
+
 // (L, lines per rows block, lines per strided tile, threads): 5-smooth lengths whose
 // two-step plan has both factors <= 36
 #define BLUE_LIST(X)                                                                                           \
@@ -170,7 +402,9 @@ __global__ __launch_bounds__(NT) void blue_strided(float2* data, int nx, long lo
   X(512, 16, 16, 256) X(640, 8, 8, 256) X(768, 8, 8, 256) X(1024, 8, 8, 256) X(1152, 8, 8, 256)                \
   X(1296, 8, 8, 256)
 
-#define BLUE_ENTRY(L, NL, T, NT) {&blue_rows<L, NL, NT>, &blue_strided<L, T, NT>, L, NL, T, NT, LineFft<L>::R2},
+#define BLUE_ENTRY(L, NL, T, NT) \
+  {&blue_rows<L, NL, NT>, &blue_strided<L, T, NT>, &blue_rows_update<L, NL, NT>, &blue_strided_green<L, T, NT>, \
+   L, NL, T, NT, LineFft<L>::R2},
 static const BlueInfo kBlue[] = {BLUE_LIST(BLUE_ENTRY)};
 
 const BlueInfo* find_blue(int m) {

@@ -9,7 +9,7 @@
 // are 2, 3, 5 and 7, and the result is multiplied by conj(c_k).  Unnormalised in both
 // directions, like the compiled line FFTs.
 #pragma once
-#include "fft_core.hpp"
+#include "cbs_kernels.hpp"
 
 namespace cbs {
 
@@ -57,9 +57,15 @@ struct BlueArgs {
 // rows: NL contiguous lines of n per block; strided: as fft_strided_rt with T = t lines
 using BlueRowsFn = void (*)(float2*, long long, BlueArgs, int);
 using BlueStridedFn = void (*)(float2*, int, long long, long long, int, BlueArgs, int, int, long long);
+// per-axis form, one field component: x inverse + pointwise update + x forward, and z
+// forward + Green's multiply + z inverse (fft_blue.hip)
+using BlueRowsUpdateFn = void (*)(StateArgs, Tables, Consts, BlueArgs, int, int, int, int, ConvArgs);
+using BlueStridedGreenFn = void (*)(float2*, Tables, Consts, BlueArgs, const uint8_t*, int, ConvArgs);
 struct BlueInfo {
   BlueRowsFn rows;
   BlueStridedFn strided;
+  BlueRowsUpdateFn rows_update;
+  BlueStridedGreenFn strided_green;
   int L, nl, t, nt;
   int r2;                   // forward output position p holds frequency p / r2 + (L / r2) (p % r2)
 };

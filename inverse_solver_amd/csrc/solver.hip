@@ -819,7 +819,8 @@ struct Solver {
       timed(2, [&] { hipLaunchKernelGGL(update_plain, dim3(pw_grid, (unsigned)nv), dim3(256), 0, stream, s, tb, c, upd,
                                         fwd, jj_inv, jj_fwd, cv); });
     };
-    auto fft3 = [&](int inverse) {
+    // the line transforms of one axis (0 x, 1 y, 2 z) over every slot
+    auto fft1 = [&](int axis, int inverse) {
       float2* sp = ws.spec;
       const RowsInfo* rw = plan.rw;
       const StridedInfo* sy = plan.sy;
@@ -851,21 +852,54 @@ struct Solver {
                                           (long long)nx * ny, (long long)nx, (int)nouter, tabs.tw_z.p, inverse, ny,
                                           (long long)nvox); });
       };
-      if (!inverse) { fx(); fy(); fz(); }
-      else { fz(); fy(); fx(); }
+      if (axis == 0) fx();
+      else if (axis == 1) fy();
+      else fz();
     };
-    update(UPD_NONE, FWD_P0, 0, 1);
+    // single-component runs whose x (z) lines are Bluestein-transformed on a compiled
+    // length: the x inverse, the update and the x forward (the z forward, the Green's
+    // multiply and the z inverse) run as one launch each
+    const BlueInfo* bx = (P == 1 && plan.rt[0]) ? rtp[0].bi : nullptr;
+    const BlueInfo* bz = (P == 1 && plan.rt[2]) ? rtp[2].bi : nullptr;
+    auto xpass = [&](int upd, int fwd, int jj_inv, int jj_fwd) {   // (x inverse,) update(, x forward)
+      if (!bx) {
+        if (upd != UPD_NONE) fft1(0, 1);
+        update(upd, fwd, jj_inv, jj_fwd);
+        if (fwd != FWD_NONE) fft1(0, 0);
+        return;
+      }
+      StateArgs s = state_args(1);
+      s.norms = norm_row(jj_inv, nv);
+      if (conv_tol > 0.0 && upd == UPD_VQ_ADD) s.fref = ws.fref;
+      const long long rows = (long long)ny * nz;
+      const dim3 g((unsigned)((rows + bx->nl - 1) / bx->nl), (unsigned)nv);
+      timed(2, [&] { hipLaunchKernelGGL(bx->rows_update, g, dim3(bx->nt), 0, stream, s, tb, c, rtp[0].ba, upd, fwd,
+                                        jj_inv, jj_fwd, cv); });
+    };
+    auto zpass = [&](int jj) {   // z forward, Green's multiply, z inverse
+      if (!bz) {
+        fft1(2, 0);
+        if (P == 1)
+          timed(2, [&] { hipLaunchKernelGGL(green_plain, dim3(pw_grid, (unsigned)nb), dim3(256), 0, stream, ws.spec,
+                                            tb, c, ws.parity, jj, cv); });
+        else
+          timed(2, [&] { hipLaunchKernelGGL(green_plain_vec, dim3(pw_grid, (unsigned)nb), dim3(256), 0, stream,
+                                            ws.spec, nvox, tb, c, ws.parity, jj, cv); });
+        fft1(2, 1);
+        return;
+      }
+      const long long nblk = (long long)((nx + bz->t - 1) / bz->t) * ny * nv;
+      require(nblk < (1ll << 31), CBS_ERR_UNSUPPORTED, "z transform batch too large for one launch");
+      timed(2, [&] { hipLaunchKernelGGL(bz->strided_green, dim3((unsigned)nblk), dim3(bz->nt), 0, stream, ws.spec,
+                                        tb, c, rtp[2].ba, ws.parity, jj, cv); });
+    };
+    xpass(UPD_NONE, FWD_P0, 0, 1);
     for (int jj = 1; jj <= M; ++jj) {
-      fft3(0);
-      if (P == 1)
-        timed(2, [&] { hipLaunchKernelGGL(green_plain, dim3(pw_grid, (unsigned)nb), dim3(256), 0, stream, ws.spec, tb,
-                                          c, ws.parity, jj, cv); });
-      else
-        timed(2, [&] { hipLaunchKernelGGL(green_plain_vec, dim3(pw_grid, (unsigned)nb), dim3(256), 0, stream, ws.spec,
-                                          nvox, tb, c, ws.parity, jj, cv); });
-      fft3(1);
+      fft1(1, 0);
+      zpass(jj);
+      fft1(1, 1);
       const int fwd = fwd_of(jj, M);
-      update(upd_of(jj, M), fwd, jj, fwd == FWD_NONE ? 0 : jj + 1);
+      xpass(upd_of(jj, M), fwd, jj, fwd == FWD_NONE ? 0 : jj + 1);
     }
     HIP_CHECK(hipGetLastError());
   }

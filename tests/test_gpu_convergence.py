@@ -9,7 +9,9 @@ Checked here:
   * the fields equal the oracle's series truncated there (<= 1e-5), with the persistent
     flip parity of an odd Bornmax carried across the rhs as without the test;
   * a tolerance that never triggers gives the fields of the fixed-Bornmax solve bit for
-    bit, eager and replayed from a captured graph.
+    bit, eager and replayed from a captured graph;
+  * the same stops on a 23^3 grid, whose per-axis form runs the fused Bluestein kernels
+    (x inverse + update + x forward, z forward + Green's multiply + z inverse).
 The tolerances are set halfway (in log scale) between the oracle's ratio at the stop
 and the smallest ratio before it, so FP32 rounding cannot move the decision."""
 import numpy as np
@@ -24,10 +26,10 @@ RI_SIZE = (16, 16, 16)
 M = 31
 
 
-def _setup():
-    p = params(RI_SIZE, iterations=M)
-    RI = sphere_ri(RI_SIZE, 0.5, n_s=1.40)
-    fields = [plane_wave(RI_SIZE, 0.2, -0.1), plane_wave(RI_SIZE, -0.1, 0.25), random_field(RI_SIZE, 5)]
+def _setup(ri=RI_SIZE, radius=0.5, **kw):
+    p = params(ri, iterations=M, **kw)
+    RI = sphere_ri(ri, radius, n_s=1.40)
+    fields = [plane_wave(ri, 0.2, -0.1), plane_wave(ri, -0.1, 0.25), random_field(ri, 5)]
     return p, RI, fields
 
 
@@ -60,8 +62,20 @@ def _tol_for(r, even):
 @pytest.mark.parametrize('form', [0, 2, 1])
 @pytest.mark.parametrize('even', [True, False])
 def test_stop_iteration_and_fields_match_oracle(form, even):
+    _stop_case(_setup(), form, even)
+
+
+@pytest.mark.parametrize('even', [True, False])
+def test_stop_on_a_bluestein_grid(even):
+    # 9^3 RI + 7 px = 23^3: every axis a prime length, the per-axis form on Bluestein
+    # transforms of compiled length 48, x and z in the fused kernels
+    plan = _stop_case(_setup((9, 9, 9), radius=0.4, bt_px_target=7), None, even)
+    assert plan['form'] == 'plain'
+
+
+def _stop_case(setup, form, even):
     from inverse_solver_amd import FORWARD_SOLVER_CONVERGENT_BORN
-    p, RI, fields = _setup()
+    p, RI, fields = setup
     ratios = _ratios(p, RI, fields)
     tol, n0 = _tol_for(ratios[0], even)
     o = Oracle(p, io_kernels=False)
@@ -74,7 +88,7 @@ def test_stop_iteration_and_fields_match_oracle(form, even):
     assert stops[0] == n0 and min(stops) < M
     for a in range(len(fields)):     # no ratio of any rhs sits within FP32 noise of tol
         assert np.min(np.abs(ratios[a, 2:] / tol - 1.0)) > 1e-3, a
-    s = FORWARD_SOLVER_CONVERGENT_BORN(p, options=dict(form=form))
+    s = FORWARD_SOLVER_CONVERGENT_BORN(p, options=dict(form=form) if form is not None else None)
     s.set_RI(RI)
     s.set_convergence(tol)
     got = s.solve_incident(np.stack(fields, axis=-1))
@@ -87,7 +101,9 @@ def test_stop_iteration_and_fields_match_oracle(form, even):
     nrm = s.norms(len(fields))
     for a in range(len(fields)):
         assert np.all(nrm[a, stops[a]:] == 0) and np.all(nrm[a, :stops[a]] > 0)
+    plan = s.plan()
     s.close()
+    return plan
 
 
 def test_tolerance_that_never_triggers_is_bit_identical():
