@@ -178,20 +178,19 @@ __global__ __launch_bounds__(NT) void blue_rows_update(StateArgs st, Tables tb, 
   constexpr int NMAX = (L + 1) / 2;
   constexpr int K = (NL * NMAX + NT - 1) / NT;
   constexpr int KZ = (NL * (L - NMAX) + NT - 1) / NT;
-  constexpr int CH = 4;                                  // pointwise elements in flight per thread
   __shared__ float2 lds[L * (NL + 1)];
   __shared__ float2 s_tw[L];
   __shared__ float2 s_b[L];
   const int n = p.n;
   const int slot = blockIdx.y;
-  const long long rows = (long long)c.ny * c.nz;
-  const long long r0 = (long long)blockIdx.x * NL;
-  const int nr = rows - r0 < NL ? (int)(rows - r0) : NL;
-  const long long ro = (long long)slot * st.rhs_stride + r0 * n;
+  const int rows = c.ny * c.nz;                          // < 2^31: a slot's voxels fit a launch
+  const int r0 = blockIdx.x * NL;
+  const int nr = rows - r0 < NL ? rows - r0 : NL;
+  const long long ro = (long long)slot * st.rhs_stride + (long long)r0 * n;
   float2* __restrict__ spec = st.spec + ro;
   float2* __restrict__ F = st.field + ro;
   float2* __restrict__ f = st.fstate + ro;
-  const float2* __restrict__ V = st.V + r0 * n;
+  const float2* __restrict__ V = st.V + (long long)r0 * n;
   if (cv.norms && jj_inv > 0) {
     const int rr = slot / cv.P;
     if (conv_stopped(cv, jj_inv - 1, rr)) {
@@ -201,25 +200,33 @@ __global__ __launch_bounds__(NT) void blue_rows_update(StateArgs st, Tables tb, 
     }
   }
   const BlueTile<NL> ad;
-  blue_tables<L, NT>(p, s_tw, s_b);
-  if (upd != UPD_NONE) {
-    // inverse x transform: conj(DFT(conj q))
-    float2 xv[K], cv2[K];
+  const bool rq = upd != UPD_NONE;
+  const bool rf = reads_f(upd, fwd), rF = reads_F(upd);
+  // every global operand of the block's elements is requested before the first is used:
+  // the spectrum, the chirp, V, f and Field (element k of a thread: l = i / NMAX, e = i % NMAX)
+  float2 xq[K], ch[K], vv[K], xo[K], Fo[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int i = threadIdx.x + k * NT;
-      const int l = i / NMAX, e = i - l * NMAX;
-      xv[k] = cv2[k] = make_float2(0.f, 0.f);
-      if (i < NL * NMAX && e < n && l < nr) {
-        xv[k] = spec[(long long)l * n + e];
-        cv2[k] = p.chirp[e];
-      }
+  for (int k = 0; k < K; ++k) {
+    const int i = threadIdx.x + k * NT;
+    const int l = i / NMAX, e = i - l * NMAX;
+    xq[k] = ch[k] = vv[k] = xo[k] = Fo[k] = make_float2(0.f, 0.f);
+    if (i < NL * NMAX && e < n && l < nr) {
+      const int g = l * n + e;
+      ch[k] = p.chirp[e];
+      if (rq) xq[k] = spec[g];
+      vv[k] = V[g];
+      if (rf) xo[k] = f[g];
+      if (rF) Fo[k] = F[g];
     }
+  }
+  blue_tables<L, NT>(p, s_tw, s_b);
+  if (rq) {
+    // inverse x transform: conj(DFT(conj q))
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int i = threadIdx.x + k * NT;
       const int l = i / NMAX, e = i - l * NMAX;
-      if (i < NL * NMAX) lds[ad(l, e)] = cmulc(cconj(xv[k]), cv2[k]);
+      if (i < NL * NMAX) lds[ad(l, e)] = cmulc(cconj(xq[k]), ch[k]);
     }
 #pragma unroll
     for (int k = 0; k < KZ; ++k) {
@@ -232,57 +239,36 @@ __global__ __launch_bounds__(NT) void blue_rows_update(StateArgs st, Tables tb, 
   } else {
     __syncthreads();   // the tables
   }
-  const bool rf = reads_f(upd, fwd), rF = reads_F(upd);
   const bool wF = writes_F(upd), wf = fwd == FWD_GAMMA;
   const bool want_norm = st.norms != nullptr;
   const bool want_fref = st.fref != nullptr && upd == UPD_VQ_ADD;
   const float sg_inv = sign_inv(jj_inv ? flip_of(jj_inv, st.parity, slot) : 0);
   const float sg_fwd = sign_fwd(jj_fwd ? flip_of(jj_fwd, st.parity, slot) : 0);
   double nrm = 0.0, nrmF = 0.0;
-  // pointwise: element (l, e) of the tile is voxel r0 + l rows, x = e of this slot; the
-  // inverse result is c_e conj(w_e), the forward input p conj(c_e) (zero above n)
+  // pointwise: element (l, e) is voxel x = e of row r0 + l; the inverse result is
+  // c_e conj(w_e), the forward input p conj(c_e) (zero outside the lines)
 #pragma unroll
-  for (int k0 = 0; k0 < K; k0 += CH) {
-    float2 ch[CH], vv[CH], xo[CH], Fo[CH], w[CH];
-    bool ok[CH];
-#pragma unroll
-    for (int k = 0; k < CH; ++k) {
-      const int i = threadIdx.x + (k0 + k) * NT;
-      const int l = i / NMAX, e = i - l * NMAX;
-      ok[k] = k0 + k < K && i < NL * NMAX && e < n && l < nr;
-      if (ok[k]) {
-        const long long g = (long long)l * n + e;
-        ch[k] = p.chirp[e];
-        vv[k] = V[g];
-        xo[k] = rf ? f[g] : make_float2(0.f, 0.f);
-        Fo[k] = rF ? F[g] : make_float2(0.f, 0.f);
-        w[k] = upd != UPD_NONE ? lds[ad(l, e)] : make_float2(0.f, 0.f);
-      }
+  for (int k = 0; k < K; ++k) {
+    const int i = threadIdx.x + k * NT;
+    const int l = i / NMAX, e = i - l * NMAX;
+    if (i >= NL * NMAX) continue;
+    if (e >= n || l >= nr) {
+      lds[ad(l, e)] = make_float2(0.f, 0.f);
+      continue;
     }
-#pragma unroll
-    for (int k = 0; k < CH; ++k) {
-      if (k0 + k >= K) continue;
-      const int i = threadIdx.x + (k0 + k) * NT;
-      const int l = i / NMAX, e = i - l * NMAX;
-      if (i >= NL * NMAX) continue;
-      if (!ok[k]) {
-        lds[ad(l, e)] = make_float2(0.f, 0.f);
-        continue;
-      }
-      const long long g = (long long)l * n + e;
-      const long long row = r0 + l;
-      const int y = (int)(row % c.ny), z = (int)(row / c.ny);
-      const float2 r = cmul(cmul(tb.ramp_x[e], tb.ramp_y[y]), tb.ramp_z[z]);
-      const float m = tb.mask_x[e] * tb.mask_y[y] * tb.mask_z[z];
-      const float2 q = cmul(ch[k], cconj(w[k]));
-      float2 Fn, xn;
-      const float2 pn = born_step(upd, fwd, q, r, m, vv[k], xo[k], Fo[k], c.inv_eps, sg_inv, sg_fwd, Fn, xn,
-                                  want_norm, nrm);
-      if (wF) F[g] = Fn;
-      if (want_fref) nrmF += (double)Fn.x * Fn.x + (double)Fn.y * Fn.y;
-      if (wf) f[g] = xn;
-      lds[ad(l, e)] = cmulc(pn, ch[k]);
-    }
+    const int g = l * n + e;
+    const int row = r0 + l;
+    const int z = row / c.ny, y = row - z * c.ny;
+    const float2 r = cmul(cmul(tb.ramp_x[e], tb.ramp_y[y]), tb.ramp_z[z]);
+    const float m = tb.mask_x[e] * tb.mask_y[y] * tb.mask_z[z];
+    const float2 q = rq ? cmul(ch[k], cconj(lds[ad(l, e)])) : make_float2(0.f, 0.f);
+    float2 Fn, xn;
+    const float2 pn = born_step(upd, fwd, q, r, m, vv[k], xo[k], Fo[k], c.inv_eps, sg_inv, sg_fwd, Fn, xn,
+                                want_norm, nrm);
+    if (wF) F[g] = Fn;
+    if (want_fref) nrmF += (double)Fn.x * Fn.x + (double)Fn.y * Fn.y;
+    if (wf) f[g] = xn;
+    lds[ad(l, e)] = cmulc(pn, ch[k]);
   }
   if (want_norm) flush_norm(nrm, st.norms + slot);
   if (want_fref) flush_norm(nrmF, st.fref + slot);
@@ -295,18 +281,11 @@ __global__ __launch_bounds__(NT) void blue_rows_update(StateArgs st, Tables tb, 
   }
   __syncthreads();
   blue_convolve<L, NL, NT>(lds, s_tw, s_b);
-  float2 cv3[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int i = threadIdx.x + k * NT;
     const int l = i / NMAX, e = i - l * NMAX;
-    if (i < NL * NMAX && e < n && l < nr) cv3[k] = p.chirp[e];
-  }
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const int i = threadIdx.x + k * NT;
-    const int l = i / NMAX, e = i - l * NMAX;
-    if (i < NL * NMAX && e < n && l < nr) spec[(long long)l * n + e] = cmulc(lds[ad(l, e)], cv3[k]);
+    if (i < NL * NMAX && e < n && l < nr) spec[l * n + e] = cmulc(lds[ad(l, e)], ch[k]);
   }
 }
 
