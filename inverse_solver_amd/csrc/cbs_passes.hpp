@@ -107,6 +107,27 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
       if (i < E2) reg[k] = load_stream(spec4 + i, args.nt);
     }
   }
+  const bool rf = reads_f(upd, fwd), rF = reads_F(upd);
+  const float2* __restrict__ V = args.st.V + zoff;
+  float2* __restrict__ F = args.st.field + roff + zoff;
+  float2* __restrict__ f = args.st.fstate + roff + zoff;
+  // small tiles (few blocks, e.g. a single rhs): the pointwise operands are requested with
+  // the tile, so their latency overlaps the inverse transforms instead of following them
+  constexpr bool PF = L <= 5;
+  constexpr int LP = PF ? L : 1;
+  float4 pv[LP], px[LP], pF[LP];
+  if constexpr (PF) {
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+      const int i = tid + k * NT;
+      if (i < E2) {
+        const unsigned g = pair_off(i);
+        pv[k] = *byte_at<const float4>(V, g);
+        if (rf) px[k] = load_stream(byte_at<const float4>(f, g), args.nt);
+        if (rF) pF[k] = load_stream(byte_at<const float4>(F, g), args.nt);
+      }
+    }
+  }
   for (int i = tid; i < NX; i += NT) {
     s_twx[i] = tb.tw_x[i];
   }
@@ -138,11 +159,7 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
   }
 
   // pointwise update, CH pairs per thread at a time
-  const bool rf = reads_f(upd, fwd), rF = reads_F(upd);
   const bool wF = writes_F(upd), wf = fwd == FWD_GAMMA;
-  const float2* __restrict__ V = args.st.V + zoff;
-  float2* __restrict__ F = args.st.field + roff + zoff;
-  float2* __restrict__ f = args.st.fstate + roff + zoff;
   const bool want_norm = args.st.norms != nullptr;
   // |F_1 + F_2|^2 for the convergence test, formed by the jj = 2 launch
   const bool want_fref = !STEADY && args.st.fref != nullptr && upd == UPD_VQ_ADD;
@@ -163,12 +180,18 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
         const unsigned y1 = e / NX, x = e - y1 * NX;
         const unsigned g = pair_off(i);
         gi[k] = g;
-        vv[k] = *byte_at<const float4>(V, g);
         // per-axis x ramp and mask pairs (read through the cache: the LDS holds three tiles per CU)
         rx[k] = *byte_at<const float4>(tb.ramp_x, x * 8u);
         mx[k] = *byte_at<const float2>(tb.mask_x, x * 4u);
-        if (rf) xx[k] = load_stream(byte_at<const float4>(f, g), args.nt);
-        if (rF) FF[k] = load_stream(byte_at<const float4>(F, g), args.nt);
+        if constexpr (PF) {
+          vv[k] = pv[k0 + k];
+          if (rf) xx[k] = px[k0 + k];
+          if (rF) FF[k] = pF[k0 + k];
+        } else {
+          vv[k] = *byte_at<const float4>(V, g);
+          if (rf) xx[k] = load_stream(byte_at<const float4>(f, g), args.nt);
+          if (rF) FF[k] = load_stream(byte_at<const float4>(F, g), args.nt);
+        }
       }
     }
 #pragma unroll
