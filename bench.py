@@ -377,6 +377,74 @@ def hbm_ceiling(stream, log, gib=2, reps=10):
     return res
 
 
+def workload_demo(args):
+    """The reference's own demo geometry (SIMULATION_BEAD.m:11-19, 48): a 161x161x61 RI
+    grid at 0.532/4/1.2 um holding a 5-um bead (n 1.4609 in 1.336).  With the default
+    boundary_thickness 6 it is a 183x183x83 simulation grid (183 = 3 x 61 and the prime
+    83: Bluestein transforms on every axis); the backward solver's [2 2 4] gives
+    169x169x75 (169 = 13^2 on run-time Stockham stages, 75 compiled).  12 circle angles,
+    20 iterations, the batched per-axis form, inputs and outputs in HBM.  Beside it: the
+    reference's GPU loop restated op for op on rocFFT (one angle), and the FP64 oracle
+    at 6 iterations on one angle."""
+    import torch
+    torch.zeros(1, device='cuda:0')                 # torch's HIP context first (reference-form loop)
+    from inverse_solver_amd import FORWARD_SOLVER_CONVERGENT_BORN, get_default_parameters
+    from inverse_solver_amd import device as dv
+    from inverse_solver_amd.phantoms import bead_ri, incident_plane_waves
+    from oracle.cbs import ForwardSolverConvergentBorn as Oracle
+    ri, res, A, M = [161, 161, 61], 0.532 / 4 / 1.2, 12, 20
+    RI = bead_ri(ri, 5.0 / res, 1.4609, 1.336)
+
+    def prm(bt, iters):
+        return get_default_parameters(dict(
+            size=ri, wavelength=0.532, NA=1.2, RI_bg=1.336, resolution=[res] * 3, vector_simulation=False,
+            use_abbe_sine=True, iterations_number=iters, boundary_thickness=bt, boundary_sharpness=1,
+            acyclic=True, return_transmission=True, return_reflection=False, return_3D=True))
+    u = incident_plane_waves(ri, A, 1.2, 0.532, 1.336, [res] * 3)
+    u_host = np.ascontiguousarray(np.transpose(u, (3, 2, 1, 0))).astype(np.complex64)
+    d_in = dv.DeviceArray.from_host(u_host)
+    d_out = dv.DeviceArray(u_host.shape, np.complex64)
+    out, rates = {}, {}
+    for name, bt in (('forward', [6, 6, 6]), ('backward', [2, 2, 4])):
+        s = FORWARD_SOLVER_CONVERGENT_BORN(prm(bt, M))
+        s.set_RI(RI)
+        s.solve_incident_device(d_in.ptr, A, d_out.ptr)
+        dv.synchronize()
+        times = []
+        for _ in range(max(args.steps, 3)):
+            t0 = time.perf_counter()
+            s.solve_incident_device(d_in.ptr, A, d_out.ptr)
+            dv.synchronize()
+            times.append((time.perf_counter() - t0) * 1e3)
+        ms = float(np.median(times))
+        sim = list(s.simulation_size)
+        rates[name] = {'sim_grid': sim, 'value': round(float(np.prod(sim)) * M * A / (ms / 1e3) / 1e9, 3),
+                       'unit': 'Gvoxel-iter/s', 'ms_per_solve': round(ms, 3), 'plan': s.plan()['form']}
+        if name == 'forward':
+            fwd_ms, fwd_sim = ms, sim
+            ref_ms, _ = reference_form_gpu(s, u[..., 0], M, lambda *a: print(*a, file=sys.stderr, flush=True))
+            out['reference_form_single_angle'] = {
+                'value': round(float(np.prod(sim)) * M / (ref_ms / 1e3) / 1e9, 3), 'unit': 'Gvoxel-iter/s',
+                'ms_per_angle': round(ref_ms, 3)}
+        s.close()
+    # accuracy: one angle, 6 iterations, against the FP64 oracle
+    p6 = prm([6, 6, 6], 6)
+    s = FORWARD_SOLVER_CONVERGENT_BORN(p6)
+    s.set_RI(RI)
+    got = s.solve_incident(u[..., :1])[..., 0]
+    s.close()
+    o = Oracle(p6, io_kernels=False, workers=os.cpu_count() or 1)
+    o.set_RI(RI)
+    out['accuracy'] = {'rel_l2_err_vs_fp64_oracle': _rel(got, o.born_loop(u[..., 0])), 'angles': 1, 'iterations': 6,
+                       'grid': fwd_sim}
+    d_in.free()
+    d_out.free()
+    out['geometries'] = rates
+    return _line('Gvoxel-iter/s (reference demo geometry, per-axis form)', rates['forward']['value'], 'Gvoxel-iter/s',
+                 'SIMULATION_BEAD: 161x161x61 RI + default boundary = 183x183x83 sim, 5-um bead, %d angles, '
+                 '%d iterations' % (A, M), fwd_ms, args, out)
+
+
 def workload_ceiling(args):
     """The practical HBM ceilings alone (hbm_ceiling on the null stream): what the
     passes' achieved GB/s are compared against."""
@@ -616,7 +684,7 @@ def main():
     ap.add_argument('--copy-probe-mb', type=int, default=0,
                     help='run one device-to-device copy of this many MiB first (calibrates HBM byte counters)')
     ap.add_argument('--workload', default='batched',
-                    choices=['batched', 'c1', 'inverse', 'serving', 'large', 'vector', 'retrieval', 'ceiling'],
+                    choices=['batched', 'c1', 'inverse', 'serving', 'large', 'vector', 'retrieval', 'ceiling', 'demo'],
                     help='batched: the headline 49-angle solve (default); the others are the secondary '
                          'configurations, each printing its own JSON line')
     ap.add_argument('--requests', type=int, default=200, help='serving: number of requests')
@@ -635,7 +703,8 @@ def main():
     if args.workload != 'batched':
         out = {'c1': workload_c1, 'inverse': workload_inverse, 'serving': workload_serving,
                'large': workload_large, 'vector': workload_vector,
-               'retrieval': workload_retrieval, 'ceiling': workload_ceiling}[args.workload](args)
+               'retrieval': workload_retrieval, 'ceiling': workload_ceiling,
+               'demo': workload_demo}[args.workload](args)
         print(json.dumps(out), flush=True)
         return
 
