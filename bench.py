@@ -302,7 +302,7 @@ def workload_large(args):
     # the per-axis plain form on the same grid, 1 angle x 10 iterations, for comparison
     pp = workload_params(ri, 10, 1)
     pp.update(boundary_thickness=[16, 16, 16])
-    spn = FORWARD_SOLVER_CONVERGENT_BORN(pp, options=dict(form=1, use_graph=0))
+    spn = FORWARD_SOLVER_CONVERGENT_BORN(pp, options=dict(form=3, use_graph=0))
     spn.set_RI(RI)
     spn.solve_incident_device(d_in.ptr, 1, d_out.ptr)
     dv.synchronize()
@@ -882,7 +882,7 @@ def main():
 
     if not args.no_plain:
         # plain single-angle form (unfused per-axis kernels, batch 1) on the same inputs
-        sp_ = FORWARD_SOLVER_CONVERGENT_BORN(p, options=dict(form=1, use_graph=0))
+        sp_ = FORWARD_SOLVER_CONVERGENT_BORN(p, options=dict(form=3, use_graph=0))
         sp_.set_RI(RI)
         d1 = dv.DeviceArray.from_host(u_host[:1])
         d1o = dv.DeviceArray(u_host[:1].shape, np.complex64)

@@ -41,7 +41,7 @@ enum {
 };
 
 enum { CBS_MEM_HOST = 0, CBS_MEM_DEVICE = 1 };
-enum { CBS_FORM_FUSED = 0, CBS_FORM_PLAIN = 1, CBS_FORM_ROWS = 2 };
+enum { CBS_FORM_FUSED = 0, CBS_FORM_PLAIN = 1, CBS_FORM_ROWS = 2, CBS_FORM_AXES = 3 };
 
 /* Geometry of a solver instance.
  * Replaces the struct of CudaBorn action 0 (main.cpp:57-101: simulation_size,
@@ -86,9 +86,13 @@ typedef struct {
   int32_t form;         /* CBS_FORM_FUSED (two passes per iteration; grids without such a plan fall
                            back to CBS_FORM_ROWS, then CBS_FORM_PLAIN), CBS_FORM_ROWS (four passes:
                            row blocks in real space, y transforms, z transforms with the Green's
-                           multiply) or CBS_FORM_PLAIN (per-axis transforms, 8 launches per
-                           iteration for the whole batch; grids whose line lengths have no compiled
-                           kernel run it on run-time-planned line FFTs) */
+                           multiply), CBS_FORM_PLAIN (per-axis transforms over the whole batch:
+                           for one field component the x inverse, the pointwise update and the x
+                           forward run as one launch and the z forward, the Green's multiply and
+                           the z inverse as another, 4 launches per iteration; grids whose line
+                           lengths have no compiled kernel run it on run-time-planned line FFTs)
+                           or CBS_FORM_AXES (the per-axis form with every transform, the Green's
+                           multiply and the update in a launch of its own: 8 per iteration) */
   int32_t split_a;      /* fused form: force ny = a*b split (0 = planner's choice) */
   int32_t tile_t;       /* fused form: force the k-space x tile (0 = planner's choice) */
   int32_t max_batch;    /* right-hand sides per launch (0 = all of the call) */

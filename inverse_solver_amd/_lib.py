@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, 'libcbs_hip.so')
 
 CBS_OK, CBS_ERR_ARG, CBS_ERR_HANDLE, CBS_ERR_HIP, CBS_ERR_UNSUPPORTED, CBS_ERR_STATE, CBS_ERR_MEMORY = range(7)
 CBS_MEM_HOST, CBS_MEM_DEVICE = 0, 1
-CBS_FORM_FUSED, CBS_FORM_PLAIN, CBS_FORM_ROWS = 0, 1, 2
+CBS_FORM_FUSED, CBS_FORM_PLAIN, CBS_FORM_ROWS, CBS_FORM_AXES = 0, 1, 2, 3
 
 EXPORTED = [
     'cbs_create', 'cbs_destroy', 'cbs_set_options', 'cbs_get_options', 'cbs_set_potential', 'cbs_potential_max',

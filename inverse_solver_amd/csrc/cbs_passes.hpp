@@ -630,6 +630,156 @@ void pass_kspace_seq3(PassKArgs args) {
 // Unfused single-rhs iteration: per-axis FFT launches on a natural-layout
 // spectrum, a Green's multiply launch and a pointwise update launch.
 
+// Per-axis form of a single-component run on a compiled x length N: the inverse x
+// transform of the spectrum rows, the pointwise update of update_plain (born_step) and the
+// forward x transform of the next p in one launch.  Block (bx, slot) takes rows bx NL ..
+// of slot `slot` (ny nz rows of N per slot, contiguous).  upd = UPD_NONE skips the inverse
+// transform, fwd = FWD_NONE the forward one; a slot the convergence test has stopped
+// leaves Field, f and the spectrum alone after the flush of a deferred increment.  Every
+// global operand of a thread's elements is requested before the first transform.
+template <int N, int NL, int NT>
+__global__ __launch_bounds__(NT) void line_rows_update(StateArgs st, Tables tb, Consts c, const float2* tw, int upd,
+                                                       int fwd, int jj_inv, int jj_fwd, ConvArgs cv) {
+  constexpr int PR = row_pitch(N);
+  constexpr int K = (NL * N + NT - 1) / NT;
+  __shared__ float2 lds[NL * PR];
+  const int slot = blockIdx.y;
+  const int rows = c.ny * c.nz;                          // < 2^31: a slot's voxels fit a launch
+  const int r0 = blockIdx.x * NL;
+  const int nr = rows - r0 < NL ? rows - r0 : NL;
+  const long long ro = (long long)slot * st.rhs_stride + (long long)r0 * N;
+  float2* __restrict__ spec = st.spec + ro;
+  float2* __restrict__ F = st.field + ro;
+  float2* __restrict__ f = st.fstate + ro;
+  const float2* __restrict__ V = st.V + (long long)r0 * N;
+  if (cv.norms && jj_inv > 0) {
+    const int rr = slot / cv.P;
+    if (conv_stopped(cv, jj_inv - 1, rr)) {
+      if (!conv_stopped(cv, jj_inv - 2, rr) && upd_kind(jj_inv - 1, cv.M) == UPD_DEFER)
+        for (int i = threadIdx.x; i < nr * N; i += NT) F[i] = cadd(F[i], f[i]);
+      return;
+    }
+  }
+  auto row = [](int l, int e) { return l * PR + e; };
+  const bool rq = upd != UPD_NONE;
+  const bool rf = reads_f(upd, fwd), rF = reads_F(upd);
+  float2 xq[K], vv[K], xo[K], Fo[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = threadIdx.x + k * NT;
+    xq[k] = vv[k] = xo[k] = Fo[k] = make_float2(0.f, 0.f);
+    if (i < nr * N) {
+      if (rq) xq[k] = spec[i];
+      vv[k] = V[i];
+      if (rf) xo[k] = f[i];
+      if (rF) Fo[k] = F[i];
+    }
+  }
+  if (rq) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = threadIdx.x + k * NT;
+      const int l = i / N, e = i - l * N;
+      if (i < NL * N) lds[row(l, e)] = xq[k];
+    }
+    __syncthreads();
+    line_fft<N, true, NL, NT>(lds, row, tw, threadIdx.x);
+  }
+  const bool wF = writes_F(upd), wf = fwd == FWD_GAMMA;
+  const bool want_norm = st.norms != nullptr;
+  const bool want_fref = st.fref != nullptr && upd == UPD_VQ_ADD;
+  const float sg_inv = sign_inv(jj_inv ? flip_of(jj_inv, st.parity, slot) : 0);
+  const float sg_fwd = sign_fwd(jj_fwd ? flip_of(jj_fwd, st.parity, slot) : 0);
+  double nrm = 0.0, nrmF = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = threadIdx.x + k * NT;
+    const int l = i / N, e = i - l * N;
+    if (i >= NL * N) continue;
+    if (l >= nr) {
+      lds[row(l, e)] = make_float2(0.f, 0.f);
+      continue;
+    }
+    const int rw = r0 + l;
+    const int z = rw / c.ny, y = rw - z * c.ny;
+    const float2 r = cmul(cmul(tb.ramp_x[e], tb.ramp_y[y]), tb.ramp_z[z]);
+    const float m = tb.mask_x[e] * tb.mask_y[y] * tb.mask_z[z];
+    const float2 q = rq ? lds[row(l, e)] : make_float2(0.f, 0.f);
+    float2 Fn, xn;
+    const float2 pn = born_step(upd, fwd, q, r, m, vv[k], xo[k], Fo[k], c.inv_eps, sg_inv, sg_fwd, Fn, xn,
+                                want_norm, nrm);
+    if (wF) F[i] = Fn;
+    if (want_fref) nrmF += (double)Fn.x * Fn.x + (double)Fn.y * Fn.y;
+    if (wf) f[i] = xn;
+    lds[row(l, e)] = pn;
+  }
+  if (want_norm) flush_norm(nrm, st.norms + slot);
+  if (want_fref) flush_norm(nrmF, st.fref + slot);
+  if (fwd == FWD_NONE) return;
+  __syncthreads();
+  line_fft<N, false, NL, NT>(lds, row, tw, threadIdx.x);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = threadIdx.x + k * NT;
+    const int l = i / N, e = i - l * N;
+    if (i < nr * N) spec[i] = lds[row(l, e)];
+  }
+}
+
+// Per-axis form of a single-component run on a compiled z length N: the forward z
+// transform, the Green's multiply of green_plain (at the transform's output positions, the
+// order the per-axis tables are built in) and the inverse z transform in one launch.
+// Tiles as fft_strided: T consecutive x positions of z lines, outer set o = y + ny slot.
+template <int N, int T, int NT>
+__global__ __launch_bounds__(NT) void line_strided_green(float2* data, Tables tb, Consts c, const float2* tw,
+                                                         const uint8_t* __restrict__ parity, int jj, ConvArgs cv) {
+  constexpr int K = (N * T + NT - 1) / NT;
+  __shared__ float2 lds[N * T];
+  __shared__ float2 s_tw[N];
+  const int ntile = (c.nx + T - 1) / T;
+  const int xt = blockIdx.x % ntile;
+  const int o = blockIdx.x / ntile;
+  const int y = o % c.ny, slot = o / c.ny;      // the grid holds ceil(nx / T) ny slots blocks
+  if (cv.norms && conv_stopped(cv, jj - 1, slot)) return;
+  const int flip = flip_of(jj, parity, slot);
+  const int x0 = xt * T;
+  const int tv = c.nx - x0 < T ? c.nx - x0 : T;
+  const long long es = (long long)c.nx * c.ny;
+  float2* base = data + (long long)slot * c.nvox + (long long)y * c.nx + x0;
+  auto ad = [](int l, int e) { return e * T + l; };
+  float2 xv[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = threadIdx.x + k * NT;
+    const int e = i / T, t = i - e * T;
+    xv[k] = make_float2(0.f, 0.f);
+    if (i < N * T && t < tv) xv[k] = base[e * es + t];
+  }
+  for (int i = threadIdx.x; i < N; i += NT) s_tw[i] = tw[i];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = threadIdx.x + k * NT;
+    if (i < N * T) lds[i] = xv[k];              // e T + t == i
+  }
+  __syncthreads();
+  const float* gx = tb.green_x + flip * c.nx + x0;
+  const float* gz = tb.green_z + flip * c.nz;
+  const float gy = tb.green_y[flip * c.ny + y] - c.km2;
+  const float eps2 = c.eps * c.eps, inv_n = c.inv_n, eps = c.eps;
+  auto green = [&](int l, int pz0, int pz1, bool hi, float2 v) {
+    const float d = gx[l < tv ? l : 0] + gy + pick2(gz, pz0, pz1, hi);
+    const float s = inv_n * __builtin_amdgcn_rcpf(fmaf(d, d, eps2));   // hardware reciprocal (1 ulp)
+    return cmul(v, make_float2(d * s, eps * s));
+  };
+  line_fft_mul_ifft<N, T, NT>(lds, ad, s_tw, threadIdx.x, green);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = threadIdx.x + k * NT;
+    const int e = i / T, t = i - e * T;
+    if (i < N * T && t < tv) base[e * es + t] = lds[i];
+  }
+}
+
 // FFT along x for R rows per block (rows are contiguous nx-vectors).
 template <int NX, int ROWS, int NT>
 __global__ __launch_bounds__(NT) void fft_rows(float2* data, long long nrows, const float2* tw, int inverse) {

@@ -49,6 +49,14 @@ const PassRInfo* find_pass_real(int nx, int a);
 const PassKInfo* find_pass_kspace(int b, int nz, int t, int pole);
 const RowsInfo* find_fft_rows(int n);
 const StridedInfo* find_fft_strided(int n, int t);
+// per-axis form, one field component, compiled line lengths (lines_fused_inst.hip):
+// x inverse + pointwise update + x forward, and z forward + Green's multiply + z inverse
+using RowsUpdateFn = void (*)(StateArgs, Tables, Consts, const float2*, int, int, int, int, ConvArgs);
+using StridedGreenFn = void (*)(float2*, Tables, Consts, const float2*, const uint8_t*, int, ConvArgs);
+struct RowsUpdateInfo { RowsUpdateFn fn; int n, rows, nt; };
+struct StridedGreenInfo { StridedGreenFn fn; int n, t, nt; };
+const RowsUpdateInfo* find_rows_update(int n);
+const StridedGreenInfo* find_strided_green(int n);
 // enumerate for planning
 int list_pass_real(const PassRInfo** out);
 int list_pass_kspace(const PassKInfo** out);

@@ -859,10 +859,16 @@ struct Solver {
     // single-component runs whose x (z) lines are Bluestein-transformed on a compiled
     // length: the x inverse, the update and the x forward (the z forward, the Green's
     // multiply and the z inverse) run as one launch each
-    const BlueInfo* bx = (P == 1 && plan.rt[0]) ? rtp[0].bi : nullptr;
-    const BlueInfo* bz = (P == 1 && plan.rt[2]) ? rtp[2].bi : nullptr;
+    // (on a compiled line length, the compiled fused passes of lines_fused_inst.hip)
+    const bool fuse = P == 1 && plan.form == CBS_FORM_PLAIN;   // CBS_FORM_AXES: every pass on its own
+    const BlueInfo* bx = (fuse && plan.rt[0]) ? rtp[0].bi : nullptr;
+    const BlueInfo* bz = (fuse && plan.rt[2]) ? rtp[2].bi : nullptr;
+    const RowsUpdateInfo* cx = (fuse && !plan.rt[0]) ? find_rows_update(nx) : nullptr;
+    const StridedGreenInfo* cz = (fuse && !plan.rt[2]) ? find_strided_green(nz) : nullptr;
+    const long long rows = (long long)ny * nz;
+    require(rows < (1ll << 31), CBS_ERR_UNSUPPORTED, "grid too large for the per-axis passes");
     auto xpass = [&](int upd, int fwd, int jj_inv, int jj_fwd) {   // (x inverse,) update(, x forward)
-      if (!bx) {
+      if (!bx && !cx) {
         if (upd != UPD_NONE) fft1(0, 1);
         update(upd, fwd, jj_inv, jj_fwd);
         if (fwd != FWD_NONE) fft1(0, 0);
@@ -871,12 +877,23 @@ struct Solver {
       StateArgs s = state_args(1);
       s.norms = norm_row(jj_inv, nv);
       if (conv_tol > 0.0 && upd == UPD_VQ_ADD) s.fref = ws.fref;
-      const long long rows = (long long)ny * nz;
-      const dim3 g((unsigned)((rows + bx->nl - 1) / bx->nl), (unsigned)nv);
-      timed(2, [&] { hipLaunchKernelGGL(bx->rows_update, g, dim3(bx->nt), 0, stream, s, tb, c, rtp[0].ba, upd, fwd,
-                                        jj_inv, jj_fwd, cv); });
+      const int nl = bx ? bx->nl : cx->rows;
+      const dim3 g((unsigned)((rows + nl - 1) / nl), (unsigned)nv);
+      if (bx)
+        timed(2, [&] { hipLaunchKernelGGL(bx->rows_update, g, dim3(bx->nt), 0, stream, s, tb, c, rtp[0].ba, upd, fwd,
+                                          jj_inv, jj_fwd, cv); });
+      else
+        timed(2, [&] { hipLaunchKernelGGL(cx->fn, g, dim3(cx->nt), 0, stream, s, tb, c, tabs.tw_x.p, upd, fwd,
+                                          jj_inv, jj_fwd, cv); });
     };
     auto zpass = [&](int jj) {   // z forward, Green's multiply, z inverse
+      if (cz) {
+        const long long nblk = (long long)((nx + cz->t - 1) / cz->t) * ny * nv;
+        require(nblk < (1ll << 31), CBS_ERR_UNSUPPORTED, "z transform batch too large for one launch");
+        timed(2, [&] { hipLaunchKernelGGL(cz->fn, dim3((unsigned)nblk), dim3(cz->nt), 0, stream, ws.spec, tb, c,
+                                          tabs.tw_z.p, ws.parity, jj, cv); });
+        return;
+      }
       if (!bz) {
         fft1(2, 0);
         if (P == 1)
@@ -1188,7 +1205,7 @@ struct Solver {
 
   void run_loop(int nb, int M) {
     if (plan.form == CBS_FORM_ROWS) run_rows(nb, M);
-    else if (plan.form == CBS_FORM_PLAIN) run_plain(nb, M);
+    else if (plan.form == CBS_FORM_PLAIN || plan.form == CBS_FORM_AXES) run_plain(nb, M);
     else run_fused(nb, M);
   }
 
@@ -2005,7 +2022,8 @@ int cbs_set_options(cbs_handle h, const cbs_options* opt) {
   return guarded([&] {
     Solver* s = lookup(h);
     require(opt != nullptr, CBS_ERR_ARG, "null options");
-    require(opt->form == CBS_FORM_FUSED || opt->form == CBS_FORM_PLAIN || opt->form == CBS_FORM_ROWS, CBS_ERR_ARG,
+    require(opt->form == CBS_FORM_FUSED || opt->form == CBS_FORM_PLAIN || opt->form == CBS_FORM_ROWS ||
+                opt->form == CBS_FORM_AXES, CBS_ERR_ARG,
             "unknown form");
     require(opt->max_batch >= 0 && opt->split_a >= 0 && opt->tile_t >= 0, CBS_ERR_ARG, "negative option");
     require(opt->block_order >= 0 && opt->block_order <= 3, CBS_ERR_ARG, "block_order must be 0..3");

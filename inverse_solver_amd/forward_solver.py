@@ -126,7 +126,8 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
         check(lib.cbs_get_plan(self._h, C.byref(f), C.byref(a), C.byref(b), C.byref(t)))
         o = _lib.Options()
         check(lib.cbs_get_options(self._h, C.byref(o)))
-        form = {_lib.CBS_FORM_PLAIN: 'plain', _lib.CBS_FORM_ROWS: 'rows'}.get(f.value, 'fused')
+        form = {_lib.CBS_FORM_PLAIN: 'plain', _lib.CBS_FORM_ROWS: 'rows', _lib.CBS_FORM_AXES: 'axes'}.get(f.value,
+                                                                                                  'fused')
         return {'form': form, 'a': a.value, 'b': b.value,
                 'tile_t': t.value, 'threads_real': o.threads_real, 'threads_kspace': o.threads_kspace,
                 'block_order': o.block_order}

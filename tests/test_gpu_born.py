@@ -31,6 +31,8 @@ CASES = [
     ((16, 16, 16), 20, dict(form=0, split_a=6)),
     ((16, 12, 10), 12, dict(form=0)),
     ((16, 12, 10), 12, dict(form=1)),
+    ((16, 16, 16), 20, dict(form=3)),            # per-axis form, every pass in its own launch
+    ((16, 12, 10), 12, dict(form=3)),
     ((17, 17, 17), 10, dict(form=0)),
     ((17, 17, 19), 10, dict(form=0)),
     ((16, 16, 16), 20, dict(form=2)),            # four-pass form (row blocks, y and z passes)
@@ -93,7 +95,7 @@ def test_block_orders_give_identical_results():
         np.testing.assert_array_equal(o, outs[0])
 
 
-@pytest.mark.parametrize('form', [0, 1, 2])
+@pytest.mark.parametrize('form', [0, 1, 2, 3])
 def test_batch_equals_individual_and_oracle(form):
     ri = (16, 16, 16)
     p = params(ri, iterations=16)
@@ -111,7 +113,7 @@ def test_batch_equals_individual_and_oracle(form):
     s.close()
 
 
-@pytest.mark.parametrize('form', [0, 1, 2])
+@pytest.mark.parametrize('form', [0, 1, 2, 3])
 def test_odd_bornmax_alternation_persists_across_rhs(form):
     """With odd Bornmax the reference leaves Greenp flipped after each angle
     (FORWARD_SOLVER_CONVERGENT_BORN.m:579-585), so consecutive angles start on
@@ -129,7 +131,7 @@ def test_odd_bornmax_alternation_persists_across_rhs(form):
     s.close()
 
 
-@pytest.mark.parametrize('form', [0, 1, 2])
+@pytest.mark.parametrize('form', [0, 1, 2, 3])
 @pytest.mark.parametrize('iters', [1, 2, 3, 4, 5, 6, 7])
 def test_short_series(iters, form):
     """Series lengths around the special first iterations and the paired Field
@@ -243,7 +245,7 @@ def test_host_buffer_pipeline_equals_device_path():
     dev.close()
 
 
-@pytest.mark.parametrize('form', [0, 1, 2])
+@pytest.mark.parametrize('form', [0, 1, 2, 3])
 def test_repeat_set_ri_on_one_handle(form):
     """A new potential on the same geometry keeps the handle's tables (they depend on the
     plan and the geometry scalars only); solves after each set_RI equal fresh handles', and

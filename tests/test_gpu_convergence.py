@@ -59,7 +59,7 @@ def _tol_for(r, even):
     raise AssertionError('no clean stop found')
 
 
-@pytest.mark.parametrize('form', [0, 2, 1])
+@pytest.mark.parametrize('form', [0, 2, 1, 3])
 @pytest.mark.parametrize('even', [True, False])
 def test_stop_iteration_and_fields_match_oracle(form, even):
     _stop_case(_setup(), form, even)
