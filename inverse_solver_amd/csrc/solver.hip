@@ -952,9 +952,10 @@ struct Solver {
     return b;
   }
   int batch_width(uint32_t n_rhs) const {
-    int nb = (int)n_rhs;
+    // the per-axis form's launches put the component slots on grid y (at most 65535)
+    long long nb = std::min<long long>((long long)n_rhs, 65535 / P);
     if (opt.max_batch > 0 && opt.max_batch < nb) nb = opt.max_batch;
-    return nb;
+    return (int)nb;
   }
 
   // Born loop for a batch whose incident fields sit at d_uin (device, RI grid).
