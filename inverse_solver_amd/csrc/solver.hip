@@ -1465,11 +1465,12 @@ struct Solver {
     double best = 1e30;
     Plan best_plan;
     int best_order = 0;
+    int best_w = nb;
     opt.block_order = 0;
-    auto time_plan = [&]() {
-      born_batch(src.p, dst.p, nb, 0);      // warm-up (and graph capture)
+    auto time_plan = [&](int w) {
+      born_batch(src.p, dst.p, w, 0);       // warm-up
       HIP_CHECK(hipEventRecord(e0, stream));
-      born_batch(src.p, dst.p, nb, 0);
+      born_batch(src.p, dst.p, w, 0);
       HIP_CHECK(hipEventRecord(e1, stream));
       HIP_CHECK(hipEventSynchronize(e1));
       float ms = 0.f;
@@ -1489,7 +1490,7 @@ struct Solver {
         p.t = cd.second->t;
         plan = p;
         build_tables();
-        timed_plans.emplace_back(time_plan(), p);
+        timed_plans.emplace_back(time_plan(nb), p);
       }
       std::sort(timed_plans.begin(), timed_plans.end(),
                 [](const std::pair<double, Plan>& x, const std::pair<double, Plan>& y) { return x.first < y.first; });
@@ -1499,7 +1500,7 @@ struct Solver {
         build_tables();
         for (int ord = 0; ord < 4; ++ord) {
           opt.block_order = ord;
-          const double ms = ord == 0 ? timed_plans[i].first : time_plan();
+          const double ms = ord == 0 ? timed_plans[i].first : time_plan(nb);
           if (ms < best) {
             best = ms;
             best_plan = plan;
@@ -1507,6 +1508,21 @@ struct Solver {
           }
         }
         opt.block_order = 0;
+      }
+      // batch width: the chosen plan with nb / 2, nb / 4, nb / 8, 2 and 1 rhs per launch
+      // (narrower batches keep more of the state in the caches between launches but give
+      // the k-space pass fewer blocks); a narrower width is kept when its predicted time
+      // for nb rhs is at least 3 % below the full batch's
+      plan = best_plan;
+      build_tables();
+      opt.block_order = best_order;
+      for (int w : {(nb + 1) / 2, (nb + 3) / 4, (nb + 7) / 8, 2, 1}) {
+        if (w >= best_w || w < 1) continue;
+        const double total = time_plan(w) * (double)((nb + w - 1) / w);
+        if (total < 0.97 * best) {
+          best = total;
+          best_w = w;
+        }
       }
     } catch (...) {
       prm.Bornmax = saved_iters;
@@ -1530,6 +1546,7 @@ struct Solver {
     opt.threads_real = best_plan.pr->nt;
     opt.threads_kspace = best_plan.pk->nt;
     opt.block_order = best_order;
+    if (best_w < nb) opt.max_batch = best_w;
     opt.time_kernels = saved_time;
     opt.track_norms = saved_norm;
     conv_tol = saved_tol;

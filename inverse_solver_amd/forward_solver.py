@@ -116,7 +116,8 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
         check(lib.cbs_set_options(self._h, C.byref(o)))
 
     def autotune(self, n_rhs, iterations=6):
-        """Measure every fused kernel plan on the current potential and keep the fastest."""
+        """Measure every fused kernel plan on the current potential and keep the fastest, then
+        the batch width (rhs per launch) for n_rhs right-hand sides; returns ms per solve."""
         ms = C.c_double()
         check(lib.cbs_autotune(self._h, int(n_rhs), int(iterations), C.byref(ms)))
         return ms.value
@@ -130,7 +131,7 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
                                                                                                   'fused')
         return {'form': form, 'a': a.value, 'b': b.value,
                 'tile_t': t.value, 'threads_real': o.threads_real, 'threads_kspace': o.threads_kspace,
-                'block_order': o.block_order}
+                'block_order': o.block_order, 'max_batch': o.max_batch}
 
     # ---------------------------------------------------------------- potential (:180-396)
     def set_RI(self, RI, flip_conj=False):

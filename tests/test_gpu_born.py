@@ -113,6 +113,31 @@ def test_batch_equals_individual_and_oracle(form):
     s.close()
 
 
+def test_autotuned_plan_and_batch_width_match_oracle():
+    """cbs_autotune times every fused kernel pair, the block orders of the fastest and
+    then the batch width; whatever it keeps, the solve of the tuned handle equals the
+    oracle (odd Bornmax, so a split batch must also carry the flip parity across its
+    launches), and the chosen width is the full call or one of the narrower trials."""
+    ri = (16, 16, 16)
+    n = 5
+    p = params(ri, iterations=11)
+    RI = sphere_ri(ri, 0.6, n_s=1.40)
+    fields = [random_field(ri, 20 + k) for k in range(n)]
+    ref, _ = _oracle_loop(p, RI, fields)
+    s = _gpu()(p)
+    s.set_RI(RI)
+    ms = s.autotune(n, 4)
+    plan = s.plan()
+    print('autotuned', plan, '%.3f ms' % ms, flush=True)
+    assert ms > 0 and plan['form'] == 'fused'
+    assert plan['max_batch'] in (0, 1, 2, 3, 4)
+    s.set_RI(RI)                      # the flip state as at the oracle's start
+    got = s.solve_incident(np.stack(fields, -1))
+    for j in range(n):
+        assert rel(got[..., j], ref[..., j]) < TOL, j
+    s.close()
+
+
 @pytest.mark.parametrize('form', [0, 1, 2, 3])
 def test_odd_bornmax_alternation_persists_across_rhs(form):
     """With odd Bornmax the reference leaves Greenp flipped after each angle
