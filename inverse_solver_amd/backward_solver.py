@@ -155,12 +155,14 @@ class BACKWARD_SOLVER_MULTI:
             init_RI = init_solver.solve(input_field, output_field)
             if isinstance(init_RI, tuple):          # BACKWARD_SOLVER_RYTOV returns (RI, ORytov)
                 init_RI = init_RI[0]
-        RI = np.asarray(init_RI).astype(np.complex128)
+        # the iterates are kept x-fastest (Fortran order), the layout the device takes,
+        # so no gradient re-lays the RI grid out before its upload
+        RI = np.asfortranarray(np.asarray(init_RI).astype(np.complex128))
         n_m = p['RI_bg']
         alpha = 1.0 / p['step']
         t_np = 1.0
         u_n = RI ** 2 / n_m ** 2 - 1
-        x_n = u_n.copy()
+        x_n = u_n.copy(order='F')
         if RI.shape[2] % 2 == 0:
             raise ValueError('need to be odd size')                                # :117-119
         Vmin = (p['nmin'] ** 2 - p['kappamax'] ** 2) / n_m ** 2 - 1

@@ -7,9 +7,11 @@ Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:32-45, Codes/sub_code/update_struct.m:1-1
 from __future__ import annotations
 
 import copy
+import os
 import warnings
 
 import numpy as np
+import scipy.fft as sfft
 
 
 def update_struct(base, update, warn=True):
@@ -59,16 +61,27 @@ def pupil(size_xy, resolution, wavelength, NA, n_m):
     return na_circle, k3 / k0_nm, k3
 
 
+_FFT_WORKERS = max(1, min(8, os.cpu_count() or 1))
+
+
+def _centered(a, fn):
+    """fftshift(fn(ifftshift(a))) over the first two axes (result types as numpy.fft:
+    single precision stays single), on several threads when the array holds several 2D
+    transforms."""
+    a = np.asarray(a)
+    ax = (0, 1)
+    workers = _FFT_WORKERS if a.size >= 4 * a.shape[0] * a.shape[1] else 1
+    return np.fft.fftshift(fn(np.fft.ifftshift(a, axes=ax), axes=ax, workers=workers), axes=ax)
+
+
 def centered_fft2(a):
     """fftshift(fft2(ifftshift(a))) over the first two axes."""
-    ax = (0, 1)
-    return np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(a, axes=ax), axes=ax), axes=ax)
+    return _centered(a, sfft.fft2)
 
 
 def centered_ifft2(a):
     """fftshift(ifft2(ifftshift(a))) over the first two axes."""
-    ax = (0, 1)
-    return np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(a, axes=ax), axes=ax), axes=ax)
+    return _centered(a, sfft.ifft2)
 
 
 def polarisation_basis(size_xy, resolution, wavelength, n_m):
