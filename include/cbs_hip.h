@@ -97,9 +97,13 @@ typedef struct {
   int32_t tile_t;       /* fused form: force the k-space x tile (0 = planner's choice) */
   int32_t max_batch;    /* right-hand sides per launch (0 = all of the call) */
   int32_t track_norms;  /* 1: accumulate FP64 |f_n|^2 per rhs and iteration */
-  int32_t use_graph;    /* 1 (the default): capture the iteration loop of the fused and four-pass
-                           forms in a hipGraph on first use and replay it (not while time_kernels
-                           is set); 0: launch every kernel */
+  int32_t use_graph;    /* 1 (the default): capture the iteration loop in a hipGraph on first use
+                           and replay it (not while time_kernels is set); 0: launch every kernel.
+                           The library keeps its own device-wide operations (allocations, frees,
+                           stream and graph teardown) from overlapping a capture on another
+                           thread; a caller that frees device memory or synchronises the device
+                           from one thread while another thread's handle solves must hold off
+                           the same way (or set use_graph = 0) */
   int32_t time_kernels; /* 1: time each pass kind with HIP events (cbs_get_kernel_times) */
   int32_t threads_real;   /* fused form: force threads per block of the real-space pass (0 = planner) */
   int32_t threads_kspace; /* fused form: force threads per block of the k-space pass (0 = planner) */
@@ -213,6 +217,20 @@ int cbs_tv_prox_complex(cbs_handle h, const void* in, void* out, const uint32_t 
  * MATLAB; this keeps both fields in device memory. */
 int cbs_field_gradient(cbs_handle h, const void* u, const void* w, uint32_t n_rhs, const uint32_t dims[3], void* grad,
                        int mem);
+
+/* The outer FISTA iteration of BACKWARD_SOLVER_MULTI (:152-160) on device arrays of n
+ * values, x fastest (the reference runs it on gpuArrays in MATLAB), so the iterates, the
+ * gradient and the RI stay in device memory between gradients.  Both run on the
+ * handle's stream and return when their work is done.
+ * cbs_fista_prox_input: out = u - inv_alpha * grad (u, grad complex128), rounded to
+ * complex64 when cplx = 1 (the input of cbs_tv_prox_complex) or its real part to float32
+ * when cplx = 0 (cbs_tv_prox).
+ * cbs_fista_momentum: with s the proximal output (complex64 when cplx = 1, else float32),
+ * u = s + c (s - x), x = s and ri = n_m sqrt(u + 1) (principal branch), all complex128. */
+int cbs_fista_prox_input(cbs_handle h, const void* u, const void* grad, double inv_alpha, uint64_t n, int cplx,
+                         void* out);
+int cbs_fista_momentum(cbs_handle h, const void* s, int cplx, void* x, void* u, void* ri, uint64_t n, double c,
+                       double n_m);
 
 /* Phase unwrapping for the Rytov initialiser of BACKWARD_SOLVER_MULTI (handle-free;
  * synchronous on the default stream of `device`).  Arrays are column-major n1 x n2

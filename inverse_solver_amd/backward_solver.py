@@ -16,7 +16,7 @@ import numpy as np
 
 from .forward_solver import FORWARD_SOLVER_CONVERGENT_BORN, get_default_parameters as fwd_defaults
 from .params import update_struct
-from .tv import tv_fista_inner, tv_fista_inner_device
+from .tv import tv_fista_inner
 
 
 def get_default_parameters(init=None):
@@ -106,17 +106,26 @@ class BACKWARD_SOLVER_MULTI:
         """Codes/BACKWARD_SOLVER_MULTI.m:30-54: forward solve, residual, adjoint solve on
         conj(flip_z(RI)) with conj(residual), gradient -i mean(conj(u) w).  Both 3D fields
         stay in device memory; cbs_field_gradient forms the gradient in FP64 there."""
-        import ctypes as C
-        from ._lib import check, lib, CBS_MEM_DEVICE
-        self._share_io()
         RI = np.asarray(RI)
-        inp = self.forward._inputs(input_field)
-        A = inp.shape[3]
-        du, dw, dg = self._fields(A)
         if list(RI.shape[:3]) != self.forward.expected_RI_size:
             raise ValueError('The refractiv index does not have the expected size : %d %d %d'
                              % tuple(self.forward.expected_RI_size))
         ri_dev, cplx = self._ri_device(RI.reshape(self.forward.expected_RI_size))
+        dg, err = self._gradient_device(ri_dev, cplx, input_field, output_field)
+        rx, ry, rz = self.forward.expected_RI_size
+        grad = dg.to_host().reshape(rz, ry, rx).transpose(2, 1, 0)
+        return grad, err
+
+    def _gradient_device(self, ri_dev, cplx, input_field, output_field):
+        """The gradient of get_gradiant_static for an RI grid already in device memory (FP64,
+        x fastest; complex when cplx): returns the device buffer holding the complex128
+        gradient (x fastest) and the misfit."""
+        import ctypes as C
+        from ._lib import check, lib, CBS_MEM_DEVICE
+        self._share_io()
+        inp = self.forward._inputs(input_field)
+        A = inp.shape[3]
+        du, dw, dg = self._fields(A)
         vmax = self.forward.set_RI_device(ri_dev.ptr, cplx)                      # :32
         if self._autotune and not self._tuned:
             self._tune(A)
@@ -129,19 +138,19 @@ class BACKWARD_SOLVER_MULTI:
         # mean over polarisation components and angles (:53 averages dims 4 and 5)
         check(lib.cbs_field_gradient(self.adjoint._h, C.c_void_p(du.ptr), C.c_void_p(dw.ptr),
                                      A * self.forward.pole_num, dims, C.c_void_p(dg.ptr), CBS_MEM_DEVICE))   # :50-53
-        rx, ry, rz = self.forward.expected_RI_size
-        grad = dg.to_host().reshape(rz, ry, rx).transpose(2, 1, 0)
-        return grad, err
+        return dg, err
+
+    def _scan(self, A, rng=None):
+        """Angles of one gradient (:80-91): all, or num_scan_per_iteration random ones."""
+        n = self.parameters['num_scan_per_iteration']
+        if n == 0:
+            return np.arange(A)
+        rng = rng or np.random.default_rng()
+        return np.unique(np.concatenate([[0], rng.permutation(A)[:n]]))[:n]
 
     def get_gradiant(self, RI, input_field, output_field, rng=None):
         """:80-91 (random angle subsets when num_scan_per_iteration > 0)."""
-        n = self.parameters['num_scan_per_iteration']
-        A = np.asarray(input_field).shape[3]
-        if n == 0:
-            scan = np.arange(A)
-        else:
-            rng = rng or np.random.default_rng()
-            scan = np.unique(np.concatenate([[0], rng.permutation(A)[:n]]))[:n]
+        scan = self._scan(np.asarray(input_field).shape[3], rng)
         return self.get_gradiant_static(RI, np.asarray(input_field)[:, :, :, scan],
                                         np.asarray(output_field)[:, :, :, scan])
 
@@ -168,6 +177,8 @@ class BACKWARD_SOLVER_MULTI:
         Vmin = (p['nmin'] ** 2 - p['kappamax'] ** 2) / n_m ** 2 - 1
         Vmax = p['nmax'] ** 2 / n_m ** 2 - 1
         Vimag_max = 2 * p['kappamax'] * p['nmax'] / n_m ** 2
+        if p.get('tv_on_device', True):
+            return self._solve_device(RI, u_n, x_n, input_field, output_field, alpha, Vmin, Vmax, Vimag_max)
         self.err_list = []
         self.timings = []
         for ii in range(int(p['itter_max'])):
@@ -176,12 +187,8 @@ class BACKWARD_SOLVER_MULTI:
             t1 = time.perf_counter()
             self.err_list.append(err)
             t_n = t_np
-            if p.get('tv_on_device', True):
-                s_n = tv_fista_inner_device(self.forward, u_n - (1 / alpha) * grad, p['tv_param'] / alpha, Vmin,
-                                            Vmax, Vimag_max, p['use_non_negativity'], False, p['inner_itt'])
-            else:
-                s_n = tv_fista_inner(u_n - (1 / alpha) * grad, p['tv_param'] / alpha, Vmin, Vmax, Vimag_max,
-                                     p['use_non_negativity'], False, p['inner_itt'])
+            s_n = tv_fista_inner(u_n - (1 / alpha) * grad, p['tv_param'] / alpha, Vmin, Vmax, Vimag_max,
+                                 p['use_non_negativity'], False, p['inner_itt'])
             t_np = (1 + np.sqrt(1 + 4 * t_n ** 2)) / 2
             u_n = s_n + (t_n - 1) / t_np * (s_n - x_n)
             x_n = s_n
@@ -191,4 +198,61 @@ class BACKWARD_SOLVER_MULTI:
             if p.get('verbose'):
                 print('itter : %d  err %.6e  gradient %.2f s  TV %.2f s' % (ii + 1, err, t1 - t0,
                                                                              time.perf_counter() - t1))
+        return RI
+
+    def _solve_device(self, RI, u_n, x_n, input_field, output_field, alpha, Vmin, Vmax, Vimag_max):
+        """The loop of solve (:120-161) with the iterates, the gradient and the RI in device
+        memory: the proximal input (cbs_fista_prox_input), the TV prox of TV_FISTA_inner_v2
+        (cbs_tv_prox, or cbs_tv_prox_complex when Vimag_max != 0) and the momentum and RI
+        update (cbs_fista_momentum), in the host loop's order and precision; only the
+        misfit's 2D fields cross to the host, and the RI once at the end."""
+        import ctypes as C
+        from . import device as dv
+        from ._lib import check, lib, CBS_MEM_DEVICE
+        p = self.parameters
+        shape = tuple(RI.shape)
+        n = int(np.prod(shape))
+        cplx_tv = 0 if Vimag_max == 0 else 1
+        tv_dt = np.complex64 if cplx_tv else np.float32
+        bufs = [dv.DeviceArray.from_host(np.ravel(a, order='F').astype(np.complex128)) for a in (RI, u_n, x_n)]
+        d_ri, d_u, d_x = bufs
+        d_in, d_out = dv.DeviceArray((n,), tv_dt), dv.DeviceArray((n,), tv_dt)
+        dims = (C.c_uint32 * 3)(*shape)
+        h = self.forward._h
+        lam = p['tv_param'] / alpha
+        non_neg = int(bool(p['use_non_negativity']))
+        inp_all, out_all = np.asarray(input_field), np.asarray(output_field)
+        t_np = 1.0
+        self.err_list = []
+        self.timings = []
+        try:
+            for ii in range(int(p['itter_max'])):
+                t0 = time.perf_counter()
+                scan = self._scan(inp_all.shape[3])
+                dg, err = self._gradient_device(d_ri, True, inp_all[:, :, :, scan], out_all[:, :, :, scan])
+                t1 = time.perf_counter()
+                self.err_list.append(err)
+                t_n = t_np
+                check(lib.cbs_fista_prox_input(h, C.c_void_p(d_u.ptr), C.c_void_p(dg.ptr), 1.0 / alpha, n, cplx_tv,
+                                               C.c_void_p(d_in.ptr)))
+                if cplx_tv:
+                    check(lib.cbs_tv_prox_complex(h, C.c_void_p(d_in.ptr), C.c_void_p(d_out.ptr), dims, float(lam),
+                                                  float(Vmin), float(Vmax), float(Vimag_max), non_neg, 0,
+                                                  int(p['inner_itt']), CBS_MEM_DEVICE))
+                else:
+                    check(lib.cbs_tv_prox(h, C.c_void_p(d_in.ptr), C.c_void_p(d_out.ptr), dims, float(lam),
+                                          float(Vmin), float(Vmax), non_neg, 0, int(p['inner_itt']), CBS_MEM_DEVICE))
+                t_np = (1 + np.sqrt(1 + 4 * t_n ** 2)) / 2
+                check(lib.cbs_fista_momentum(h, C.c_void_p(d_out.ptr), cplx_tv, C.c_void_p(d_x.ptr),
+                                             C.c_void_p(d_u.ptr), C.c_void_p(d_ri.ptr), n, (t_n - 1) / t_np,
+                                             float(p['RI_bg'])))
+                self.timings.append((t1 - t0, time.perf_counter() - t1))
+                if p.get('verbose'):
+                    print('itter : %d  err %.6e  gradient %.2f s  TV %.2f s' % (ii + 1, err, t1 - t0,
+                                                                                 time.perf_counter() - t1))
+            RI = d_ri.to_host().reshape(shape, order='F')
+        finally:
+            for b in bufs + [d_in, d_out]:
+                b.free()
+        self.RI_inter = RI
         return RI

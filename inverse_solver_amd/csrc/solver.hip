@@ -42,6 +42,15 @@ static void require(bool cond, int code, const std::string& msg) {
 }
 
 // ------------------------------------------------------------------ device buffers
+// Operations that synchronise the whole device (frees, allocations, stream and graph
+// teardown) and graph capture exclude each other across threads: a device-wide
+// synchronisation waits on a stream that another handle is capturing, which invalidates
+// that capture.
+static std::recursive_mutex& device_mutex() {
+  static std::recursive_mutex m;
+  return m;
+}
+
 template <class T>
 struct DevBuf {
   T* p = nullptr;
@@ -51,12 +60,16 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      std::lock_guard<std::recursive_mutex> lk(device_mutex());
+      (void)hipFree(p);
+    }
     p = nullptr;
     n = 0;
   }
   void ensure(size_t count) {
     if (count <= n && p) return;
+    std::lock_guard<std::recursive_mutex> lk(device_mutex());
     release();
     hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
     if (e != hipSuccess) {
@@ -348,6 +361,7 @@ struct Solver {
   }
 
   void drop_graph() {
+    std::lock_guard<std::recursive_mutex> lk(device_mutex());
     for (auto& g : gslot) {
       if (g.exec) (void)hipGraphExecDestroy(g.exec);
       if (g.graph) (void)hipGraphDestroy(g.graph);
@@ -357,6 +371,7 @@ struct Solver {
   }
 
   ~Solver() {
+    std::lock_guard<std::recursive_mutex> lk(device_mutex());
     int prev = 0;
     (void)hipGetDevice(&prev);
     (void)hipSetDevice(device);
@@ -1005,6 +1020,7 @@ struct Solver {
         if (!gs) {
           gs = &gslot[gslot_next];
           gslot_next ^= 1;
+          std::lock_guard<std::recursive_mutex> lk(device_mutex());   // no device-wide sync while capturing
           if (gs->exec) (void)hipGraphExecDestroy(gs->exec);
           if (gs->graph) (void)hipGraphDestroy(gs->graph);
           *gs = GraphSlot();
@@ -1055,6 +1071,7 @@ struct Solver {
 
   void ensure_copy_stream() {
     if (cstream) return;
+    std::lock_guard<std::recursive_mutex> lk(device_mutex());
     HIP_CHECK(hipStreamCreateWithFlags(&cstream, hipStreamNonBlocking));
     for (int k = 0; k < 2; ++k) {
       HIP_CHECK(hipEventCreateWithFlags(&ev_up[k], hipEventDisableTiming));
@@ -1821,7 +1838,7 @@ void gs_step_maps(float* dres, int32_t* dstep1, int32_t* dstep2, int n1, int n2,
     dbr.upload(br);
     hipLaunchKernelGGL(phase_raster, dim3((nb + 127) / 128), dim3(128), 0, 0, dbr.p, nb, dstep1, dstep2, n1, n2, n3);
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipDeviceSynchronize());
+    HIP_CHECK(hipStreamSynchronize(nullptr));   // this call's work, on the null stream
   }
 }
 
@@ -1868,7 +1885,7 @@ int cbs_phase_residues(int device, const float* wrapped, float* residues, const 
     hipLaunchKernelGGL(phase_residues, dim3(grid), dim3(256), 0, 0, dp, dr, (int)dims[0], (int)dims[1], (int)dims[2]);
     HIP_CHECK(hipGetLastError());
     if (mem == CBS_MEM_HOST) HIP_CHECK(hipMemcpy(residues, br.p, m * sizeof(float), hipMemcpyDeviceToHost));
-    HIP_CHECK(hipDeviceSynchronize());
+    HIP_CHECK(hipStreamSynchronize(nullptr));   // this call's work, on the null stream
   });
 }
 
@@ -1905,7 +1922,7 @@ int cbs_phase_integrate(int device, const float* wrapped, const int32_t* step1, 
                        (int)dims[0], (int)dims[1], (int)dims[2]);
     HIP_CHECK(hipGetLastError());
     if (mem == CBS_MEM_HOST) HIP_CHECK(hipMemcpy(unwrapped, bp.p, n * sizeof(float), hipMemcpyDeviceToHost));
-    HIP_CHECK(hipDeviceSynchronize());
+    HIP_CHECK(hipStreamSynchronize(nullptr));   // this call's work, on the null stream
   });
 }
 
@@ -1936,7 +1953,7 @@ int cbs_phase_branch_cuts(int device, const float* residues, int32_t* step1, int
       HIP_CHECK(hipMemcpy(step1, b1.p, (size_t)n1 * n2 * n3 * sizeof(int32_t), hipMemcpyDeviceToHost));
       HIP_CHECK(hipMemcpy(step2, b2.p, (size_t)n2 * n3 * sizeof(int32_t), hipMemcpyDeviceToHost));
     }
-    HIP_CHECK(hipDeviceSynchronize());
+    HIP_CHECK(hipStreamSynchronize(nullptr));   // this call's work, on the null stream
   });
 }
 
@@ -1971,7 +1988,7 @@ int cbs_phase_unwrap(int device, const float* wrapped, float* unwrapped, const u
                        d1.p, n1, n2, n3);
     HIP_CHECK(hipGetLastError());
     if (mem == CBS_MEM_HOST) HIP_CHECK(hipMemcpy(unwrapped, dp, n * sizeof(float), hipMemcpyDeviceToHost));
-    HIP_CHECK(hipDeviceSynchronize());
+    HIP_CHECK(hipStreamSynchronize(nullptr));   // this call's work, on the null stream
   });
 }
 
@@ -2005,7 +2022,10 @@ int cbs_create(const cbs_geometry* geo, cbs_handle* out) {
     s->nri = (long long)s->rx * s->ry * s->rz;
     s->opt.form = CBS_FORM_FUSED;
     s->opt.use_graph = 1;   // the fused and four-pass loops replay a captured graph unless kernels are timed
-    HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    {
+      std::lock_guard<std::recursive_mutex> lk(device_mutex());
+      HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    }
     // set-up of the no-allocation workspace on this device (a failure is reported by
     // cbs_solve_3d_noalloc, which resolves it again)
     try {
@@ -2131,6 +2151,40 @@ int cbs_field_gradient(cbs_handle h, const void* u, const void* w, uint32_t n_rh
     Solver* s = lookup(h);
     DeviceGuard dg(s->device);
     s->field_grad(u, w, n_rhs, dims, grad, mem);
+  });
+}
+
+int cbs_fista_prox_input(cbs_handle h, const void* u, const void* grad, double inv_alpha, uint64_t n, int cplx,
+                         void* out) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    require(u && grad && out, CBS_ERR_ARG, "null FISTA argument");
+    require(n >= 1 && n < (1ull << 40), CBS_ERR_ARG, "FISTA length out of range");
+    require(cplx == 0 || cplx == 1, CBS_ERR_ARG, "cplx must be 0 or 1");
+    require(std::isfinite(inv_alpha), CBS_ERR_ARG, "inv_alpha must be finite");
+    DeviceGuard dg(s->device);
+    hipLaunchKernelGGL(fista_prox_input, dim3(Solver::grid_for((long long)n)), dim3(256), 0, s->stream,
+                       static_cast<const double2*>(u), static_cast<const double2*>(grad), inv_alpha, (long long)n, cplx,
+                       out);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(s->stream));
+  });
+}
+
+int cbs_fista_momentum(cbs_handle h, const void* sp, int cplx, void* x, void* u, void* ri, uint64_t n, double c,
+                       double n_m) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    require(sp && x && u && ri, CBS_ERR_ARG, "null FISTA argument");
+    require(n >= 1 && n < (1ull << 40), CBS_ERR_ARG, "FISTA length out of range");
+    require(cplx == 0 || cplx == 1, CBS_ERR_ARG, "cplx must be 0 or 1");
+    require(std::isfinite(c) && std::isfinite(n_m) && n_m > 0, CBS_ERR_ARG, "momentum and RI_bg must be finite");
+    DeviceGuard dg(s->device);
+    hipLaunchKernelGGL(fista_momentum, dim3(Solver::grid_for((long long)n)), dim3(256), 0, s->stream, sp, cplx,
+                       static_cast<double2*>(x), static_cast<double2*>(u), static_cast<double2*>(ri), (long long)n, c,
+                       n_m);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(s->stream));
   });
 }
 

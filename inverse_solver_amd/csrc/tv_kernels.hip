@@ -130,4 +130,49 @@ __global__ void field_gradient(const float2* __restrict__ u, const float2* __res
   }
 }
 
+// FISTA of BACKWARD_SOLVER_MULTI (Codes/BACKWARD_SOLVER_MULTI.m:152-160) on the device,
+// the iterates complex128 and x fastest.  Proximal input u - inv_alpha grad, rounded to
+// complex64 (cplx) or its real part to float32, as the host loop forms it before the TV
+// prox of that type.
+__global__ void fista_prox_input(const double2* __restrict__ u, const double2* __restrict__ grad, double inv_alpha,
+                                 long long n, int cplx, void* __restrict__ out) {
+  GRID_STRIDE_TV(i, n) {
+    const double2 g = grad[i], v = u[i];
+    const double re = v.x - inv_alpha * g.x, im = v.y - inv_alpha * g.y;
+    if (cplx) static_cast<float2*>(out)[i] = make_float2((float)re, (float)im);
+    else static_cast<float*>(out)[i] = (float)re;
+  }
+}
+
+// Principal square root of a + ib (a branch cut on the negative real axis, the sign of
+// b kept for a <= 0): t = sqrt((|z| + |a|) / 2), then (t, b / 2t) for a >= 0 and
+// (|b| / 2t, copysign(t, b)) for a < 0, so that neither part suffers cancellation.
+__device__ __forceinline__ double2 csqrt_principal(double a, double b) {
+  if (a == 0.0 && b == 0.0) return make_double2(0.0, b);
+  const double t = sqrt(0.5 * (hypot(a, b) + fabs(a)));
+  if (a >= 0.0) return make_double2(t, b / (2.0 * t));
+  return make_double2(fabs(b) / (2.0 * t), copysign(t, b));
+}
+
+// After the proximal step s (complex64 when cplx, else float32, widened to FP64):
+// u = s + c (s - x), x = s, ri = n_m sqrt(u + 1).
+__global__ void fista_momentum(const void* __restrict__ s, int cplx, double2* __restrict__ x, double2* __restrict__ u,
+                               double2* __restrict__ ri, long long n, double c, double n_m) {
+  GRID_STRIDE_TV(i, n) {
+    double2 sv;
+    if (cplx) {
+      const float2 f = static_cast<const float2*>(s)[i];
+      sv = make_double2((double)f.x, (double)f.y);
+    } else {
+      sv = make_double2((double)static_cast<const float*>(s)[i], 0.0);
+    }
+    const double2 xv = x[i];
+    const double2 un = make_double2(sv.x + c * (sv.x - xv.x), sv.y + c * (sv.y - xv.y));
+    u[i] = un;
+    x[i] = sv;
+    const double2 r = csqrt_principal(un.x + 1.0, un.y);
+    ri[i] = make_double2(n_m * r.x, n_m * r.y);
+  }
+}
+
 }  // namespace cbs

@@ -25,4 +25,11 @@ __global__ void tv_zero(float* __restrict__ buf, long long n);
 __global__ void field_gradient(const float2* __restrict__ u, const float2* __restrict__ w, int nx, int ny, int nz,
                                int n_rhs, double2* __restrict__ grad);
 
+// FISTA iterates on the device (complex128, x fastest): the proximal input and the
+// momentum / RI update of BACKWARD_SOLVER_MULTI.m:152-160.
+__global__ void fista_prox_input(const double2* __restrict__ u, const double2* __restrict__ grad, double inv_alpha,
+                                 long long n, int cplx, void* __restrict__ out);
+__global__ void fista_momentum(const void* __restrict__ s, int cplx, double2* __restrict__ x, double2* __restrict__ u,
+                               double2* __restrict__ ri, long long n, double c, double n_m);
+
 }  // namespace cbs

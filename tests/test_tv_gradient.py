@@ -95,7 +95,11 @@ def test_oracle_gradient_matches_finite_differences_in_direction():
 
 
 @pytest.mark.gpu
-def test_gpu_gradient_and_fista_match_oracle():
+@pytest.mark.parametrize('nmax', [np.inf, 1.6])
+def test_gpu_gradient_and_fista_match_oracle(nmax):
+    # nmax = Inf (the default): Vimag_max = NaN, the complex TV prox; a finite nmax with
+    # kappamax = 0: Vimag_max = 0, the real one.  The FISTA iterates, the gradient and the
+    # RI stay in device memory between the two outer iterations.
     from inverse_solver_amd.backward_solver import BACKWARD_SOLVER_MULTI, get_default_parameters
     from inverse_solver_amd.phantoms import circle_illumination
     from oracle.cbs import ForwardSolverConvergentBorn as Oracle
@@ -112,7 +116,7 @@ def test_gpu_gradient_and_fista_match_oracle():
                                                                + 1j * rng.standard_normal(meas.shape)) / np.sqrt(2)
     RI0 = np.full(ri, 1.336) + 0.002 * (sphere_ri(ri, 0.4, n_s=1.337) - 1.336) / 0.001
     bp = get_default_parameters(dict(forward_solver_parameters=p, step=0.01, tv_param=0.001, inner_itt=20,
-                                     itter_max=2, verbose=False))
+                                     itter_max=2, verbose=False, nmax=nmax))
     b = BACKWARD_SOLVER_MULTI(bp)
     g, err = b.get_gradiant_static(RI0, u, meas)
     g_ref, err_ref = get_gradiant_static(RI0, u.astype(np.complex128), meas, o)
@@ -121,7 +125,7 @@ def test_gpu_gradient_and_fista_match_oracle():
     assert abs(err - err_ref) < 1e-4 * err_ref
     RI_gpu = b.solve(u, meas, init_RI=RI0)
     RI_ref, errs = backward_multi_solve(RI0, u.astype(np.complex128), meas, o, dict(
-        RI_bg=1.336, step=0.01, tv_param=0.001, use_non_negativity=False, nmin=-np.inf, nmax=np.inf, kappamax=0,
+        RI_bg=1.336, step=0.01, tv_param=0.001, use_non_negativity=False, nmin=-np.inf, nmax=nmax, kappamax=0,
         inner_itt=20, itter_max=2))
     d = rel(RI_gpu - 1.336, RI_ref - 1.336)
     print('FISTA 2 outer iterations: RI contrast rel err %.2e' % d)
