@@ -218,6 +218,29 @@ int cbs_tv_prox_complex(cbs_handle h, const void* in, void* out, const uint32_t 
 int cbs_field_gradient(cbs_handle h, const void* u, const void* w, uint32_t n_rhs, const uint32_t dims[3], void* grad,
                        int mem);
 
+/* cbs_solve_2d for scalar fields from the incident field itself: the source is
+ * cbs_field_pupil(in_field, pupil_in), and the outputs are
+ * trans = cbs_field_pupil(t + source, pupil_out) and refl = cbs_field_pupil(r, pupil_out)
+ * (fft2(t) + U = fft2(t + source), U the filtered input spectrum), t and r being what
+ * cbs_solve_2d returns; field_3d as there.  The whole of
+ * FORWARD_SOLVER_CONVERGENT_BORN.solve (FORWARD_SOLVER_CONVERGENT_BORN.m:397-520) for
+ * pole_num 1 in one call, with no host transform between the MEX actions; pupils as in
+ * cbs_field_pupil; mem applies to in_field, trans, refl and field_3d. */
+int cbs_solve_field(cbs_handle h, const void* in_field, uint32_t n_rhs, const float* pupil_in, const float* pupil_out,
+                    void* trans, void* refl, void* field_3d, int mem);
+
+/* Centred pupil filter of scalar fields on the field grid (field_size x, y):
+ *   out_f = fftshift(ifft2(ifftshift(P .* fftshift(fft2(ifftshift(in_f + add_f))))))
+ * for n_fields fields of complex64 (x fastest); add may be null; out may be in.  P:
+ * field_x * field_y floats on the host, x fastest, in centred frequency order.  The
+ * scalar case of FORWARD_SOLVER.transform_field_3D / transform_field_2D
+ * (FORWARD_SOLVER.m:38-97) between the centred transforms of
+ * FORWARD_SOLVER_CONVERGENT_BORN.m:415-417 and :481-508, which the reference runs in
+ * MATLAB around the MEX: P is the NA circle times 1/sqrt(cos theta) (input) or
+ * sqrt(cos theta) (output) under use_abbe_sine.  mem applies to in, add and out. */
+int cbs_field_pupil(cbs_handle h, const void* in, const void* add, uint32_t n_fields, const float* pupil, void* out,
+                    int mem);
+
 /* The outer FISTA iteration of BACKWARD_SOLVER_MULTI (:152-160) on device arrays of n
  * values, x fastest (the reference runs it on gpuArrays in MATLAB), so the iterates, the
  * gradient and the RI stay in device memory between gradients.  Both run on the

@@ -24,7 +24,7 @@ EXPORTED = [
     'cbs_footprint', 'cbs_get_plan', 'cbs_autotune', 'cbs_tv_prox', 'cbs_tv_prox_complex', 'cbs_field_gradient',
     'cbs_phase_residues', 'cbs_phase_integrate', 'cbs_phase_branch_cuts', 'cbs_phase_unwrap',
     'cbs_set_convergence', 'cbs_get_iterations', 'cbs_solve_3d_noalloc', 'cbs_fista_prox_input',
-    'cbs_fista_momentum', 'cbs_last_error', 'cbs_abi_version',
+    'cbs_fista_momentum', 'cbs_field_pupil', 'cbs_solve_field', 'cbs_last_error', 'cbs_abi_version',
 ]
 
 
@@ -92,6 +92,8 @@ def _load():
         'cbs_get_iterations': ([vp, C.POINTER(C.c_uint32), u32], i32),
         'cbs_fista_prox_input': ([vp, vp, vp, C.c_double, C.c_uint64, i32, vp], i32),
         'cbs_fista_momentum': ([vp, vp, i32, vp, vp, vp, C.c_uint64, C.c_double, C.c_double], i32),
+        'cbs_field_pupil': ([vp, vp, vp, u32, vp, vp, i32], i32),
+        'cbs_solve_field': ([vp, vp, u32, vp, vp, vp, vp, vp, i32], i32),
         'cbs_last_error': ([], C.c_char_p),
         'cbs_abi_version': ([], i32),
     }

@@ -138,4 +138,48 @@ __global__ void io_crop_out(const float2* __restrict__ buf2, float2* __restrict_
   }
 }
 
+// Centred 2D transforms of the scalar field grid (nx x ny, x fastest, nf fields) on the
+// device: fftshift(ifft2(P . fft2(ifftshift(a)))) as FORWARD_SOLVER.m:38-97 filters the
+// input and output fields with the pupil.  field_shift_in gathers ifftshift(a [+ b]),
+// field_pupil_mul scales each spectrum position by the pupil at the frequency it holds
+// (the 1/N of the inverse transform folded in), field_shift_out scatters fftshift(c).
+__global__ void field_shift_in(const float2* __restrict__ a, const float2* __restrict__ b, float2* __restrict__ out,
+                               int nx, int ny, long long nf) {
+  const long long n2 = (long long)nx * ny;
+  const int hx = nx / 2, hy = ny / 2;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n2 * nf;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long f = i / n2;
+    const int r = (int)(i - f * n2);
+    const int y = r / nx, x = r - y * nx;
+    const int xs = x + hx < nx ? x + hx : x + hx - nx;
+    const int ys = y + hy < ny ? y + hy : y + hy - ny;
+    const long long j = f * n2 + (long long)ys * nx + xs;
+    float2 v = a[j];
+    if (b) v = cadd(v, b[j]);
+    out[i] = v;
+  }
+}
+
+__global__ void field_pupil_mul(float2* __restrict__ buf, const float* __restrict__ pupil, int nx, int ny, long long nf) {
+  const long long n2 = (long long)nx * ny;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n2 * nf;
+       i += (long long)gridDim.x * blockDim.x)
+    buf[i] = cscale(buf[i], pupil[i % n2]);
+}
+
+__global__ void field_shift_out(const float2* __restrict__ c, float2* __restrict__ out, int nx, int ny, long long nf) {
+  const long long n2 = (long long)nx * ny;
+  const int hx = nx / 2, hy = ny / 2;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n2 * nf;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long f = i / n2;
+    const int r = (int)(i - f * n2);
+    const int y = r / nx, x = r - y * nx;
+    const int xs = x - hx >= 0 ? x - hx : x - hx + nx;
+    const int ys = y - hy >= 0 ? y - hy : y - hy + ny;
+    out[i] = c[f * n2 + (long long)ys * nx + xs];
+  }
+}
+
 }  // namespace cbs

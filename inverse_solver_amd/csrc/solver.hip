@@ -1326,6 +1326,120 @@ struct Solver {
     set_io_kernels(R.data(), T.data(), K.data(), CBS_MEM_HOST);
   }
 
+  // ---- centred pupil transforms of the field grid (cbs_field_pupil)
+  struct FieldFft {
+    int nx = 0, ny = 0;
+    const RowsInfo* rw = nullptr;
+    const StridedInfo* st = nullptr;
+    RtPlan rtx, rty;
+    DevBuf<float2> twx, twy;
+  } ffp;
+  DevBuf<float2> fp_in, fp_add, fp_buf, io_fsrc;
+  // the last two pupils (a solve alternates the input and the output one): host copy and
+  // position-ordered device table
+  struct PupilSlot { std::vector<float> host; DevBuf<float> tab; int nx = 0, ny = 0; long long used = 0; } fp_slot[2];
+  long long fp_clock = 0;
+
+  // the device table of a host pupil; the least recently used slot is replaced, so two
+  // tables looked up one after the other both stay valid
+  const float* pupil_table(const float* pupil) {
+    const size_t n2 = (size_t)fx * fy;
+    for (auto& sl : fp_slot)
+      if (sl.nx == fx && sl.ny == fy && std::memcmp(sl.host.data(), pupil, n2 * sizeof(float)) == 0) {
+        sl.used = ++fp_clock;
+        return sl.tab.p;
+      }
+    PupilSlot& sl = fp_slot[0].used <= fp_slot[1].used ? fp_slot[0] : fp_slot[1];
+    sl.used = ++fp_clock;
+    // the pupil at the frequency each spectrum position holds (compiled transforms leave
+    // a transposed order, run-time ones the natural order), over fx fy
+    const bool cx_ = find_fft_rows(fx) != nullptr, cy_ = find_fft_strided(fy, 0) != nullptr;
+    std::vector<int> kx(fx);
+    for (int px = 0; px < fx; ++px) kx[px] = ((cx_ ? pos2freq(fx, px) : px) + fx / 2) % fx;
+    std::vector<float> tab(n2);
+    const double inv_n = 1.0 / (double)n2;
+    for (int py = 0; py < fy; ++py) {
+      const int cyi = ((cy_ ? pos2freq(fy, py) : py) + fy / 2) % fy;
+      for (int px = 0; px < fx; ++px)
+        tab[(size_t)py * fx + px] = (float)((double)pupil[(size_t)cyi * fx + kx[px]] * inv_n);
+    }
+    sl.tab.upload(tab);
+    sl.host.assign(pupil, pupil + n2);
+    sl.nx = fx;
+    sl.ny = fy;
+    return sl.tab.p;
+  }
+
+  void field_fft2d(float2* buf, long long nf, int inverse) {
+    if (ffp.nx != fx || ffp.ny != fy) {
+      ffp.rw = find_fft_rows(fx);
+      ffp.st = find_fft_strided(fy, 0);
+      if (!ffp.rw) build_rt(ffp.rtx, fx);
+      if (!ffp.st) build_rt(ffp.rty, fy);
+      ffp.twx.upload(twiddles(fx));
+      ffp.twy.upload(twiddles(fy));
+      ffp.nx = fx;
+      ffp.ny = fy;
+    }
+    auto rows = [&] {
+      const long long nrows = nf * fy;
+      if (!ffp.rw) return rt_rows(ffp.rtx, buf, nrows, inverse);
+      const dim3 g((unsigned)((nrows + ffp.rw->rows - 1) / ffp.rw->rows));
+      timed(2, [&] { hipLaunchKernelGGL(ffp.rw->fn, g, dim3(ffp.rw->nt), 0, stream, buf, nrows, ffp.twx.p, inverse); });
+    };
+    auto cols = [&] {
+      if (!ffp.st) return rt_strided(ffp.rty, buf, fx, (long long)fx, (long long)fx * fy, nf, inverse);
+      const int nt = (fx + ffp.st->t - 1) / ffp.st->t;
+      require((long long)nt * nf < (1ll << 31), CBS_ERR_UNSUPPORTED, "field transform batch too large for one launch");
+      timed(2, [&] { hipLaunchKernelGGL(ffp.st->fn, dim3((unsigned)(nt * nf)), dim3(ffp.st->nt), 0, stream, buf, fx,
+                                        (long long)fx, (long long)fx * fy, (int)nf, ffp.twy.p, inverse, (int)nf, 0ll); });
+    };
+    if (!inverse) { rows(); cols(); }
+    else { cols(); rows(); }
+  }
+
+  // out = centred_ifft2(P .* centred_fft2(in + add)) over nf fields of the field grid, on
+  // device pointers (add may be null, out may be in); tab from pupil_table
+  void pupil_apply(const float2* in, const float2* add, long long nf, const float* tab, float2* out) {
+    const long long total = (long long)fx * fy * nf;
+    fp_buf.ensure((size_t)total);
+    const unsigned g = grid_for(total);
+    timed(2, [&] { hipLaunchKernelGGL(field_shift_in, dim3(g), dim3(256), 0, stream, in, add, fp_buf.p, fx, fy, nf); });
+    field_fft2d(fp_buf.p, nf, 0);
+    timed(2, [&] { hipLaunchKernelGGL(field_pupil_mul, dim3(g), dim3(256), 0, stream, fp_buf.p, tab, fx, fy, nf); });
+    field_fft2d(fp_buf.p, nf, 1);
+    timed(2, [&] { hipLaunchKernelGGL(field_shift_out, dim3(g), dim3(256), 0, stream, fp_buf.p, out, fx, fy, nf); });
+  }
+
+  // P: fx x fy floats, x fastest, in centred (fftshifted) frequency order, on the host
+  void field_pupil(const void* in, const void* add, uint32_t nf, const float* pupil, void* out, int mem) {
+    require(in && out && pupil, CBS_ERR_ARG, "null field argument");
+    require(nf >= 1, CBS_ERR_ARG, "n_fields must be >= 1");
+    require(mem == CBS_MEM_HOST || mem == CBS_MEM_DEVICE, CBS_ERR_ARG, "mem must be CBS_MEM_HOST or CBS_MEM_DEVICE");
+    const long long total = (long long)fx * fy * (long long)nf;
+    require(total < (1ll << 40), CBS_ERR_ARG, "field batch too large");
+    const float* tab = pupil_table(pupil);
+    const float2* din = static_cast<const float2*>(in);
+    const float2* dadd = static_cast<const float2*>(add);
+    float2* dout = static_cast<float2*>(out);
+    if (mem == CBS_MEM_HOST) {
+      fp_in.ensure((size_t)total);
+      HIP_CHECK(hipMemcpyAsync(fp_in.p, in, (size_t)total * sizeof(float2), hipMemcpyHostToDevice, stream));
+      din = fp_in.p;
+      if (add) {
+        fp_add.ensure((size_t)total);
+        HIP_CHECK(hipMemcpyAsync(fp_add.p, add, (size_t)total * sizeof(float2), hipMemcpyHostToDevice, stream));
+        dadd = fp_add.p;
+      }
+      dout = fp_in.p;
+    }
+    pupil_apply(din, dadd, nf, tab, dout);
+    HIP_CHECK(hipGetLastError());
+    if (mem == CBS_MEM_HOST)
+      HIP_CHECK(hipMemcpyAsync(out, fp_in.p, (size_t)total * sizeof(float2), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+  }
+
   // batched 2D FFT over nslices (cx x cy) planes
   void fft2d(float2* buf, long long nslices, int inverse) {
     const RowsInfo* rw = io_rows;
@@ -1349,7 +1463,10 @@ struct Solver {
     HIP_CHECK(hipGetLastError());
   }
 
-  void solve_2d(const float2* src, uint32_t n_rhs, float2* trans, float2* refl, float2* f3d, int mem) {
+  // pupil_in / pupil_out (host, may be null): the input field is filtered into the source,
+  // and the outputs are formed as the centred pupil transforms of cbs_solve_field
+  void solve_2d(const float2* src, uint32_t n_rhs, float2* trans, float2* refl, float2* f3d, int mem,
+                const float* pupil_in = nullptr, const float* pupil_out = nullptr) {
     require(have_potential, CBS_ERR_STATE, "cbs_set_potential must be called before solving");
     require(io.ready, CBS_ERR_STATE, "cbs_set_io_kernels must be called before cbs_solve_2d");
     require(n_rhs >= 1 && src != nullptr, CBS_ERR_ARG, "null source or n_rhs = 0");
@@ -1365,11 +1482,14 @@ struct Solver {
     if (refl) io_out_r.ensure(nvw * n2);
     io_uin.ensure(nvw * nri);
     if (mem == CBS_MEM_HOST || !f3d) io_u3d.ensure(nvw * nri);
-    if (mem == CBS_MEM_HOST) {
-      io_src.ensure(nvw * nf);
+    if (mem == CBS_MEM_HOST) io_src.ensure(nvw * nf);
+    if (mem == CBS_MEM_HOST || pupil_out) {
       if (trans) io_tr.ensure(nvw * nf);
       if (refl) io_rf.ensure(nvw * nf);
     }
+    const float* tab_in = pupil_in ? pupil_table(pupil_in) : nullptr;
+    const float* tab_out = pupil_out ? pupil_table(pupil_out) : nullptr;
+    if (pupil_in) io_fsrc.ensure(nvw * nf);
     const int M = (int)prm.Bornmax;
     begin_solve(n_rhs, M);
     ev_used.clear();
@@ -1384,6 +1504,10 @@ struct Solver {
       if (mem == CBS_MEM_HOST) {
         HIP_CHECK(hipMemcpyAsync(io_src.p, s_d, (size_t)nv * nf * sizeof(float2), hipMemcpyHostToDevice, stream));
         s_d = io_src.p;
+      }
+      if (tab_in) {
+        pupil_apply(s_d, nullptr, nv, tab_in, io_fsrc.p);
+        s_d = io_fsrc.p;
       }
       // incident field: pad -> fft2 -> x refocus(z) -> ifft2 -> crop  (:543-554)
       timed(2, [&] { hipLaunchKernelGGL(io_pad_in, dim3(grid_for(nv * n2)), dim3(256), 0, stream, s_d, io_spec2.p, iog, nv); });
@@ -1407,14 +1531,19 @@ struct Solver {
         if (trans) {
           fft2d(io_out_t.p, nv, 1);
           float2* dst = mem == CBS_MEM_HOST ? io_tr.p : trans + v0 * nf;
+          float2* crop = tab_out ? io_tr.p : dst;
           timed(2, [&] { hipLaunchKernelGGL(io_crop_out, dim3(grid_for(nv * nf)), dim3(256), 0, stream, io_out_t.p,
-                                            dst, iog, nv, sc); });
+                                            crop, iog, nv, sc); });
+          // fft2(t) + U = fft2(t + source): the scattered field plus the filtered incident one
+          if (tab_out) pupil_apply(io_tr.p, s_d, nv, tab_out, dst);
         }
         if (refl) {
           fft2d(io_out_r.p, nv, 1);
           float2* dst = mem == CBS_MEM_HOST ? io_rf.p : refl + v0 * nf;
+          float2* crop = tab_out ? io_rf.p : dst;
           timed(2, [&] { hipLaunchKernelGGL(io_crop_out, dim3(grid_for(nv * nf)), dim3(256), 0, stream, io_out_r.p,
-                                            dst, iog, nv, sc); });
+                                            crop, iog, nv, sc); });
+          if (tab_out) pupil_apply(io_rf.p, nullptr, nv, tab_out, dst);
         }
       }
       HIP_CHECK(hipGetLastError());
@@ -2151,6 +2280,27 @@ int cbs_field_gradient(cbs_handle h, const void* u, const void* w, uint32_t n_rh
     Solver* s = lookup(h);
     DeviceGuard dg(s->device);
     s->field_grad(u, w, n_rhs, dims, grad, mem);
+  });
+}
+
+int cbs_solve_field(cbs_handle h, const void* in_field, uint32_t n_rhs, const float* pupil_in, const float* pupil_out,
+                    void* trans, void* refl, void* field3d, int mem) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    DeviceGuard dg(s->device);
+    require(pupil_in && pupil_out, CBS_ERR_ARG, "null pupil");
+    require(s->P == 1, CBS_ERR_UNSUPPORTED, "cbs_solve_field takes scalar fields (pole_num 1)");
+    s->solve_2d(static_cast<const float2*>(in_field), n_rhs, static_cast<float2*>(trans), static_cast<float2*>(refl),
+                static_cast<float2*>(field3d), mem, pupil_in, pupil_out);
+  });
+}
+
+int cbs_field_pupil(cbs_handle h, const void* in, const void* add, uint32_t n_fields, const float* pupil, void* out,
+                    int mem) {
+  return guarded([&] {
+    Solver* s = lookup(h);
+    DeviceGuard dg(s->device);
+    s->field_pupil(in, add, n_fields, pupil, out, mem);
   });
 }
 

@@ -342,7 +342,62 @@ class FORWARD_SOLVER_CONVERGENT_BORN:
         src = np.asfortranarray(centered_ifft2(U).astype(np.complex64))                # _CUDA.m:122
         return U, src
 
+    def _pupils32(self):
+        """The input and output pupils of the scalar filters as float32 (x fastest, centred
+        frequency order): NA circle, times 1/sqrt(cos theta) and sqrt(cos theta) under
+        use_abbe_sine (FORWARD_SOLVER.m:45-50, :76-81)."""
+        na, inv, fwd = self._pupil_factors()
+        abbe = bool(self.parameters['use_abbe_sine'])
+        cache = getattr(self, '_pupil32', None)
+        if cache is None or cache[0] is not na or cache[1] != abbe:
+            if abbe:
+                pin, pout = na * inv, na * fwd
+            else:
+                pin = pout = na
+            cache = (na, abbe, np.asfortranarray(pin, np.float32), np.asfortranarray(pout, np.float32))
+            self._pupil32 = cache
+        return cache[2], cache[3]
+
+    def _solve_scalar(self, input_field, field_3d_dev, return_2d):
+        """Scalar solve with the pupil filters and centred transforms on the device
+        (cbs_solve_field): the input field goes in, the output fields come out."""
+        p = self.parameters
+        inp = self._inputs(input_field)
+        fx, fy = int(p['size'][0]), int(p['size'][1])
+        if list(inp.shape[:2]) != [fx, fy]:
+            raise ValueError('input field must have the field size %s' % (p['size'][:2],))
+        if not hasattr(self, 'Bornmax'):
+            raise RuntimeError('set_RI must be called before solve')
+        self._ensure_io()
+        A = inp.shape[3]
+        rx, ry, rz = self.expected_RI_size
+        want_t = bool(p['return_transmission']) and return_2d
+        want_r = bool(p['return_reflection']) and return_2d
+        want_3 = bool(p['return_3D']) or field_3d_dev is not None
+        pin, pout = self._pupils32()
+        nul = C.c_void_p(None)
+        fin = np.asfortranarray(inp, dtype=np.complex64)
+        if field_3d_dev is None:
+            t2 = np.empty((fx, fy, 1, A), np.complex64, order='F') if want_t else None
+            r2 = np.empty((fx, fy, 1, A), np.complex64, order='F') if want_r else None
+            f3 = np.empty((rx, ry, rz, 1, A), np.complex64, order='F') if want_3 else None
+            check(lib.cbs_solve_field(self._h, _fptr(fin), A, _fptr(pin), _fptr(pout), _fptr(t2) if want_t else nul,
+                                      _fptr(r2) if want_r else nul, _fptr(f3) if want_3 else nul, _lib.CBS_MEM_HOST))
+            return t2, r2, f3
+        from . import device as dv
+        if field_3d_dev.nbytes != 8 * rx * ry * rz * A or field_3d_dev.dtype != np.complex64:
+            raise ValueError('field_3d must hold %d x %s complex64 values' % (A, [rx, ry, rz]))
+        d_in = dv.DeviceArray.from_host(fin.T)
+        d_t = dv.DeviceArray((A, 1, fy, fx)) if want_t else None
+        d_r = dv.DeviceArray((A, 1, fy, fx)) if want_r else None
+        check(lib.cbs_solve_field(self._h, C.c_void_p(d_in.ptr), A, _fptr(pin), _fptr(pout),
+                                  C.c_void_p(d_t.ptr) if want_t else nul, C.c_void_p(d_r.ptr) if want_r else nul,
+                                  C.c_void_p(field_3d_dev.ptr), _lib.CBS_MEM_DEVICE))
+        return (d_t.to_host().T if want_t else None), (d_r.to_host().T if want_r else None), None
+
     def _solve(self, input_field, field_3d_dev, return_2d=True):
+        if self.pole_num == 1:
+            return self._solve_scalar(input_field, field_3d_dev, return_2d)
         p = self.parameters
         U, src = self.source_2d(input_field)
         A = src.shape[3]
