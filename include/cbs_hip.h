@@ -107,8 +107,11 @@ typedef struct {
   int32_t time_kernels; /* 1: time each pass kind with HIP events (cbs_get_kernel_times) */
   int32_t threads_real;   /* fused form: force threads per block of the real-space pass (0 = planner) */
   int32_t threads_kspace; /* fused form: force threads per block of the k-space pass (0 = planner) */
-  int32_t block_order;    /* fused form: bit 0 real-space pass, bit 1 k-space pass; 0 = rhs-fastest
-                             block numbering, 1 = rows of one rhs together (cbs_autotune picks) */
+  int32_t block_order;    /* fused form, 0..15 (default 4; cbs_autotune picks): bit 0 real-space
+                             pass, bit 1 k-space pass, 0 = rhs-fastest block numbering, 1 = rows
+                             of one rhs together; bit 2 real-space tiles in storage order, rhs
+                             slowest (overrides bit 0); bit 3 nontemporal k-space loads (large
+                             batches) */
 } cbs_options;
 
 /* Creates a solver on geo->device.  Replaces CudaBorn action 0 (main.cpp:57-101). */

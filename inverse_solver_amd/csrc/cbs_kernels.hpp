@@ -63,6 +63,8 @@ struct Tables {
 
 struct Consts {
   int nx, ny, nz, a, b;
+  int tiled;          // 1 (fused form): the rows of each z plane of V, Field and f are stored
+                      // in the real-space pass's tile order, row y at (y % b) a + y / b
   int nrhs;
   long long nvox;
   float inv_eps;
@@ -72,6 +74,11 @@ struct Consts {
   float c_re, c_im;   // vector runs: k^2 / (k^2 + i eps), the dyadic absorption correction
 };
 
+// storage row of simulation row y in a z plane of the state arrays
+__host__ __device__ __forceinline__ int state_row(const Consts& c, int y) {
+  return c.tiled ? (y % c.b) * c.a + y / c.b : y;
+}
+
 struct StateArgs {
   const float2* V;
   float2* field;
@@ -79,9 +86,13 @@ struct StateArgs {
   float2* spec;
   const uint8_t* parity;   // per-rhs flip parity (0/1)
   long long rhs_stride;    // elements between consecutive rhs in field/fstate/spec
-  double* norms;           // optional [nrhs] FP64 |f_n|^2 accumulators (may be null)
-  double* fref;            // optional [nrhs] FP64 |Field|^2 accumulators, filled by the jj = 2 launch
+  double* norms;           // optional [nrhs][kNormSlots] FP64 |f_n|^2 partial sums (may be null)
+  double* fref;            // optional [nrhs][kNormSlots] FP64 |Field|^2 partial sums, filled by the jj = 2 launch
 };
+
+// FP64 norm accumulators: every (iteration, component slot) sum is held in kNormSlots
+// partial sums (flush_norm), added up by their readers.
+constexpr int kNormSlots = 64;
 
 // FP64 convergence test (an option; the reference runs a fixed Bornmax).  Rhs r has
 // stopped after iteration n >= 3 when |f_n|^2 <= tol^2 |F_1 + F_2|^2, both summed in
@@ -90,8 +101,8 @@ struct StateArgs {
 // an iteration that skips, the increment it adds is zero, so the test stays true and
 // every later launch skips as well.  norms = null: no test.
 struct ConvArgs {
-  const double* norms;     // [M][nslots] |f_n|^2 (row n - 1 holds iteration n)
-  const double* fref;      // [nslots] |F_1 + F_2|^2
+  const double* norms;     // [M][nslots][kNormSlots] |f_n|^2 partial sums (row n - 1: iteration n)
+  const double* fref;      // [nslots][kNormSlots] |F_1 + F_2|^2 partial sums
   double tol2;
   int P;                   // component slots per rhs
   int nslots;              // slots per norms row
@@ -102,8 +113,11 @@ __device__ __forceinline__ bool conv_stopped(const ConvArgs& cv, int n, int r) {
   if (n < 3) return false;
   double s = 0.0, ref = 0.0;
   for (int q = 0; q < cv.P; ++q) {
-    s += cv.norms[(long long)(n - 1) * cv.nslots + cv.P * r + q];
-    ref += cv.fref[cv.P * r + q];
+    const long long slot = (long long)cv.P * r + q;
+    for (int k = 0; k < kNormSlots; ++k) {
+      s += cv.norms[((long long)(n - 1) * cv.nslots + slot) * kNormSlots + k];
+      ref += cv.fref[slot * kNormSlots + k];
+    }
   }
   return s <= cv.tol2 * ref;
 }
@@ -170,11 +184,16 @@ __device__ __forceinline__ float sign_inv(int flip_inv) { return flip_inv ? 1.f 
 __device__ __forceinline__ float sign_fwd(int flip_fwd) { return flip_fwd ? -1.f : 1.f; }
 
 // Wave-level reduction of a double then one atomic per wave.
-__device__ __forceinline__ void flush_norm(double v, double* dst) {
-  if (!dst) return;
+// Adds a wave's partial sum to the accumulator of component slot `slot`, which is
+// spread over kNormSlots sub-slots picked by block and wave: blocks in flight that all
+// work on one slot (storage-order numbering) then do not queue on one address.  The
+// readers sum the sub-slots (conv_stopped, sum_slots).
+__device__ __forceinline__ void flush_norm(double v, double* base, long long slot) {
+  if (!base) return;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(dst, v);
+  const int sub = (int)((blockIdx.x * 8u + (threadIdx.x >> 6)) & (unsigned)(kNormSlots - 1));
+  if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(base + slot * kNormSlots + sub, v);
 }
 
 // 16-byte store of a state array.  With nt set (state larger than the caches: the

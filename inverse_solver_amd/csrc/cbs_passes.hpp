@@ -22,16 +22,21 @@ struct PassRArgs {
   StateArgs st;
   int upd, fwd, jj_inv, jj_fwd;
   int order;   // block order: 0 = rhs fastest (V rows reused by consecutive blocks),
-               // 1 = y2 fastest, then rhs (the b blocks of one (rhs, z) slab run together)
+               // 1 = y2 fastest, then rhs (the b blocks of one (rhs, z) slab run together),
+               // 2 = y2 fastest, then z, then rhs (the tiles of one rhs in storage order, so
+               // the blocks in flight stream one contiguous window of each state array)
   int nt;      // 1: nontemporal access to the state arrays (they exceed the caches)
   ConvArgs cv; // convergence test (cv.norms = null: off)
 };
 
 // ROWS = 0: the tile is the y coset y = y1 * b + y2 and the pass does the a-point
-// part of the y transform (pass_kspace applies the cross twiddles).  ROWS = 1 (four-pass form
-// for grids without a two-pass plan): the tile is A contiguous rows
-// y = y2 * A + y1, only the x transforms are done here, and the spectrum stays
-// in the natural [z][y][x] layout for the separate y and z passes.
+// part of the y transform (pass_kspace applies the cross twiddles); V, Field and f store
+// each z plane's rows in tile order (row y at y2 * A + y1, state_row), so a tile is one
+// contiguous run of every array.  ROWS = 1 (four-pass form for grids without a two-pass
+// plan): the tile is A contiguous rows y = y2 * A + y1 of the natural layout, only the x
+// transforms are done here, and the spectrum stays in the natural [z][y][x] layout for
+// the separate y and z passes.  Either way pair i of the tile sits at 2 i past the
+// tile's first element in every array.
 //
 // STEADY = 1: the launch is known to be a steady-state one (update UPD_DEFER or
 // UPD_PAIR, next p = gamma x), so the pointwise step compiles to straight-line code;
@@ -59,11 +64,16 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
     const int tile = bid / c.nrhs;
     y2 = tile % c.b;
     z = tile / c.b;
-  } else {
+  } else if (args.order == 1) {
     y2 = bid % c.b;
     const int t = bid / c.b;
     r = t % c.nrhs;
     z = t / c.nrhs;
+  } else {
+    y2 = bid % c.b;
+    const int t = bid / c.b;
+    z = t % c.nz;
+    r = t / c.nz;
   }
   const long long roff = (long long)r * args.st.rhs_stride;
   float4* __restrict__ spec4 = reinterpret_cast<float4*>(args.st.spec + roff + (long long)(z * c.b + y2) * (A * NX));
@@ -73,12 +83,9 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
   const float sg_fwd = sign_fwd(args.jj_fwd ? flip_of(args.jj_fwd, args.st.parity, r) : 0);
   // block-uniform base of this z plane; per-lane offsets inside a plane fit in 32 bits
   const long long zoff = (long long)z * c.ny * NX;
-  // byte offset inside the z plane of pair i of the tile
-  auto pair_off = [&](int i) {
-    const unsigned e = 2u * (unsigned)i;
-    const unsigned y1 = e / NX, x = e - y1 * NX;
-    return ((ROWS ? (unsigned)y2 * A + y1 : y1 * (unsigned)c.b + (unsigned)y2) * NX + x) * 8u;
-  };
+  // byte offset inside the z plane of pair i of the tile (the tile's rows are stored together)
+  const unsigned tile_off = (unsigned)y2 * (unsigned)(A * NX) * 8u;
+  auto pair_off = [&](int i) { return tile_off + 16u * (unsigned)i; };
 
   if (args.cv.norms && args.jj_inv > 0) {
     // convergence test: this slot's rhs stopped after an earlier iteration
@@ -177,7 +184,7 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
       gi[k] = 0;
       if (ok[k]) {
         const unsigned e = 2u * (unsigned)i;
-        const unsigned y1 = e / NX, x = e - y1 * NX;
+        const unsigned x = e - (e / NX) * NX;
         const unsigned g = pair_off(i);
         gi[k] = g;
         // per-axis x ramp and mask pairs (read through the cache: the LDS holds three tiles per CU)
@@ -221,8 +228,8 @@ __global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
       }
     }
   }
-  if (want_norm) flush_norm(nrm, args.st.norms + r);
-  if (want_fref) flush_norm(nrmF, args.st.fref + r);
+  if (want_norm) flush_norm(nrm, args.st.norms, r);
+  if (want_fref) flush_norm(nrmF, args.st.fref, r);
   if (fwd == FWD_NONE) return;
   __syncthreads();
   line_fft<NX, false, A, NT>(lds, row, s_twx, tid);
@@ -251,6 +258,7 @@ struct PassKArgs {
   int order;   // block order: 0 = rhs fastest, then a-index, then x tile;
                // 1 = x tile fastest, then a-index, then rhs (whole rows read together)
   int nt;      // 1: nontemporal stores of the spectrum (it exceeds the caches)
+  int ntl;     // 1: nontemporal loads of the spectrum as well
   ConvArgs cv; // convergence test (cv.norms = null: off): a stopped rhs is skipped
 };
 
@@ -320,7 +328,8 @@ __global__ __launch_bounds__(NT) void pass_kspace(PassKArgs args) {
         const float2* src = spec + (long long)q * args.rhs_stride + (long long)z * B * rowstride + 2 * tp;
 #pragma unroll
         for (int y2 = 0; y2 < B; ++y2)
-          reg[k][q][y2] = (2 * tp < tvalid) ? *reinterpret_cast<const float4*>(src + y2 * rowstride)
+          reg[k][q][y2] = (2 * tp < tvalid) ? load_stream(reinterpret_cast<const float4*>(src + y2 * rowstride),
+                                                          args.ntl)
                                             : make_float4(0.f, 0.f, 0.f, 0.f);
       }
     }
@@ -486,7 +495,7 @@ void pass_kspace_seq3(PassKArgs args) {
         const float2* src = srcq + (long long)z * B * rowstride + 2 * tp;
 #pragma unroll
         for (int y2 = 0; y2 < B; ++y2)
-          reg[k][y2] = (2 * tp < tvalid) ? *reinterpret_cast<const float4*>(src + y2 * rowstride)
+          reg[k][y2] = (2 * tp < tvalid) ? load_stream(reinterpret_cast<const float4*>(src + y2 * rowstride), args.ntl)
                                          : make_float4(0.f, 0.f, 0.f, 0.f);
       }
     }
@@ -713,8 +722,8 @@ __global__ __launch_bounds__(NT) void line_rows_update(StateArgs st, Tables tb, 
     if (wf) f[i] = xn;
     lds[row(l, e)] = pn;
   }
-  if (want_norm) flush_norm(nrm, st.norms + slot);
-  if (want_fref) flush_norm(nrmF, st.fref + slot);
+  if (want_norm) flush_norm(nrm, st.norms, slot);
+  if (want_fref) flush_norm(nrmF, st.fref, slot);
   if (fwd == FWD_NONE) return;
   __syncthreads();
   line_fft<N, false, NL, NT>(lds, row, tw, threadIdx.x);

@@ -270,8 +270,8 @@ __global__ __launch_bounds__(NT) void blue_rows_update(StateArgs st, Tables tb, 
     if (wf) f[g] = xn;
     lds[ad(l, e)] = cmulc(pn, ch[k]);
   }
-  if (want_norm) flush_norm(nrm, st.norms + slot);
-  if (want_fref) flush_norm(nrmF, st.fref + slot);
+  if (want_norm) flush_norm(nrm, st.norms, slot);
+  if (want_fref) flush_norm(nrmF, st.fref, slot);
   if (fwd == FWD_NONE) return;
 #pragma unroll
   for (int k = 0; k < KZ; ++k) {                         // zero padding above NMAX

@@ -82,8 +82,9 @@ __global__ void io_emitter(const float2* __restrict__ field, const float2* __res
     float2 e = make_float2(0.f, 0.f);
     if (xr >= 0 && xr < g.rx && yr >= 0 && yr < g.ry) {
       const long long gs = ((long long)(z + oz) * c.ny + (yr + oy)) * c.nx + (xr + ox);
+      const long long gf = ((long long)(z + oz) * c.ny + state_row(c, yr + oy)) * c.nx + (xr + ox);
       const long long gr = (((long long)r * g.rz + z) * g.ry + yr) * g.rx + xr;
-      const float2 u = cadd(field[(long long)r * rhs_stride + gs], u_in[gr]);
+      const float2 u = cadd(field[(long long)r * rhs_stride + gf], u_in[gr]);
       const float2 v = V[gs];
       const float2 vt = make_float2(v.x * dV, (v.y + c.eps) * dV);
       e = cmul(u, vt);

@@ -98,8 +98,8 @@ __global__ void update_plain(StateArgs st, Tables tb, Consts c, int upd, int fwd
     if (fwd == FWD_GAMMA) f[i] = xn;
     if (fwd != FWD_NONE) spec[i] = p;
   }
-  if (st.norms) flush_norm(nrm, st.norms + rhs);
-  if (want_fref) flush_norm(nrmF, st.fref + rhs);
+  if (st.norms) flush_norm(nrm, st.norms, rhs);
+  if (want_fref) flush_norm(nrmF, st.fref, rhs);
 }
 
 // Flip parity of each component slot: (flip_state + (rhs0 + v / P) * M) & 1, the
@@ -135,7 +135,27 @@ __global__ void seed_source(const float2* __restrict__ u_in, const float2* __res
       const float h = 0.5f * c.inv_eps;
       p = make_float2(-s.y * h, s.x * h);                         // (i / eps) s / 2
     }
-    fstate[(long long)r * rhs_stride + g] = p;
+    fstate[(long long)r * rhs_stride + ((long long)z * c.ny + state_row(c, y)) * c.nx + x] = p;
+  }
+}
+
+// dst[i] = sum of the kNormSlots partial sums of accumulator i (src), n accumulators.
+__global__ void sum_slots(const double* __restrict__ src, double* __restrict__ dst, long long n) {
+  GRID_STRIDE(i, n) {
+    double s = 0.0;
+    for (int k = 0; k < kNormSlots; ++k) s += src[i * kNormSlots + k];
+    dst[i] = s;
+  }
+}
+
+// The potential in the state arrays' row order (V natural -> Vt, c.tiled = 1).
+__global__ void tile_rows(const float2* __restrict__ V, float2* __restrict__ Vt, Consts c) {
+  GRID_STRIDE(i, c.nvox) {
+    const int x = (int)(i % c.nx);
+    const long long yz = i / c.nx;
+    const int y = (int)(yz % c.ny);
+    const int z = (int)(yz / c.ny);
+    Vt[((long long)z * c.ny + state_row(c, y)) * c.nx + x] = V[i];
   }
 }
 
@@ -149,7 +169,7 @@ __global__ void finalize_field(const float2* __restrict__ field, const float2* _
     const long long yz = gi / rg.nx;
     const int yr = (int)(yz % rg.ny);
     const int zr = (int)(yz / rg.ny);
-    const long long g = ((long long)(zr + rg.oz) * c.ny + (yr + rg.oy)) * c.nx + (xr + rg.ox);
+    const long long g = ((long long)(zr + rg.oz) * c.ny + state_row(c, yr + rg.oy)) * c.nx + (xr + rg.ox);
     u_out[i] = cadd(field[(long long)r * rhs_stride + g], u_in[i]);
   }
 }

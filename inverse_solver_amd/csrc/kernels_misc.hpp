@@ -20,6 +20,8 @@ __global__ void seed_source(const float2* __restrict__ u_in, const float2* __res
                             Consts c, RoiGeom rg, long long rhs_stride, int nrhs);
 __global__ void finalize_field(const float2* __restrict__ field, const float2* __restrict__ u_in, float2* __restrict__ u_out,
                                Consts c, RoiGeom rg, long long rhs_stride, int nrhs);
+__global__ void tile_rows(const float2* __restrict__ V, float2* __restrict__ Vt, Consts c);
+__global__ void sum_slots(const double* __restrict__ src, double* __restrict__ dst, long long n);
 
 // RI grid given as FP64 values, x fastest: real (cplx = 0) or interleaved complex.
 struct RiGrid {

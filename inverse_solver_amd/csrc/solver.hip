@@ -274,6 +274,11 @@ struct Solver {
   bool have_potential = false;
   cbs_potential_params prm{};
   DevBuf<float2> V;
+  // the fused form's copy of V in the state arrays' tile row order (state_row), made when
+  // the potential or the plan changes, never inside a solve
+  DevBuf<float2> Vt;
+  long long v_version = 0;
+  std::vector<long long> vt_key;
   Plan plan;
   TableSet tabs;        // tables of the active plan
   int flip_state = 0;   // persistent alternating-shift parity (reference handle semantics)
@@ -286,8 +291,8 @@ struct Solver {
     float2* fstate = nullptr;
     float2* spec = nullptr;
     uint8_t* parity = nullptr;
-    double* norms = nullptr;   // [M][slots] when norms are tracked
-    double* fref = nullptr;    // [slots] when the convergence test is on
+    double* norms = nullptr;   // [M][slots][kNormSlots] partial sums, then [M][slots] totals, when tracked
+    double* fref = nullptr;    // [slots][kNormSlots] partial sums, then [slots] totals, with the test on
   } ws;
   std::vector<double> last_norms;
   int last_norm_rhs = 0, last_norm_iter = 0;
@@ -388,6 +393,7 @@ struct Solver {
   Consts consts(int nb) const {
     Consts c;
     c.nx = nx; c.ny = ny; c.nz = nz; c.a = plan.a; c.b = plan.b; c.nrhs = nb; c.nvox = nvox;
+    c.tiled = plan.form == CBS_FORM_FUSED ? 1 : 0;
     c.eps = (float)prm.eps_imag;
     c.inv_eps = (float)(1.0 / prm.eps_imag);
     const double km = 2.0 * M_PI * prm.k0_nm;
@@ -652,6 +658,7 @@ struct Solver {
       T.rad_z.upload(qz);
     }
     tab_key = table_key();
+    sync_vtile();
   }
 
   // ---- kernel timing
@@ -691,7 +698,8 @@ struct Solver {
   StateArgs state_args(int nb) {
     (void)nb;
     StateArgs s;
-    s.V = V.p; s.field = ws.field; s.fstate = ws.fstate; s.spec = ws.spec; s.parity = ws.parity;
+    s.V = plan.form == CBS_FORM_FUSED ? Vt.p : V.p;
+    s.field = ws.field; s.fstate = ws.fstate; s.spec = ws.spec; s.parity = ws.parity;
     s.rhs_stride = nvox;
     s.norms = nullptr;
     s.fref = nullptr;
@@ -710,7 +718,7 @@ struct Solver {
   }
   // norm accumulators of the launch that completes iteration jj_inv (null: not tracked)
   double* norm_row(int jj_inv, int nv) const {
-    return (tracking() && jj_inv > 0) ? ws.norms + (size_t)(jj_inv - 1) * nv : nullptr;
+    return (tracking() && jj_inv > 0) ? ws.norms + (size_t)(jj_inv - 1) * nv * kNormSlots : nullptr;
   }
 
   static int upd_of(int jj, int M) { return upd_kind(jj, M); }
@@ -734,7 +742,7 @@ struct Solver {
       args.st.norms = norm_row(jj_inv, nv);
       if (conv_tol > 0.0 && upd == UPD_VQ_ADD) args.st.fref = ws.fref;
       args.upd = upd; args.fwd = fwd; args.jj_inv = jj_inv; args.jj_fwd = jj_fwd;
-      args.order = opt.block_order & 1;
+      args.order = (opt.block_order & 4) ? 2 : (opt.block_order & 1);
       args.nt = stream_hint(nv);
       args.cv = cv;
       const dim3 grid((unsigned)((long long)nv * plan.b * nz));
@@ -747,6 +755,7 @@ struct Solver {
       args.tb = tb; args.c = ck; args.spec = ws.spec; args.parity = ws.parity; args.rhs_stride = nvox; args.jj = jj;
       args.order = (opt.block_order >> 1) & 1;
       args.nt = stream_hint(nv);
+      args.ntl = args.nt && (opt.block_order & 8) ? 1 : 0;
       args.cv = cv;
       const int ntile = (nx + pk->t - 1) / pk->t;
       const dim3 grid((unsigned)((long long)nb * plan.a * ntile));
@@ -794,6 +803,7 @@ struct Solver {
       args.tb = tb; args.c = ck; args.spec = ws.spec; args.parity = ws.parity; args.rhs_stride = nvox; args.jj = jj;
       args.order = 0;
       args.nt = stream_hint(nv);
+      args.ntl = 0;
       args.cv = cv;
       const int ntile = (nx + pk->t - 1) / pk->t;
       const dim3 grid((unsigned)((long long)nb * ny * ntile));
@@ -944,8 +954,8 @@ struct Solver {
     fstate.ensure(n);
     spec.ensure(n);
     parity.ensure(nv);
-    if (tracking()) norms.ensure(std::max<size_t>(1, (size_t)prm.Bornmax) * nv);
-    if (conv_tol > 0.0) fref.ensure(nv);
+    if (tracking()) norms.ensure(std::max<size_t>(1, (size_t)prm.Bornmax) * nv * (kNormSlots + 1));
+    if (conv_tol > 0.0) fref.ensure(nv * (kNormSlots + 1));
     if (host_io) {
       uin.ensure((size_t)host_bufs * nv * nri);
       uout.ensure((size_t)host_bufs * nv * nri);
@@ -962,7 +972,7 @@ struct Solver {
     const int nv = batch_width(n_rhs) * P;
     uint64_t b = 3ull * nv * nvox * sizeof(float2);   // field, fstate, spec
     b += 2ull * nv * nri * sizeof(float2);             // staging for host I/O
-    b += (uint64_t)nvox * sizeof(float2);              // V
+    b += (uint64_t)nvox * sizeof(float2) * (plan.form == CBS_FORM_FUSED ? 2 : 1);   // V (and its tile-order copy)
     b += io.footprint(nv, cx, cy, rz) + (io.ready ? (unsigned long long)nv * nri * 16ull : 0ull);
     return b;
   }
@@ -989,12 +999,12 @@ struct Solver {
     const Consts c = consts(nv);
     const RoiGeom g = roi();
     if (tracking() && M > 0) {
-      const long long words = 2ll * M * nv;   // FP64 accumulators as 32-bit words
+      const long long words = 2ll * M * nv * kNormSlots;   // FP64 partial sums as 32-bit words
       hipLaunchKernelGGL(zero_words, dim3(grid_for(words)), dim3(256), 0, stream,
                          reinterpret_cast<uint32_t*>(ws.norms), words);
       if (conv_tol > 0.0)
-        hipLaunchKernelGGL(zero_words, dim3(grid_for(2ll * nv)), dim3(256), 0, stream,
-                           reinterpret_cast<uint32_t*>(ws.fref), 2ll * nv);
+        hipLaunchKernelGGL(zero_words, dim3(grid_for(2ll * nv * kNormSlots)), dim3(256), 0, stream,
+                           reinterpret_cast<uint32_t*>(ws.fref), 2ll * nv * kNormSlots);
     }
     {
       const long long tot = (long long)nv * nvox;
@@ -1045,10 +1055,17 @@ struct Solver {
     }
     HIP_CHECK(hipGetLastError());
     if (tracking() && M > 0) {
+      // the totals go after the partial sums
       std::vector<double> tmp((size_t)M * nv), ref((size_t)nv, 0.0);
-      HIP_CHECK(hipMemcpyAsync(tmp.data(), ws.norms, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
-      if (conv_tol > 0.0)
-        HIP_CHECK(hipMemcpyAsync(ref.data(), ws.fref, ref.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+      double* tot = ws.norms + (size_t)M * nv * kNormSlots;
+      hipLaunchKernelGGL(sum_slots, dim3(grid_for((long long)M * nv)), dim3(256), 0, stream, ws.norms, tot,
+                         (long long)M * nv);
+      HIP_CHECK(hipMemcpyAsync(tmp.data(), tot, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+      if (conv_tol > 0.0) {
+        double* rtot = ws.fref + (size_t)nv * kNormSlots;
+        hipLaunchKernelGGL(sum_slots, dim3(grid_for(nv)), dim3(256), 0, stream, ws.fref, rtot, (long long)nv);
+        HIP_CHECK(hipMemcpyAsync(ref.data(), rtot, ref.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+      }
       HIP_CHECK(hipStreamSynchronize(stream));
       for (int r = 0; r < nb; ++r) {
         double fr = 0.0;
@@ -1181,9 +1198,9 @@ struct Solver {
     const size_t o_par = off;
     off += up(nv);
     const size_t o_nrm = off;
-    if (tracking()) off += up((size_t)std::max(M, 1) * nv * sizeof(double));
+    if (tracking()) off += up((size_t)std::max(M, 1) * nv * (kNormSlots + 1) * sizeof(double));
     const size_t o_ref = off;
-    if (conv_tol > 0.0) off += up(nv * sizeof(double));
+    if (conv_tol > 0.0) off += up(nv * (kNormSlots + 1) * sizeof(double));
     const size_t state = up(nv * (size_t)nvox * sizeof(float2));
     const size_t o_state = off;
     off += 3 * state;
@@ -1610,9 +1627,9 @@ struct Solver {
     HIP_CHECK(hipEventCreate(&e1));
     double best = 1e30;
     Plan best_plan;
-    int best_order = 0;
+    int best_order = 4;
     int best_w = nb;
-    opt.block_order = 0;
+    opt.block_order = 4;
     auto time_plan = [&](int w) {
       born_batch(src.p, dst.p, w, 0);       // warm-up
       HIP_CHECK(hipEventRecord(e0, stream));
@@ -1624,7 +1641,7 @@ struct Solver {
       return (double)ms;
     };
     try {
-      // every kernel pair in block order 0, then the block orders of the three fastest
+      // every kernel pair in block order 4, then the block orders of the three fastest
       std::vector<std::pair<double, Plan>> timed_plans;
       for (auto& cd : cands) {
         Plan p;
@@ -1644,16 +1661,19 @@ struct Solver {
       for (size_t i = 0; i < ntop; ++i) {
         plan = timed_plans[i].second;
         build_tables();
-        for (int ord = 0; ord < 4; ++ord) {
+        // real-space tiles in storage order (4), rhs first (0) or rows of one rhs first (1);
+        // k-space rhs or x tile first (2).  Nontemporal k-space loads (8) measured 13-25 %
+        // slower on the 160^3 x 49 batch and are not tried.
+        for (int ord : {4, 0, 1, 2, 3, 6}) {
           opt.block_order = ord;
-          const double ms = ord == 0 ? timed_plans[i].first : time_plan(nb);
+          const double ms = ord == 4 ? timed_plans[i].first : time_plan(nb);
           if (ms < best) {
             best = ms;
             best_plan = plan;
             best_order = ord;
           }
         }
-        opt.block_order = 0;
+        opt.block_order = 4;
       }
       // batch width: the chosen plan with nb / 2, nb / 4, nb / 8, 2 and 1 rhs per launch
       // (narrower batches keep more of the state in the caches between launches but give
@@ -1852,12 +1872,25 @@ struct Solver {
     finish_potential();
   }
 
+  // Vt for the fused plan (its a and b) from the current V
+  void sync_vtile() {
+    if (plan.form != CBS_FORM_FUSED || !V.p) return;
+    const std::vector<long long> key = {v_version, (long long)plan.a, (long long)plan.b, nvox};
+    if (Vt.p && key == vt_key) return;
+    Vt.ensure((size_t)nvox);
+    hipLaunchKernelGGL(tile_rows, dim3(grid_for(nvox)), dim3(256), 0, stream, V.p, Vt.p, consts(1));
+    HIP_CHECK(hipGetLastError());
+    vt_key = key;
+  }
+
   void finish_potential() {
+    ++v_version;
     make_plan();
     // the tables depend on the plan and on the geometry scalars, not on V or eps: a new
     // potential on the same geometry (every set_RI of a reconstruction or a serving loop)
     // keeps them
     if (table_key() != tab_key) build_tables();
+    sync_vtile();
     HIP_CHECK(hipStreamSynchronize(stream));
     flip_state = 0;
     have_potential = true;
@@ -2151,6 +2184,7 @@ int cbs_create(const cbs_geometry* geo, cbs_handle* out) {
     s->nri = (long long)s->rx * s->ry * s->rz;
     s->opt.form = CBS_FORM_FUSED;
     s->opt.use_graph = 1;   // the fused and four-pass loops replay a captured graph unless kernels are timed
+    s->opt.block_order = 4; // real-space tiles in storage order: 2.5 % faster than rhs first at 160^3 x 49
     {
       std::lock_guard<std::recursive_mutex> lk(device_mutex());
       HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
@@ -2193,7 +2227,7 @@ int cbs_set_options(cbs_handle h, const cbs_options* opt) {
                 opt->form == CBS_FORM_AXES, CBS_ERR_ARG,
             "unknown form");
     require(opt->max_batch >= 0 && opt->split_a >= 0 && opt->tile_t >= 0, CBS_ERR_ARG, "negative option");
-    require(opt->block_order >= 0 && opt->block_order <= 3, CBS_ERR_ARG, "block_order must be 0..3");
+    require(opt->block_order >= 0 && opt->block_order <= 15, CBS_ERR_ARG, "block_order must be 0..15");
     DeviceGuard dg(s->device);
     s->opt = *opt;
     s->drop_graph();

@@ -79,13 +79,14 @@ def test_small_grid_plans_match_oracle(opts):
 
 
 def test_block_orders_give_identical_results():
-    # every block order (rhs or row numbering of each pass) computes the same values
+    # every block order (rhs, row or storage-order numbering of each pass, cached or
+    # nontemporal k-space loads) computes the same values
     ri = (16, 16, 16)
     p = params(ri, iterations=10)
     RI = sphere_ri(ri, 0.6, n_s=1.40)
     fields = np.stack([random_field(ri, 90 + j) for j in range(8)], -1)
     outs = []
-    for order in range(4):
+    for order in range(16):
         s = _gpu()(p, options=dict(form=0, block_order=order))
         s.set_RI(RI)
         outs.append(s.solve_incident(fields))

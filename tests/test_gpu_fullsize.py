@@ -50,6 +50,23 @@ def test_160cubed_49rhs_batch_invariance_linearity_and_oracle():
     s.close()
 
 
+@pytest.mark.parametrize('order', [4, 14, 15])
+def test_160cubed_49rhs_block_orders_equal(order):
+    # at this size the state streams are nontemporal, so order bit 3 (nontemporal k-space
+    # loads) takes effect; the numbering of the blocks changes no value
+    from inverse_solver_amd import FORWARD_SOLVER_CONVERGENT_BORN
+    p, RI, u = _c2()
+    p['iterations_number'] = 3
+    outs = []
+    for opts in (dict(block_order=0), dict(block_order=order)):
+        s = FORWARD_SOLVER_CONVERGENT_BORN(p, options=opts)
+        s.set_RI(RI)
+        outs.append(s.solve_incident(u))
+        assert s.plan()['block_order'] == opts['block_order']
+        s.close()
+    np.testing.assert_array_equal(outs[1], outs[0])
+
+
 def test_576sq_x320_fourpass_linearity_and_batch():
     from inverse_solver_amd import FORWARD_SOLVER_CONVERGENT_BORN
     from inverse_solver_amd.phantoms import spheres, incident_plane_waves
