@@ -377,6 +377,48 @@ def hbm_ceiling(stream, log, gib=2, reps=10):
     return res
 
 
+def pattern_ceilings(stream, log, dims, reps=10):
+    """The fused passes' own access patterns with no arithmetic (cbs_probe_pattern in
+    inverse_solver_amd/csrc/stream_probe.hip, libcbs_probe.so), on arrays of the solve's
+    size: the k-space pass's in-place 128-byte row segments, and the real-space pass's tile
+    streams of an even launch (spec, V, f) and an odd one (with Field), blocks in storage
+    order.  Best of 3 x `reps` launches by HIP events on `stream`; rates count the
+    algorithmic bytes of the pass (V once per launch).  dims = [nx, ny, nz, a, b, nrhs]."""
+    import ctypes as C
+    from inverse_solver_amd import device as dv
+    lib = C.CDLL(os.path.join(ROOT, 'inverse_solver_amd', 'libcbs_probe.so'))
+    f = lib.cbs_probe_pattern
+    f.restype = C.c_int
+    f.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_void_p,
+                  C.POINTER(C.c_float)]
+    nx, ny, nz, a, b, nrhs = [int(v) for v in dims]
+    nvox = nx * ny * nz
+    arrs = [dv.DeviceArray((nrhs * nvox,), np.complex64) for _ in range(3)] + [dv.DeviceArray((nvox,), np.complex64)]
+    for d in arrs:
+        dv._chk(dv.hip.hipMemset(C.c_void_p(d.ptr), 0, C.c_size_t(d.nbytes)), 'hipMemset')
+    cd = (C.c_int * 6)(nx, ny, nz, a, b, nrhs)
+    res = {}
+    for kind, name, nbytes in ((0, 'kspace', 16.0 * nrhs * nvox), (1, 'real_even', (32.0 * nrhs + 8.0) * nvox),
+                               (2, 'real_odd', (48.0 * nrhs + 8.0) * nvox)):
+        best = 1e30
+        for _ in range(3):
+            ms = C.c_float(0.0)
+            rc = f(kind, arrs[0].ptr, arrs[3].ptr, arrs[1].ptr, arrs[2].ptr, cd, reps, C.c_void_p(stream), C.byref(ms))
+            if rc != 0:
+                raise RuntimeError('cbs_probe_pattern(%d) failed: %d' % (kind, rc))
+            best = min(best, ms.value)
+        res[name] = {'ms': round(best, 4), 'GB_s': round(nbytes / (best / 1e3) / 1e9, 1)}
+    for d in arrs:
+        d.free()
+    res['what'] = ('the passes\' access patterns without their arithmetic on %dx%dx%d x %d rhs (a %d, b %d): '
+                   'k-space 128-byte in-place row segments, cached loads and nontemporal stores; real-space '
+                   'storage-order tiles, nontemporal streams and cached V' % (nx, ny, nz, nrhs, a, b))
+    log('pattern ceilings: k-space %.3f ms (%.0f GB/s), real-space even %.3f ms (%.0f GB/s), odd %.3f ms (%.0f GB/s)'
+        % (res['kspace']['ms'], res['kspace']['GB_s'], res['real_even']['ms'], res['real_even']['GB_s'],
+           res['real_odd']['ms'], res['real_odd']['GB_s']))
+    return res
+
+
 def workload_demo(args):
     """The reference's own demo geometry (SIMULATION_BEAD.m:11-19, 48): a 161x161x61 RI
     grid at 0.532/4/1.2 um holding a 5-um bead (n 1.4609 in 1.336).  With the default
@@ -448,10 +490,12 @@ def workload_demo(args):
 def workload_ceiling(args):
     """The practical HBM ceilings alone (hbm_ceiling on the null stream): what the
     passes' achieved GB/s are compared against."""
-    c = hbm_ceiling(None, lambda *a: print(*a, file=sys.stderr, flush=True))
+    lg = lambda *a: print(*a, file=sys.stderr, flush=True)
+    c = hbm_ceiling(None, lg)
+    pc = pattern_ceilings(None, lg, [160, 160, 160, 40, 4, 49])
     return _line('streaming HBM ceiling (3 read + 2 write streams)', c['stream_3r2w']['GB_s'], 'GB/s',
                  'nontemporal 16-byte streaming probe over 2 GiB arrays', c['stream_3r2w']['ms'], args,
-                 {'hbm_ceiling': c}, dtype='f32')
+                 {'hbm_ceiling': c, 'pattern_ceiling': pc}, dtype='f32')
 
 
 def reference_form_gpu(s, u0, M, log_fn):
@@ -878,6 +922,14 @@ def main():
     c = out['hbm_ceiling']['stream_3r2w']['GB_s']
     out['roofline']['frac_of_stream_ceiling'] = round(achieved_pr / c, 4)
     out['roofline']['pass_kspace']['frac_of_stream_ceiling'] = round(achieved_pk / c, 4)
+    if plan['form'] == 'fused':
+        # each pass against its own access pattern without the arithmetic (pass_real
+        # alternates even and odd launches)
+        pc = pattern_ceilings(stream, log, list(sim) + [plan['a'], plan['b'], A])
+        out['pattern_ceiling'] = pc
+        pat_pr = 0.5 * (pc['real_even']['ms'] + pc['real_odd']['ms'])
+        out['roofline']['frac_of_pattern_ceiling'] = round(pat_pr / pr_avg_ms, 4)
+        out['roofline']['pass_kspace']['frac_of_pattern_ceiling'] = round(pc['kspace']['ms'] / pk_avg_ms, 4)
     s.close()
 
     if not args.no_plain:
