@@ -83,6 +83,8 @@ def measured_traffic(sim, A, plan, alg_per_launch):
     if not os.path.exists(fn) or list(sim) != [160, 160, 160] or A != 49 or plan.get('a') != 40:
         return None
     rec = json.load(open(fn))
+    if plan.get('block_order', 0) & 4 != rec.get('block_order', 0) & 4:
+        return None     # the real-space block numbering changes how often V is fetched
     return round(alg_per_launch * rec['measured_over_algorithmic'])
 
 
