@@ -41,8 +41,16 @@ struct PassRArgs {
 // STEADY = 1: the launch is known to be a steady-state one (update UPD_DEFER or
 // UPD_PAIR, next p = gamma x), so the pointwise step compiles to straight-line code;
 // STEADY = 0 takes every update kind at run time.
+// Waves per SIMD to hold pass_real's register budget to (1: no bound).  The 160 x 40
+// tile's 53.6 KB of LDS lets three 512-thread blocks share a CU (6 waves per SIMD, at most
+// 80 VGPRs); its steady pass fits 80 without spilling, and left alone the compiler took
+// 81 and with them a third of the occupancy.  Other shapes either reach their LDS limit
+// unbounded or would spill under the bound (96 x 48 and 160 x 32 at 512 threads).
+constexpr int pass_real_min_waves(int nx, int a, int nt) { return nx == 160 && a == 40 && nt == 512 ? 6 : 1; }
+
 template <int NX, int A, int NT, int ROWS, int STEADY>
-__global__ __launch_bounds__(NT) void pass_real(PassRArgs args) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(pass_real_min_waves(NX, A, NT))))
+void pass_real(PassRArgs args) {
   static_assert(NX % 2 == 0, "pass_real processes x in pairs");
   constexpr int PR = row_pitch(NX);
   constexpr int E2 = A * NX / 2;              // complex pairs in the tile
