@@ -45,3 +45,26 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120, cwd=root)
     assert r.returncode == 2
     assert 'one GPU' in json.loads(r.stdout.strip().splitlines()[-1])['error']
+
+
+def test_probe_library_validates_arguments_without_gpu():
+    """libcbs_probe.so (the bench's bandwidth and access-pattern probes) loads, exports both
+    entry points, and refuses bad arguments before touching a device (status 1)."""
+    import ctypes as C
+    lib = C.CDLL(os.path.join(HERE, '..', 'inverse_solver_amd', 'libcbs_probe.so'))
+    ms = C.c_float(0.0)
+    lib.cbs_probe_stream.restype = C.c_int
+    lib.cbs_probe_pattern.restype = C.c_int
+    fake = C.c_void_p(16)
+    # unknown kind, missing arrays, bytes not a multiple of 16
+    assert lib.cbs_probe_stream(7, fake, fake, fake, C.c_longlong(1024), 1, None, C.byref(ms)) == 1
+    assert lib.cbs_probe_stream(1, fake, None, fake, C.c_longlong(1024), 1, None, C.byref(ms)) == 1
+    assert lib.cbs_probe_stream(0, fake, None, fake, C.c_longlong(1000), 1, None, C.byref(ms)) == 1
+    dims = (C.c_int * 6)(160, 160, 160, 40, 4, 49)
+    bad_ny = (C.c_int * 6)(160, 150, 160, 40, 4, 49)      # ny != a * b
+    bad_nx = (C.c_int * 6)(150, 160, 160, 40, 4, 49)      # k-space tiles of 16 need nx % 16 == 0
+    assert lib.cbs_probe_pattern(3, fake, fake, fake, fake, dims, 1, None, C.byref(ms)) == 1
+    assert lib.cbs_probe_pattern(1, fake, fake, fake, fake, bad_ny, 1, None, C.byref(ms)) == 1
+    assert lib.cbs_probe_pattern(0, fake, fake, fake, fake, bad_nx, 1, None, C.byref(ms)) == 1
+    assert lib.cbs_probe_pattern(2, fake, fake, fake, None, dims, 1, None, C.byref(ms)) == 1    # odd launch needs Field
+    assert lib.cbs_probe_pattern(1, fake, fake, fake, fake, dims, 0, None, C.byref(ms)) == 1    # reps >= 1
