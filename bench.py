@@ -50,8 +50,10 @@ def workload_params(ri, iters, A_fields):
         return_transmission=True, return_reflection=False, return_3D=True))
 
 
-def pass_real_bytes(nvox, A, M):
+def pass_real_bytes(nvox, A, M, v_bytes=8):
     """Algorithmic HBM bytes of every pass_real launch of one solve (complex64 = 8 B).
+    v_bytes: bytes of V per voxel, 4 when the fused form reads a real RI's potential as its
+    real part (the imaginary part is -eps everywhere).
 
     Per launch and voxel: the spectrum is read when the launch completes an
     iteration and written when it starts the next one; f is read by the full
@@ -72,7 +74,7 @@ def pass_real_bytes(nvox, A, M):
         fwd = 0 if jj == M else (1 if jj <= 1 else 2)
         rd = (upd != 0) + (upd >= 3 or fwd == 1) + (upd in (2, 3, 4, 6))
         wr = (upd not in (0, 5)) + (fwd == 2) + (fwd != 0)
-        return 8 * A * (rd + wr) + 8
+        return 8 * A * (rd + wr) + v_bytes
     return nvox * np.array([one(jj) for jj in range(0, M + 1)], dtype=np.float64)
 
 
@@ -379,13 +381,14 @@ def hbm_ceiling(stream, log, gib=2, reps=10):
     return res
 
 
-def pattern_ceilings(stream, log, dims, reps=10):
+def pattern_ceilings(stream, log, dims, reps=10, v_float=False):
     """The fused passes' own access patterns with no arithmetic (cbs_probe_pattern in
     inverse_solver_amd/csrc/stream_probe.hip, libcbs_probe.so), on arrays of the solve's
     size: the k-space pass's in-place 128-byte row segments, and the real-space pass's tile
     streams of an even launch (spec, V, f) and an odd one (with Field), blocks in storage
     order.  Best of 3 x `reps` launches by HIP events on `stream`; rates count the
-    algorithmic bytes of the pass (V once per launch).  dims = [nx, ny, nz, a, b, nrhs]."""
+    algorithmic bytes of the pass (V once per launch).  dims = [nx, ny, nz, a, b, nrhs];
+    v_float: V read as one float per voxel, as the pass reads a real RI's potential."""
     import ctypes as C
     from inverse_solver_amd import device as dv
     lib = C.CDLL(os.path.join(ROOT, 'inverse_solver_amd', 'libcbs_probe.so'))
@@ -400,8 +403,10 @@ def pattern_ceilings(stream, log, dims, reps=10):
         dv._chk(dv.hip.hipMemset(C.c_void_p(d.ptr), 0, C.c_size_t(d.nbytes)), 'hipMemset')
     cd = (C.c_int * 6)(nx, ny, nz, a, b, nrhs)
     res = {}
-    for kind, name, nbytes in ((0, 'kspace', 16.0 * nrhs * nvox), (1, 'real_even', (32.0 * nrhs + 8.0) * nvox),
-                               (2, 'real_odd', (48.0 * nrhs + 8.0) * nvox)):
+    vb = 4.0 if v_float else 8.0
+    ke, ko = (3, 4) if v_float else (1, 2)
+    for kind, name, nbytes in ((0, 'kspace', 16.0 * nrhs * nvox), (ke, 'real_even', (32.0 * nrhs + vb) * nvox),
+                               (ko, 'real_odd', (48.0 * nrhs + vb) * nvox)):
         best = 1e30
         for _ in range(3):
             ms = C.c_float(0.0)
@@ -414,7 +419,8 @@ def pattern_ceilings(stream, log, dims, reps=10):
         d.free()
     res['what'] = ('the passes\' access patterns without their arithmetic on %dx%dx%d x %d rhs (a %d, b %d): '
                    'k-space 128-byte in-place row segments, cached loads and nontemporal stores; real-space '
-                   'storage-order tiles, nontemporal streams and cached V' % (nx, ny, nz, nrhs, a, b))
+                   'storage-order tiles, nontemporal streams and cached V (%d bytes a voxel)'
+                   % (nx, ny, nz, nrhs, a, b, int(vb)))
     log('pattern ceilings: k-space %.3f ms (%.0f GB/s), real-space even %.3f ms (%.0f GB/s), odd %.3f ms (%.0f GB/s)'
         % (res['kspace']['ms'], res['kspace']['GB_s'], res['real_even']['ms'], res['real_even']['GB_s'],
            res['real_odd']['ms'], res['real_odd']['GB_s']))
@@ -494,7 +500,7 @@ def workload_ceiling(args):
     passes' achieved GB/s are compared against."""
     lg = lambda *a: print(*a, file=sys.stderr, flush=True)
     c = hbm_ceiling(None, lg)
-    pc = pattern_ceilings(None, lg, [160, 160, 160, 40, 4, 49])
+    pc = pattern_ceilings(None, lg, [160, 160, 160, 40, 4, 49], v_float=True)
     return _line('streaming HBM ceiling (3 read + 2 write streams)', c['stream_3r2w']['GB_s'], 'GB/s',
                  'nontemporal 16-byte streaming probe over 2 GiB arrays', c['stream_3r2w']['ms'], args,
                  {'hbm_ceiling': c, 'pattern_ceiling': pc}, dtype='f32')
@@ -678,7 +684,8 @@ def workload_vector(args):
         log('vector step', k + 1, '%.1f ms' % step_ms[-1])
     ms = float(np.mean(step_ms))
     pr_ms, pk_ms = pk[0] / max(pk[1], 1), pk[2] / max(pk[3], 1)
-    bytes_pr = float(pass_real_bytes(nvox, 3 * A, M).sum()) / (pk[1] / args.steps)   # 3A component slots share V
+    vb = 4 if s.plan()['form'] == 'fused' and np.isrealobj(RI) else 8
+    bytes_pr = float(pass_real_bytes(nvox, 3 * A, M, vb).sum()) / (pk[1] / args.steps)   # 3A component slots share V
     bytes_pk = 16.0 * 3 * A * nvox
     out = d_out.to_host()
     # parity sample: angle 0, 8 iterations, against the FP64 oracle
@@ -824,7 +831,8 @@ def main():
     p50, p99 = float(np.percentile(step_ms, 50)), float(np.percentile(step_ms, 99))
 
     # roofline of the dominant kernel
-    bytes_pr = float(pass_real_bytes(nvox, A, M).sum())   # per solve
+    vb = 4 if plan['form'] == 'fused' and np.isrealobj(RI) else 8   # a real RI's potential: 4 bytes a voxel
+    bytes_pr = float(pass_real_bytes(nvox, A, M, vb).sum())   # per solve
     pr_avg_ms = pr_ms / max(pr_n, 1)
     pr_launches = pr_n / args.steps
     achieved_pr = bytes_pr / pr_launches / (pr_avg_ms / 1e3) / 1e9
@@ -927,7 +935,7 @@ def main():
     if plan['form'] == 'fused':
         # each pass against its own access pattern without the arithmetic (pass_real
         # alternates even and odd launches)
-        pc = pattern_ceilings(stream, log, list(sim) + [plan['a'], plan['b'], A])
+        pc = pattern_ceilings(stream, log, list(sim) + [plan['a'], plan['b'], A], v_float=vb == 4)
         out['pattern_ceiling'] = pc
         pat_pr = 0.5 * (pc['real_even']['ms'] + pc['real_odd']['ms'])
         out['roofline']['frac_of_pattern_ceiling'] = round(pat_pr / pr_avg_ms, 4)

@@ -81,6 +81,8 @@ __host__ __device__ __forceinline__ int state_row(const Consts& c, int y) {
 
 struct StateArgs {
   const float2* V;
+  const float* Vr;         // fused form, real RI: Re V in the state arrays' row order (Im V = -eps
+                           // everywhere, so the real-space pass reads 4 bytes a voxel); null: use V
   float2* field;
   float2* fstate;
   float2* spec;

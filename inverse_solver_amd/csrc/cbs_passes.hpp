@@ -48,7 +48,10 @@ struct PassRArgs {
 // unbounded or would spill under the bound (96 x 48 and 160 x 32 at 512 threads).
 constexpr int pass_real_min_waves(int nx, int a, int nt) { return nx == 160 && a == 40 && nt == 512 ? 6 : 1; }
 
-template <int NX, int A, int NT, int ROWS, int STEADY>
+//
+// VR = 1 (fused form, real RI): V is read as its real part from st.Vr, 4 bytes a voxel,
+// with the imaginary part -eps that V holds at every voxel for a real RI.
+template <int NX, int A, int NT, int ROWS, int STEADY, int VR>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(pass_real_min_waves(NX, A, NT))))
 void pass_real(PassRArgs args) {
   static_assert(NX % 2 == 0, "pass_real processes x in pairs");
@@ -124,6 +127,16 @@ void pass_real(PassRArgs args) {
   }
   const bool rf = reads_f(upd, fwd), rF = reads_F(upd);
   const float2* __restrict__ V = args.st.V + zoff;
+  const float* __restrict__ Vr = args.st.Vr + zoff;
+  // the potential at the pair whose state byte offset is g
+  auto load_v = [&](unsigned g) {
+    if constexpr (VR) {
+      const float2 q = *byte_at<const float2>(Vr, g >> 1);
+      return make_float4(q.x, -c.eps, q.y, -c.eps);
+    } else {
+      return *byte_at<const float4>(V, g);
+    }
+  };
   float2* __restrict__ F = args.st.field + roff + zoff;
   float2* __restrict__ f = args.st.fstate + roff + zoff;
   // small tiles (few blocks, e.g. a single rhs): the pointwise operands are requested with
@@ -137,7 +150,7 @@ void pass_real(PassRArgs args) {
       const int i = tid + k * NT;
       if (i < E2) {
         const unsigned g = pair_off(i);
-        pv[k] = *byte_at<const float4>(V, g);
+        pv[k] = load_v(g);
         if (rf) px[k] = load_stream(byte_at<const float4>(f, g), args.nt);
         if (rF) pF[k] = load_stream(byte_at<const float4>(F, g), args.nt);
       }
@@ -203,7 +216,7 @@ void pass_real(PassRArgs args) {
           if (rf) xx[k] = px[k0 + k];
           if (rF) FF[k] = pF[k0 + k];
         } else {
-          vv[k] = *byte_at<const float4>(V, g);
+          vv[k] = load_v(g);
           if (rf) xx[k] = load_stream(byte_at<const float4>(f, g), args.nt);
           if (rF) FF[k] = load_stream(byte_at<const float4>(F, g), args.nt);
         }

@@ -159,6 +159,18 @@ __global__ void tile_rows(const float2* __restrict__ V, float2* __restrict__ Vt,
   }
 }
 
+// Re V in the state arrays' row order (V natural -> Vr, c.tiled = 1), for a potential
+// whose imaginary part is -eps everywhere (a real RI).
+__global__ void tile_rows_real(const float2* __restrict__ V, float* __restrict__ Vr, Consts c) {
+  GRID_STRIDE(i, c.nvox) {
+    const int x = (int)(i % c.nx);
+    const long long yz = i / c.nx;
+    const int y = (int)(yz % c.ny);
+    const int z = (int)(yz / c.ny);
+    Vr[((long long)z * c.ny + state_row(c, y)) * c.nx + x] = V[i].x;
+  }
+}
+
 // u = Field[ROI] + u_in (Codes/FORWARD_SOLVER_CONVERGENT_BORN.m:646-647).
 __global__ void finalize_field(const float2* __restrict__ field, const float2* __restrict__ u_in, float2* __restrict__ u_out,
                                Consts c, RoiGeom rg, long long rhs_stride, int nrhs) {

@@ -21,6 +21,7 @@ __global__ void seed_source(const float2* __restrict__ u_in, const float2* __res
 __global__ void finalize_field(const float2* __restrict__ field, const float2* __restrict__ u_in, float2* __restrict__ u_out,
                                Consts c, RoiGeom rg, long long rhs_stride, int nrhs);
 __global__ void tile_rows(const float2* __restrict__ V, float2* __restrict__ Vt, Consts c);
+__global__ void tile_rows_real(const float2* __restrict__ V, float* __restrict__ Vr, Consts c);
 __global__ void sum_slots(const double* __restrict__ src, double* __restrict__ dst, long long n);
 
 // RI grid given as FP64 values, x fastest: real (cplx = 0) or interleaved complex.
@@ -42,7 +43,8 @@ using RowsFn = void (*)(float2*, long long, const float2*, int);
 using StridedFn = void (*)(float2*, int, long long, long long, int, const float2*, int, int, long long);
 
 // fn takes every update kind; fn_steady only UPD_DEFER / UPD_PAIR with FWD_GAMMA
-struct PassRInfo { PassRFn fn, fn_steady; int nx, a, nt, lds, rows; };
+// fn_vr, fn_steady_vr: the same passes reading a real potential (st.Vr; fused form only)
+struct PassRInfo { PassRFn fn, fn_steady, fn_vr, fn_steady_vr; int nx, a, nt, lds, rows; };
 struct PassKInfo { PassKFn fn; int b, nz, t, nt, lds, pole; };
 struct RowsInfo { RowsFn fn; int n, rows, nt; };
 struct StridedInfo { StridedFn fn; int n, t, nt; };

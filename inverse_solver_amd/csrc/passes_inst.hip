@@ -49,8 +49,11 @@ namespace cbs {
 
 // LDS of pass_real: the tile plus its per-axis tables (x: twiddle; rows: twiddle, ramp, mask)
 constexpr int pr_lds(int nx, int a) { return a * row_pitch(nx) * 8 + nx * 8 + a * 20; }
-#define PR_ENTRY(NX, A, NT) {&pass_real<NX, A, NT, 0, 0>, &pass_real<NX, A, NT, 0, 1>, NX, A, NT, pr_lds(NX, A), 0},
-#define PRR_ENTRY(NX, A, NT) {&pass_real<NX, A, NT, 1, 0>, &pass_real<NX, A, NT, 1, 1>, NX, A, NT, pr_lds(NX, A), 1},
+#define PR_ENTRY(NX, A, NT)                                                                                   \
+  {&pass_real<NX, A, NT, 0, 0, 0>, &pass_real<NX, A, NT, 0, 1, 0>, &pass_real<NX, A, NT, 0, 0, 1>,                  \
+   &pass_real<NX, A, NT, 0, 1, 1>, NX, A, NT, pr_lds(NX, A), 0},
+#define PRR_ENTRY(NX, A, NT) \
+  {&pass_real<NX, A, NT, 1, 0, 0>, &pass_real<NX, A, NT, 1, 1, 0>, nullptr, nullptr, NX, A, NT, pr_lds(NX, A), 1},
 #define PK_ENTRY(B, NZ, T, NT) {&pass_kspace<B, NZ, T, NT, 1>, B, NZ, T, NT, (int)(NZ * B * T * sizeof(float2)), 1},
 #define PKV_ENTRY(B, NZ, T, NT) \
   {&pass_kspace<B, NZ, T, NT, 3>, B, NZ, T, NT, (int)((3 * NZ * B * T + 2 * NZ) * sizeof(float2)), 3},

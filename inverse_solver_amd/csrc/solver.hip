@@ -275,8 +275,11 @@ struct Solver {
   cbs_potential_params prm{};
   DevBuf<float2> V;
   // the fused form's copy of V in the state arrays' tile row order (state_row), made when
-  // the potential or the plan changes, never inside a solve
+  // the potential or the plan changes, never inside a solve: Vt, or for a real RI (V's
+  // imaginary part -eps everywhere, as potential_fill forms it) only its real part, Vr
   DevBuf<float2> Vt;
+  DevBuf<float> Vr;
+  bool v_real = false;
   long long v_version = 0;
   std::vector<long long> vt_key;
   Plan plan;
@@ -698,7 +701,9 @@ struct Solver {
   StateArgs state_args(int nb) {
     (void)nb;
     StateArgs s;
-    s.V = plan.form == CBS_FORM_FUSED ? Vt.p : V.p;
+    const bool fused = plan.form == CBS_FORM_FUSED;
+    s.V = fused && !v_real ? Vt.p : V.p;
+    s.Vr = fused && v_real ? Vr.p : nullptr;
     s.field = ws.field; s.fstate = ws.fstate; s.spec = ws.spec; s.parity = ws.parity;
     s.rhs_stride = nvox;
     s.norms = nullptr;
@@ -747,7 +752,9 @@ struct Solver {
       args.cv = cv;
       const dim3 grid((unsigned)((long long)nv * plan.b * nz));
       const bool steady = (upd == UPD_DEFER || upd == UPD_PAIR) && fwd == FWD_GAMMA;
-      const PassRFn fn = steady ? pr->fn_steady : pr->fn;
+      const bool vr = args.st.Vr != nullptr;   // real potential: its 4-byte form
+      const PassRFn fn = steady ? (vr ? pr->fn_steady_vr : pr->fn_steady) : (vr ? pr->fn_vr : pr->fn);
+      require(fn != nullptr, CBS_ERR_UNSUPPORTED, "no real-potential form of the real-space pass");
       timed(0, [&] { hipLaunchKernelGGL(fn, grid, dim3(pr->nt), 0, stream, args); });
     };
     auto pass_k = [&](int jj) {
@@ -972,7 +979,8 @@ struct Solver {
     const int nv = batch_width(n_rhs) * P;
     uint64_t b = 3ull * nv * nvox * sizeof(float2);   // field, fstate, spec
     b += 2ull * nv * nri * sizeof(float2);             // staging for host I/O
-    b += (uint64_t)nvox * sizeof(float2) * (plan.form == CBS_FORM_FUSED ? 2 : 1);   // V (and its tile-order copy)
+    b += (uint64_t)nvox * sizeof(float2);   // V
+    if (plan.form == CBS_FORM_FUSED) b += (uint64_t)nvox * (v_real ? sizeof(float) : sizeof(float2));   // its tile-order copy
     b += io.footprint(nv, cx, cy, rz) + (io.ready ? (unsigned long long)nv * nri * 16ull : 0ull);
     return b;
   }
@@ -1818,6 +1826,7 @@ struct Solver {
     check_params(p);
     prm = *p;
     drop_graph();
+    v_real = false;   // a caller's V is taken as complex
     V.ensure((size_t)nvox);
     HIP_CHECK(hipMemcpyAsync(V.p, v, (size_t)nvox * sizeof(float2),
                              mem == CBS_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, stream));
@@ -1869,6 +1878,7 @@ struct Solver {
     const RiGrid g = ri_grid(ri, cplx, wavelength, n_m, mem);
     prm = *p;
     drop_graph();
+    v_real = cplx == 0;   // Im V = 0 - eps at every voxel, inside the RI grid and in the boundary
     V.ensure((size_t)nvox);
     const RoiGeom rg = roi();
     hipLaunchKernelGGL(potential_fill, dim3(grid_for(nvox)), dim3(256), 0, stream, g, flip_conj, rg, nx, ny,
@@ -1880,10 +1890,15 @@ struct Solver {
   // Vt for the fused plan (its a and b) from the current V
   void sync_vtile() {
     if (plan.form != CBS_FORM_FUSED || !V.p) return;
-    const std::vector<long long> key = {v_version, (long long)plan.a, (long long)plan.b, nvox};
-    if (Vt.p && key == vt_key) return;
-    Vt.ensure((size_t)nvox);
-    hipLaunchKernelGGL(tile_rows, dim3(grid_for(nvox)), dim3(256), 0, stream, V.p, Vt.p, consts(1));
+    const std::vector<long long> key = {v_version, (long long)plan.a, (long long)plan.b, nvox, (long long)v_real};
+    if (key == vt_key) return;
+    if (v_real) {
+      Vr.ensure((size_t)nvox);
+      hipLaunchKernelGGL(tile_rows_real, dim3(grid_for(nvox)), dim3(256), 0, stream, V.p, Vr.p, consts(1));
+    } else {
+      Vt.ensure((size_t)nvox);
+      hipLaunchKernelGGL(tile_rows, dim3(grid_for(nvox)), dim3(256), 0, stream, V.p, Vt.p, consts(1));
+    }
     HIP_CHECK(hipGetLastError());
     vt_key = key;
   }

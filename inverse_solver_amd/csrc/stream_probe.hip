@@ -97,7 +97,8 @@ __global__ __launch_bounds__(kPT) void pattern_kspace(v4f* spec, int nx, int a, 
 // rhs and read through the caches.  E2C > 0: the tile's pair count known at compile time,
 // as it is in pass_real (only the last access of a thread needs a bounds test); 0: e2
 // taken at run time.
-template <int ODD, int E2C>
+// VF: V read as one float per voxel (a real RI's potential, as the real-space pass reads it).
+template <int ODD, int E2C, int VF>
 __global__ __launch_bounds__(kPT) void pattern_real(v4f* spec, const v4f* V, v4f* f, v4f* F, int nx, int ny, int nz,
                                                      int a, int b) {
   const int e2 = E2C > 0 ? E2C : a * nx / 2;                  // pairs per tile
@@ -115,7 +116,12 @@ __global__ __launch_bounds__(kPT) void pattern_real(v4f* spec, const v4f* V, v4f
       const int i = i0 + threadIdx.x + k * kPT;
       if (i < e2) {
         q[k] = ld(spec + roff + toff + i);
-        v[k] = V[toff + i];
+        if (VF) {
+          const float2 h = reinterpret_cast<const float2*>(V)[toff + i];
+          v[k] = v4f{h.x, -1.f, h.y, -1.f};
+        } else {
+          v[k] = V[toff + i];
+        }
         x[k] = ld(f + roff + toff + i);
         if (ODD) w[k] = ld(F + roff + toff + i);
       }
@@ -134,14 +140,14 @@ __global__ __launch_bounds__(kPT) void pattern_real(v4f* spec, const v4f* V, v4f
 
 // the tile shapes of the compiled 160-point real-space passes (a = 20, 32, 40, 80)
 using PatternFn = void (*)(v4f*, const v4f*, v4f*, v4f*, int, int, int, int, int);
-template <int ODD>
+template <int ODD, int VF>
 PatternFn pattern_real_for(int e2) {
   switch (e2) {
-    case 1600: return pattern_real<ODD, 1600>;
-    case 2560: return pattern_real<ODD, 2560>;
-    case 3200: return pattern_real<ODD, 3200>;
-    case 6400: return pattern_real<ODD, 6400>;
-    default: return pattern_real<ODD, 0>;
+    case 1600: return pattern_real<ODD, 1600, VF>;
+    case 2560: return pattern_real<ODD, 2560, VF>;
+    case 3200: return pattern_real<ODD, 3200, VF>;
+    case 6400: return pattern_real<ODD, 6400, VF>;
+    default: return pattern_real<ODD, 0, VF>;
   }
 }
 
@@ -195,17 +201,18 @@ int cbs_probe_stream(int kind, const void* a, void* b, void* c, long long bytes,
 
 // Runs `reps` launches of the pass pattern `kind` after one untimed launch and writes the
 // average launch time in ms (HIP events on `stream`): kind 0 the k-space pattern on spec;
-// 1 and 2 the real-space pattern of an even (spec, V, f) and an odd (with Field) launch.
+// 1 and 2 the real-space pattern of an even (spec, V, f) and an odd (with Field) launch;
+// 3 and 4 the same with V read as one float per voxel (V then holds nx*ny*nz floats).
 // dims = {nx, ny, nz, a, b, nrhs}: nx and a*nx even, ny = a*b, nx a multiple of 16 for
 // kind 0; spec, f and F hold nrhs*nx*ny*nz complex64, V nx*ny*nz.  Returns 0, 1 for bad
 // arguments, or a hipError_t code.
 int cbs_probe_pattern(int kind, void* spec, const void* V, void* f, void* F, const int* dims, int reps, void* stream,
                       float* ms_per_launch) {
-  if (kind < 0 || kind > 2 || !spec || !dims || reps < 1 || !ms_per_launch) return 1;
+  if (kind < 0 || kind > 4 || !spec || !dims || reps < 1 || !ms_per_launch) return 1;
   const int nx = dims[0], ny = dims[1], nz = dims[2], a = dims[3], b = dims[4], nrhs = dims[5];
   if (nx < 2 || nx % 2 || a < 1 || b < 1 || ny != a * b || nz < 1 || nrhs < 1) return 1;
   if (kind == 0 && nx % 16) return 1;
-  if (kind > 0 && (!V || !f || (kind == 2 && !F))) return 1;
+  if (kind > 0 && (!V || !f || ((kind == 2 || kind == 4) && !F))) return 1;
   const long long nblk = kind == 0 ? (long long)nrhs * a * (nx / 16) : (long long)nrhs * b * nz;
   if (nblk >= (1ll << 31)) return 1;
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -213,10 +220,13 @@ int cbs_probe_pattern(int kind, void* spec, const void* V, void* f, void* F, con
     if (kind == 0)
       hipLaunchKernelGGL(pattern_kspace, dim3((unsigned)nblk), dim3(kPT), 0, s, static_cast<v4f*>(spec), nx, a, b, nz,
                          nrhs);
-    else
-      hipLaunchKernelGGL(kind == 1 ? pattern_real_for<0>(a * nx / 2) : pattern_real_for<1>(a * nx / 2),
-                         dim3((unsigned)nblk), dim3(kPT), 0, s, static_cast<v4f*>(spec), static_cast<const v4f*>(V),
+    else {
+      const int e2 = a * nx / 2;
+      const PatternFn fn = kind == 1 ? pattern_real_for<0, 0>(e2) : kind == 2 ? pattern_real_for<1, 0>(e2)
+                         : kind == 3 ? pattern_real_for<0, 1>(e2) : pattern_real_for<1, 1>(e2);
+      hipLaunchKernelGGL(fn, dim3((unsigned)nblk), dim3(kPT), 0, s, static_cast<v4f*>(spec), static_cast<const v4f*>(V),
                          static_cast<v4f*>(f), static_cast<v4f*>(F), nx, ny, nz, a, b);
+    }
   };
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipError_t err = hipEventCreate(&e0);

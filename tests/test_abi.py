@@ -63,8 +63,9 @@ def test_probe_library_validates_arguments_without_gpu():
     dims = (C.c_int * 6)(160, 160, 160, 40, 4, 49)
     bad_ny = (C.c_int * 6)(160, 150, 160, 40, 4, 49)      # ny != a * b
     bad_nx = (C.c_int * 6)(150, 160, 160, 40, 4, 49)      # k-space tiles of 16 need nx % 16 == 0
-    assert lib.cbs_probe_pattern(3, fake, fake, fake, fake, dims, 1, None, C.byref(ms)) == 1
+    assert lib.cbs_probe_pattern(5, fake, fake, fake, fake, dims, 1, None, C.byref(ms)) == 1
     assert lib.cbs_probe_pattern(1, fake, fake, fake, fake, bad_ny, 1, None, C.byref(ms)) == 1
     assert lib.cbs_probe_pattern(0, fake, fake, fake, fake, bad_nx, 1, None, C.byref(ms)) == 1
-    assert lib.cbs_probe_pattern(2, fake, fake, fake, None, dims, 1, None, C.byref(ms)) == 1    # odd launch needs Field
+    assert lib.cbs_probe_pattern(2, fake, fake, fake, None, dims, 1, None, C.byref(ms)) == 1    # odd launches need Field
+    assert lib.cbs_probe_pattern(4, fake, fake, fake, None, dims, 1, None, C.byref(ms)) == 1
     assert lib.cbs_probe_pattern(1, fake, fake, fake, fake, dims, 0, None, C.byref(ms)) == 1    # reps >= 1
