@@ -1675,9 +1675,9 @@ struct Solver {
         plan = timed_plans[i].second;
         build_tables();
         // real-space tiles in storage order (4), rhs first (0) or rows of one rhs first (1);
-        // k-space rhs or x tile first (2).  Nontemporal k-space loads (8) measured 13-25 %
-        // slower on the 160^3 x 49 batch and are not tried.
-        for (int ord : {4, 0, 1, 2, 3, 6}) {
+        // k-space rhs or x tile first (2), nontemporal k-space loads (8: 2 % faster with
+        // 512-thread k-space blocks at 160^3 x 49, 13-25 % slower with 1024-thread ones)
+        for (int ord : {4, 0, 1, 2, 3, 6, 12}) {
           opt.block_order = ord;
           const double ms = ord == 4 ? timed_plans[i].first : time_plan(nb);
           if (ms < best) {
