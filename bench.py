@@ -78,15 +78,15 @@ def pass_real_bytes(nvox, A, M, v_bytes=8):
     return nvox * np.array([one(jj) for jj in range(0, M + 1)], dtype=np.float64)
 
 
-def measured_traffic(sim, A, plan, alg_per_launch):
+def measured_traffic(sim, A, plan, alg_per_launch, v_bytes=8):
     """HBM bytes per pass_real launch from the calibrated FETCH_SIZE/WRITE_SIZE runs
     recorded in profiles/ for this exact workload and plan (None otherwise)."""
     fn = os.path.join(ROOT, 'profiles', 'pass_real_traffic_160cubed_49rhs.json')
     if not os.path.exists(fn) or list(sim) != [160, 160, 160] or A != 49 or plan.get('a') != 40:
         return None
     rec = json.load(open(fn))
-    if plan.get('block_order', 0) & 4 != rec.get('block_order', 0) & 4:
-        return None     # the real-space block numbering changes how often V is fetched
+    if plan.get('block_order', 0) & 4 != rec.get('block_order', 0) & 4 or v_bytes != rec.get('v_bytes', 8):
+        return None     # the block numbering and the form of V change how often V is fetched
     return round(alg_per_launch * rec['measured_over_algorithmic'])
 
 
@@ -855,7 +855,7 @@ def main():
                    'plan': plan, 'parallelism': 'single GPU, all angles in each launch'},
         'latency_ms': {'p50': round(p50, 3), 'p99': round(p99, 3)},
         'roofline': {'bound': 'hbm', 'achieved': round(achieved_pr, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': round(achieved_pr / HBM_PEAK_GBS, 4), 'traffic': measured_traffic(sim, A, plan, bytes_pr / pr_launches),
+                     'frac': round(achieved_pr / HBM_PEAK_GBS, 4), 'traffic': measured_traffic(sim, A, plan, bytes_pr / pr_launches, vb),
                      'traffic_source': TRAFFIC_SOURCE,
                      'kernel': 'pass_real', 'avg_launch_ms': round(pr_avg_ms, 4),
                      'algorithmic_bytes_per_launch': round(bytes_pr / pr_launches),

@@ -120,3 +120,30 @@ def test_device_potential_from_device_buffer_and_errors():
     buf.free()
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize('ri,bt,flip', [((64, 64, 64), 16, False), ((128, 128, 128), 16, True)])
+def test_real_ri_float_potential_on_the_fused_kernels(ri, bt, flip):
+    """A real RI's potential is read by the fused real-space pass as its real part alone
+    (4 bytes a voxel, Im V = -eps rebuilt in registers), a caller's V as complex64: on the
+    96^3 (C1) and 160^3 (headline) grids, in storage and rhs-first block order, both give
+    the same fields bit for bit."""
+    from inverse_solver_amd import FORWARD_SOLVER_CONVERGENT_BORN
+    from inverse_solver_amd.phantoms import bead_ri, incident_plane_waves
+    p = params(ri, bt_px_target=bt, iterations=5, return_reflection=False)
+    RI = bead_ri(ri, ri[0] // 3, 1.46, 1.336)
+    u = incident_plane_waves(list(ri), 3, 1.2, 0.532, 1.336, [0.1, 0.1, 0.1]).astype(np.complex64)
+    for order in (4, 0):
+        a = FORWARD_SOLVER_CONVERGENT_BORN(p, options=dict(block_order=order))
+        _set_host(a, RI, flip)                      # complex V from the caller
+        assert a.plan()['form'] == 'fused'
+        ua = a.solve_incident(u)
+        a.close()
+        b = FORWARD_SOLVER_CONVERGENT_BORN(p, options=dict(block_order=order))
+        b.set_RI(RI, flip_conj=flip)                # real RI: the float form
+        assert b.plan()['form'] == 'fused' and b.simulation_size == a.simulation_size
+        ub = b.solve_incident(u)
+        b.close()
+        print(ri, 'flip', flip, 'order', order, 'sim', b.simulation_size, 'max |diff|', float(np.max(np.abs(ub - ua))),
+              flush=True)
+        np.testing.assert_array_equal(ub, ua)
