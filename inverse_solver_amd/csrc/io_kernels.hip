@@ -139,27 +139,16 @@ __global__ void io_crop_out(const float2* __restrict__ buf2, float2* __restrict_
   }
 }
 
-// Centred 2D transforms of the scalar field grid (nx x ny, x fastest, nf fields) on the
-// device: fftshift(ifft2(P . fft2(ifftshift(a)))) as FORWARD_SOLVER.m:38-97 filters the
-// input and output fields with the pupil.  field_shift_in gathers ifftshift(a [+ b]),
-// field_pupil_mul scales each spectrum position by the pupil at the frequency it holds
-// (the 1/N of the inverse transform folded in), field_shift_out scatters fftshift(c).
-__global__ void field_shift_in(const float2* __restrict__ a, const float2* __restrict__ b, float2* __restrict__ out,
-                               int nx, int ny, long long nf) {
-  const long long n2 = (long long)nx * ny;
-  const int hx = nx / 2, hy = ny / 2;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n2 * nf;
-       i += (long long)gridDim.x * blockDim.x) {
-    const long long f = i / n2;
-    const int r = (int)(i - f * n2);
-    const int y = r / nx, x = r - y * nx;
-    const int xs = x + hx < nx ? x + hx : x + hx - nx;
-    const int ys = y + hy < ny ? y + hy : y + hy - ny;
-    const long long j = f * n2 + (long long)ys * nx + xs;
-    float2 v = a[j];
-    if (b) v = cadd(v, b[j]);
-    out[i] = v;
-  }
+// Centred pupil filter of fields on the field grid: the centring shifts cancel,
+// fftshift(ifft2(ifftshift(P) fft2(ifftshift(x)))) = ifft2(ifftshift(P) fft2(x)) for any
+// length n (the shift by n/2 multiplies the spectrum by exp(2 pi i k (n/2) / n) on the way
+// in and by its conjugate on the way out), so the filter is field_sum (x = a [+ b]), the
+// field-grid FFTs in place, and field_pupil_mul with the table of ifftshift(P) at each
+// spectrum position (the 1/N of the inverse transform folded in).
+__global__ void field_sum(const float2* a, const float2* __restrict__ b, float2* out,   // out may be a
+                          long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    out[i] = b ? cadd(a[i], b[i]) : a[i];
 }
 
 __global__ void field_pupil_mul(float2* __restrict__ buf, const float* __restrict__ pupil, int nx, int ny, long long nf) {
@@ -167,20 +156,6 @@ __global__ void field_pupil_mul(float2* __restrict__ buf, const float* __restric
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n2 * nf;
        i += (long long)gridDim.x * blockDim.x)
     buf[i] = cscale(buf[i], pupil[i % n2]);
-}
-
-__global__ void field_shift_out(const float2* __restrict__ c, float2* __restrict__ out, int nx, int ny, long long nf) {
-  const long long n2 = (long long)nx * ny;
-  const int hx = nx / 2, hy = ny / 2;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n2 * nf;
-       i += (long long)gridDim.x * blockDim.x) {
-    const long long f = i / n2;
-    const int r = (int)(i - f * n2);
-    const int y = r / nx, x = r - y * nx;
-    const int xs = x - hx >= 0 ? x - hx : x - hx + nx;
-    const int ys = y - hy >= 0 ? y - hy : y - hy + ny;
-    out[i] = c[f * n2 + (long long)ys * nx + xs];
-  }
 }
 
 }  // namespace cbs

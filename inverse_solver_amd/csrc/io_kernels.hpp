@@ -36,11 +36,11 @@ __global__ void io_zsum(const float2* __restrict__ buf3, const float2* __restric
                         const float2* __restrict__ kref, float2* __restrict__ out_t, float2* __restrict__ out_r,
                         const int* __restrict__ kcx, const int* __restrict__ kcy, IoGeom g, int nb);
 __global__ void io_crop_out(const float2* __restrict__ buf2, float2* __restrict__ out, IoGeom g, int nb, float scale);
-// centred pupil transforms of the field grid (field_pupil in solver.hip)
-__global__ void field_shift_in(const float2* __restrict__ a, const float2* __restrict__ b, float2* __restrict__ out,
-                               int nx, int ny, long long nf);
+// centred pupil filter of the field grid (field_pupil in solver.hip): out = a [+ b], then the
+// field-grid FFTs around the pupil product
+__global__ void field_sum(const float2* a, const float2* __restrict__ b, float2* out,   // out may be a
+                          long long n);
 __global__ void field_pupil_mul(float2* __restrict__ buf, const float* __restrict__ pupil, int nx, int ny, long long nf);
-__global__ void field_shift_out(const float2* __restrict__ c, float2* __restrict__ out, int nx, int ny, long long nf);
 
 struct IoState {
   bool ready = false;

@@ -1359,7 +1359,7 @@ struct Solver {
     RtPlan rtx, rty;
     DevBuf<float2> twx, twy;
   } ffp;
-  DevBuf<float2> fp_in, fp_add, fp_buf, io_fsrc;
+  DevBuf<float2> fp_in, fp_add, io_fsrc;
   // the last two pupils (a solve alternates the input and the output one): host copy and
   // position-ordered device table
   struct PupilSlot { std::vector<float> host; DevBuf<float> tab; int nx = 0, ny = 0; long long used = 0; } fp_slot[2];
@@ -1424,16 +1424,18 @@ struct Solver {
   }
 
   // out = centred_ifft2(P .* centred_fft2(in + add)) over nf fields of the field grid, on
-  // device pointers (add may be null, out may be in); tab from pupil_table
+  // device pointers (add may be null, out may be in); tab from pupil_table.  The centring
+  // shifts cancel (a shift by n/2 before the forward transform multiplies the spectrum by
+  // exp(2 pi i k (n/2) / n), the one after the inverse by its conjugate), so this is
+  // ifft2(ifftshift(P) .* fft2(in + add)), in place in out.
   void pupil_apply(const float2* in, const float2* add, long long nf, const float* tab, float2* out) {
     const long long total = (long long)fx * fy * nf;
-    fp_buf.ensure((size_t)total);
     const unsigned g = grid_for(total);
-    timed(2, [&] { hipLaunchKernelGGL(field_shift_in, dim3(g), dim3(256), 0, stream, in, add, fp_buf.p, fx, fy, nf); });
-    field_fft2d(fp_buf.p, nf, 0);
-    timed(2, [&] { hipLaunchKernelGGL(field_pupil_mul, dim3(g), dim3(256), 0, stream, fp_buf.p, tab, fx, fy, nf); });
-    field_fft2d(fp_buf.p, nf, 1);
-    timed(2, [&] { hipLaunchKernelGGL(field_shift_out, dim3(g), dim3(256), 0, stream, fp_buf.p, out, fx, fy, nf); });
+    if (add || in != out)
+      timed(2, [&] { hipLaunchKernelGGL(field_sum, dim3(g), dim3(256), 0, stream, in, add, out, total); });
+    field_fft2d(out, nf, 0);
+    timed(2, [&] { hipLaunchKernelGGL(field_pupil_mul, dim3(g), dim3(256), 0, stream, out, tab, fx, fy, nf); });
+    field_fft2d(out, nf, 1);
   }
 
   // P: fx x fy floats, x fastest, in centred (fftshifted) frequency order, on the host
