@@ -1518,6 +1518,11 @@ struct Solver {
     const float* tab_out = pupil_out ? pupil_table(pupil_out) : nullptr;
     if (pupil_in) io_fsrc.ensure(nvw * nf);
     const int M = (int)prm.Bornmax;
+    // host 3D fields: their download runs on the copy stream while the 2D projections of
+    // the same batch compute (it is issued after them: a copy to pageable memory can hold
+    // the host thread until it completes)
+    const bool f3_async = mem == CBS_MEM_HOST && f3d != nullptr;
+    if (f3_async) ensure_copy_stream();
     begin_solve(n_rhs, M);
     ev_used.clear();
     const float sc = (float)(1.0 / (double)n2);
@@ -1545,7 +1550,9 @@ struct Solver {
       timed(2, [&] { hipLaunchKernelGGL(io_crop_in, dim3(grid_for(nv * nri)), dim3(256), 0, stream, io_buf3.p,
                                         io_uin.p, iog, nv); });
       float2* u3 = (mem == CBS_MEM_DEVICE && f3d) ? f3d + v0 * nri : io_u3d.p;
+      if (f3_async && r0 > 0) HIP_CHECK(hipStreamWaitEvent(stream, ev_down[0], 0));   // io_u3d downloaded
       born_batch(io_uin.p, u3, nb, (int)r0);
+      if (f3_async) HIP_CHECK(hipEventRecord(ev_done[0], stream));
       if (trans || refl) {
         // emitter -> fft2 per z -> sum_z E K (FP64) -> ifft2 -> crop  (:444-511)
         timed(2, [&] { hipLaunchKernelGGL(io_emitter, dim3(grid_for(nv * n2 * rz)), dim3(256), 0, stream, ws.field,
@@ -1575,9 +1582,12 @@ struct Solver {
       }
       HIP_CHECK(hipGetLastError());
       if (mem == CBS_MEM_HOST) {
-        if (f3d)
+        if (f3_async) {
+          HIP_CHECK(hipStreamWaitEvent(cstream, ev_done[0], 0));
           HIP_CHECK(hipMemcpyAsync(f3d + v0 * nri, io_u3d.p, (size_t)nv * nri * sizeof(float2),
-                                   hipMemcpyDeviceToHost, stream));
+                                   hipMemcpyDeviceToHost, cstream));
+          HIP_CHECK(hipEventRecord(ev_down[0], cstream));
+        }
         if (trans)
           HIP_CHECK(hipMemcpyAsync(trans + v0 * nf, io_tr.p, (size_t)nv * nf * sizeof(float2),
                                    hipMemcpyDeviceToHost, stream));
@@ -1586,6 +1596,7 @@ struct Solver {
                                    hipMemcpyDeviceToHost, stream));
       }
     }
+    if (f3_async) HIP_CHECK(hipStreamSynchronize(cstream));
     HIP_CHECK(hipStreamSynchronize(stream));
     if (opt.time_kernels) resolve_times();
     if (prm.acyclic) flip_state = (int)((flip_state + (long long)n_rhs * M) & 1);
