@@ -136,6 +136,7 @@ struct RtPlan {
   DevBuf<float2> tw, chirp, bspec;
   const BlueInfo* bi = nullptr;   // Bluestein on a compiled convolution length (else k)
   BlueArgs ba{};
+  int pref = 0;                   // built with prefer_blue
 };
 
 // Stage radices of a Stockham transform of length L (16, 8, 4, 2 for the power of two,
@@ -166,12 +167,16 @@ static int smooth_at_least(int m) {
 // filter over a length L >= 2n - 1 (FP64 on the host, rounded to complex64) -- on the
 // shortest compiled convolution length when one is long enough (fft_blue.hip), else
 // on a run-time Stockham transform.
-static void build_rt(RtPlan& P, int n) {
+// prefer_blue: take Bluestein's transform on a compiled convolution length even when n
+// factors into run-time radices (the per-axis form fuses those transforms with the update
+// and the Green's multiply; run-time stages have no fused form)
+static void build_rt(RtPlan& P, int n, bool prefer_blue = false) {
   require(n >= 1, CBS_ERR_ARG, "line length must be >= 1");
   std::vector<int> rs;
   int L = n, blue = 0;
   P.bi = nullptr;
-  if (!rt_factor(n, rs)) {
+  P.pref = prefer_blue ? 1 : 0;
+  if (!rt_factor(n, rs) || (prefer_blue && find_blue(2 * n - 1))) {
     blue = 1;
     P.bi = find_blue(2 * n - 1);
     L = P.bi ? P.bi->L : smooth_at_least(2 * n - 1);
@@ -480,8 +485,17 @@ struct Solver {
     p.rt[1] = !p.sy;
     p.rt[2] = !p.sz;
     const int dims[3] = {nx, ny, nz};
-    for (int j = 0; j < 3; ++j)
-      if (p.rt[j] && rtp[j].k.n != dims[j]) build_rt(rtp[j], dims[j]);   // once per handle
+    // one-component runs fuse the x transforms with the update and the z transforms with
+    // the Green's multiply when they are compiled (line FFT or Bluestein on a compiled
+    // convolution length), so those two axes take Bluestein's transform over run-time
+    // stages (12-rhs solves on 169^2 x 75, 98^2 x 90, 126^2 x 78 and 147^2 x 77 grids: 22.0,
+    // 22.4, 17.1 and 17.5 -> 25.0, 29.3, 26.8 and 22.4 Gvoxel-iter/s); y, which no pass
+    // fuses, keeps the run-time stages
+    const bool fuse = P == 1 && p.form == CBS_FORM_PLAIN;
+    for (int j = 0; j < 3; ++j) {
+      const int pref = fuse && j != 1 ? 1 : 0;
+      if (p.rt[j] && (rtp[j].k.n != dims[j] || rtp[j].pref != pref)) build_rt(rtp[j], dims[j], pref);
+    }
     p.a = ny;
     p.b = 1;
     p.t = p.sy ? p.sy->t : 0;
