@@ -108,3 +108,25 @@ def test_centered_fft_of_centered_delta_is_constant():
 
 def test_matlab_round_half_away():
     np.testing.assert_array_equal(matlab_round([0.5, 1.5, 2.5, -0.5, 15.93, 3.49]), [1, 2, 3, -1, 16, 3])
+
+
+@pytest.mark.parametrize('shape', [(8, 8), (9, 7), (16, 11), (13, 20), (1, 5)])
+def test_centred_pupil_filter_equals_uncentred_with_shifted_pupil(shape):
+    """The device pupil filter (cbs_field_pupil, cbs_solve_field) computes
+    ifft2(ifftshift(P) * fft2(x)) without the centring shifts of the reference's
+    fftshift(ifft2(ifftshift(P * fftshift(fft2(ifftshift(x)))))): a shift by n // 2 before
+    the forward transform multiplies the spectrum by exp(2 pi i k (n // 2) / n) and the one
+    after the inverse by its conjugate, for even and odd n alike."""
+    rng = np.random.default_rng(sum(shape))
+    x = rng.standard_normal(shape + (3,)) + 1j * rng.standard_normal(shape + (3,))
+    P = rng.random(shape)
+    ax = (0, 1)
+    ref = np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(
+        P[:, :, None] * np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(x, axes=ax), axes=ax), axes=ax), axes=ax),
+        axes=ax), axes=ax)
+    got = np.fft.ifft2(np.fft.ifftshift(P)[:, :, None] * np.fft.fft2(x, axes=ax), axes=ax)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12 * np.max(np.abs(ref)))
+    # a wrong shift (fftshift of P instead of ifftshift) differs for odd lengths
+    if shape[0] % 2 or shape[1] % 2:
+        bad = np.fft.ifft2(np.fft.fftshift(P)[:, :, None] * np.fft.fft2(x, axes=ax), axes=ax)
+        assert np.max(np.abs(bad - ref)) > 1e-6 * np.max(np.abs(ref))
