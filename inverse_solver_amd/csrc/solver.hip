@@ -1542,73 +1542,81 @@ struct Solver {
     const float sc = (float)(1.0 / (double)n2);
     const Consts c = consts(1);
     const RoiGeom rg = roi();
-    for (uint32_t r0 = 0; r0 < n_rhs; r0 += nbw) {
-      const int nb = (int)std::min<uint32_t>(nbw, n_rhs - r0);
-      const int nv = nb * P;
-      const size_t v0 = (size_t)r0 * P;
-      const float2* s_d = src + v0 * nf;
-      if (mem == CBS_MEM_HOST) {
-        HIP_CHECK(hipMemcpyAsync(io_src.p, s_d, (size_t)nv * nf * sizeof(float2), hipMemcpyHostToDevice, stream));
-        s_d = io_src.p;
-      }
-      if (tab_in) {
-        pupil_apply(s_d, nullptr, nv, tab_in, io_fsrc.p);
-        s_d = io_fsrc.p;
-      }
-      // incident field: pad -> fft2 -> x refocus(z) -> ifft2 -> crop  (:543-554)
-      timed(2, [&] { hipLaunchKernelGGL(io_pad_in, dim3(grid_for(nv * n2)), dim3(256), 0, stream, s_d, io_spec2.p, iog, nv); });
-      fft2d(io_spec2.p, nv, 0);
-      timed(2, [&] { hipLaunchKernelGGL(io_refocus, dim3(grid_for(nv * n2 * rz)), dim3(256), 0, stream, io_spec2.p,
-                                        io_refocus_k.p, io_buf3.p, io_kcx.p, io_kcy.p, iog, nv, sc); });
-      fft2d(io_buf3.p, (long long)nv * rz, 1);
-      timed(2, [&] { hipLaunchKernelGGL(io_crop_in, dim3(grid_for(nv * nri)), dim3(256), 0, stream, io_buf3.p,
-                                        io_uin.p, iog, nv); });
-      float2* u3 = (mem == CBS_MEM_DEVICE && f3d) ? f3d + v0 * nri : io_u3d.p;
-      if (f3_async && r0 > 0) HIP_CHECK(hipStreamWaitEvent(stream, ev_down[0], 0));   // io_u3d downloaded
-      born_batch(io_uin.p, u3, nb, (int)r0);
-      if (f3_async) HIP_CHECK(hipEventRecord(ev_done[0], stream));
-      if (trans || refl) {
-        // emitter -> fft2 per z -> sum_z E K (FP64) -> ifft2 -> crop  (:444-511)
-        timed(2, [&] { hipLaunchKernelGGL(io_emitter, dim3(grid_for(nv * n2 * rz)), dim3(256), 0, stream, ws.field,
-                                          io_uin.p, V.p, io_buf3.p, iog, c, rg.ox, rg.oy, rg.oz, nvox, nv,
-                                          (float)prm.dV); });
-        fft2d(io_buf3.p, (long long)nv * rz, 0);
-        timed(2, [&] { hipLaunchKernelGGL(io_zsum, dim3(grid_for(nv * n2)), dim3(256), 0, stream, io_buf3.p,
-                                          io_ktrans.p, io.have_ref ? io_kref.p : nullptr, trans ? io_out_t.p : nullptr,
-                                          refl ? io_out_r.p : nullptr, io_kcx.p, io_kcy.p, iog, nv); });
-        if (trans) {
-          fft2d(io_out_t.p, nv, 1);
-          float2* dst = mem == CBS_MEM_HOST ? io_tr.p : trans + v0 * nf;
-          float2* crop = tab_out ? io_tr.p : dst;
-          timed(2, [&] { hipLaunchKernelGGL(io_crop_out, dim3(grid_for(nv * nf)), dim3(256), 0, stream, io_out_t.p,
-                                            crop, iog, nv, sc); });
-          // fft2(t) + U = fft2(t + source): the scattered field plus the filtered incident one
-          if (tab_out) pupil_apply(io_tr.p, s_d, nv, tab_out, dst);
+    // downloads still running write into the caller's buffers: they are waited for before
+    // an error leaves this call
+    try {
+      for (uint32_t r0 = 0; r0 < n_rhs; r0 += nbw) {
+        const int nb = (int)std::min<uint32_t>(nbw, n_rhs - r0);
+        const int nv = nb * P;
+        const size_t v0 = (size_t)r0 * P;
+        const float2* s_d = src + v0 * nf;
+        if (mem == CBS_MEM_HOST) {
+          HIP_CHECK(hipMemcpyAsync(io_src.p, s_d, (size_t)nv * nf * sizeof(float2), hipMemcpyHostToDevice, stream));
+          s_d = io_src.p;
         }
-        if (refl) {
-          fft2d(io_out_r.p, nv, 1);
-          float2* dst = mem == CBS_MEM_HOST ? io_rf.p : refl + v0 * nf;
-          float2* crop = tab_out ? io_rf.p : dst;
-          timed(2, [&] { hipLaunchKernelGGL(io_crop_out, dim3(grid_for(nv * nf)), dim3(256), 0, stream, io_out_r.p,
-                                            crop, iog, nv, sc); });
-          if (tab_out) pupil_apply(io_rf.p, nullptr, nv, tab_out, dst);
+        if (tab_in) {
+          pupil_apply(s_d, nullptr, nv, tab_in, io_fsrc.p);
+          s_d = io_fsrc.p;
+        }
+        // incident field: pad -> fft2 -> x refocus(z) -> ifft2 -> crop  (:543-554)
+        timed(2, [&] { hipLaunchKernelGGL(io_pad_in, dim3(grid_for(nv * n2)), dim3(256), 0, stream, s_d, io_spec2.p, iog, nv); });
+        fft2d(io_spec2.p, nv, 0);
+        timed(2, [&] { hipLaunchKernelGGL(io_refocus, dim3(grid_for(nv * n2 * rz)), dim3(256), 0, stream, io_spec2.p,
+                                          io_refocus_k.p, io_buf3.p, io_kcx.p, io_kcy.p, iog, nv, sc); });
+        fft2d(io_buf3.p, (long long)nv * rz, 1);
+        timed(2, [&] { hipLaunchKernelGGL(io_crop_in, dim3(grid_for(nv * nri)), dim3(256), 0, stream, io_buf3.p,
+                                          io_uin.p, iog, nv); });
+        float2* u3 = (mem == CBS_MEM_DEVICE && f3d) ? f3d + v0 * nri : io_u3d.p;
+        if (f3_async && r0 > 0) HIP_CHECK(hipStreamWaitEvent(stream, ev_down[0], 0));   // io_u3d downloaded
+        born_batch(io_uin.p, u3, nb, (int)r0);
+        if (f3_async) HIP_CHECK(hipEventRecord(ev_done[0], stream));
+        if (trans || refl) {
+          // emitter -> fft2 per z -> sum_z E K (FP64) -> ifft2 -> crop  (:444-511)
+          timed(2, [&] { hipLaunchKernelGGL(io_emitter, dim3(grid_for(nv * n2 * rz)), dim3(256), 0, stream, ws.field,
+                                            io_uin.p, V.p, io_buf3.p, iog, c, rg.ox, rg.oy, rg.oz, nvox, nv,
+                                            (float)prm.dV); });
+          fft2d(io_buf3.p, (long long)nv * rz, 0);
+          timed(2, [&] { hipLaunchKernelGGL(io_zsum, dim3(grid_for(nv * n2)), dim3(256), 0, stream, io_buf3.p,
+                                            io_ktrans.p, io.have_ref ? io_kref.p : nullptr, trans ? io_out_t.p : nullptr,
+                                            refl ? io_out_r.p : nullptr, io_kcx.p, io_kcy.p, iog, nv); });
+          if (trans) {
+            fft2d(io_out_t.p, nv, 1);
+            float2* dst = mem == CBS_MEM_HOST ? io_tr.p : trans + v0 * nf;
+            float2* crop = tab_out ? io_tr.p : dst;
+            timed(2, [&] { hipLaunchKernelGGL(io_crop_out, dim3(grid_for(nv * nf)), dim3(256), 0, stream, io_out_t.p,
+                                              crop, iog, nv, sc); });
+            // fft2(t) + U = fft2(t + source): the scattered field plus the filtered incident one
+            if (tab_out) pupil_apply(io_tr.p, s_d, nv, tab_out, dst);
+          }
+          if (refl) {
+            fft2d(io_out_r.p, nv, 1);
+            float2* dst = mem == CBS_MEM_HOST ? io_rf.p : refl + v0 * nf;
+            float2* crop = tab_out ? io_rf.p : dst;
+            timed(2, [&] { hipLaunchKernelGGL(io_crop_out, dim3(grid_for(nv * nf)), dim3(256), 0, stream, io_out_r.p,
+                                              crop, iog, nv, sc); });
+            if (tab_out) pupil_apply(io_rf.p, nullptr, nv, tab_out, dst);
+          }
+        }
+        HIP_CHECK(hipGetLastError());
+        if (mem == CBS_MEM_HOST) {
+          if (f3_async) {
+            HIP_CHECK(hipStreamWaitEvent(cstream, ev_done[0], 0));
+            HIP_CHECK(hipMemcpyAsync(f3d + v0 * nri, io_u3d.p, (size_t)nv * nri * sizeof(float2),
+                                     hipMemcpyDeviceToHost, cstream));
+            HIP_CHECK(hipEventRecord(ev_down[0], cstream));
+          }
+          if (trans)
+            HIP_CHECK(hipMemcpyAsync(trans + v0 * nf, io_tr.p, (size_t)nv * nf * sizeof(float2),
+                                     hipMemcpyDeviceToHost, stream));
+          if (refl)
+            HIP_CHECK(hipMemcpyAsync(refl + v0 * nf, io_rf.p, (size_t)nv * nf * sizeof(float2),
+                                     hipMemcpyDeviceToHost, stream));
         }
       }
-      HIP_CHECK(hipGetLastError());
-      if (mem == CBS_MEM_HOST) {
-        if (f3_async) {
-          HIP_CHECK(hipStreamWaitEvent(cstream, ev_done[0], 0));
-          HIP_CHECK(hipMemcpyAsync(f3d + v0 * nri, io_u3d.p, (size_t)nv * nri * sizeof(float2),
-                                   hipMemcpyDeviceToHost, cstream));
-          HIP_CHECK(hipEventRecord(ev_down[0], cstream));
-        }
-        if (trans)
-          HIP_CHECK(hipMemcpyAsync(trans + v0 * nf, io_tr.p, (size_t)nv * nf * sizeof(float2),
-                                   hipMemcpyDeviceToHost, stream));
-        if (refl)
-          HIP_CHECK(hipMemcpyAsync(refl + v0 * nf, io_rf.p, (size_t)nv * nf * sizeof(float2),
-                                   hipMemcpyDeviceToHost, stream));
-      }
+    } catch (...) {
+      if (f3_async) (void)hipStreamSynchronize(cstream);
+      (void)hipStreamSynchronize(stream);
+      throw;
     }
     if (f3_async) HIP_CHECK(hipStreamSynchronize(cstream));
     HIP_CHECK(hipStreamSynchronize(stream));
