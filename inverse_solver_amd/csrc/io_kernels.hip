@@ -106,19 +106,34 @@ __global__ void io_zsum(const float2* __restrict__ buf3, const float2* __restric
     const int px = (int)(j % g.cx), py = (int)(j / g.cx);
     const long long kj = (long long)kcy[py] * g.cx + kcx[px];
     double tr = 0.0, ti = 0.0, rr = 0.0, ri = 0.0;
-    for (int z = 0; z < g.rz; ++z) {
-      const float2 e = buf3[((long long)r * g.rz + z) * n2 + j];
-      if (out_t) {
-        const float2 k = ktrans[(long long)z * n2 + kj];
-        tr += (double)e.x * k.x - (double)e.y * k.y;
-        ti += (double)e.x * k.y + (double)e.y * k.x;
+    // z in chunks of ZC planes whose loads are all issued before the first is used (one
+    // output per thread leaves few waves per CU for a single rhs); the sums keep z order
+    constexpr int ZC = 8;
+    const float2* eb = buf3 + (long long)r * g.rz * n2 + j;
+    for (int z0 = 0; z0 < g.rz; z0 += ZC) {
+      float2 e[ZC], kt[ZC], kr[ZC];
+#pragma unroll
+      for (int q = 0; q < ZC; ++q) {
+        const int z = z0 + q;
+        if (z < g.rz) {
+          e[q] = eb[(long long)z * n2];
+          if (out_t) kt[q] = ktrans[(long long)z * n2 + kj];
+          if (out_r) kr[q] = kref[(long long)z * n2 + kj];
+        }
       }
-      if (out_r) {
-        const float2 k = kref[(long long)z * n2 + kj];
-        rr += (double)e.x * k.x - (double)e.y * k.y;
-        ri += (double)e.x * k.y + (double)e.y * k.x;
+#pragma unroll
+      for (int q = 0; q < ZC; ++q) {
+        if (z0 + q >= g.rz) break;
+        if (out_t) {
+          tr += (double)e[q].x * kt[q].x - (double)e[q].y * kt[q].y;
+          ti += (double)e[q].x * kt[q].y + (double)e[q].y * kt[q].x;
+        }
+        if (out_r) {
+          rr += (double)e[q].x * kr[q].x - (double)e[q].y * kr[q].y;
+          ri += (double)e[q].x * kr[q].y + (double)e[q].y * kr[q].x;
+        }
       }
-  }
+    }
   if (out_t) out_t[i] = make_float2((float)tr, (float)ti);
   if (out_r) out_r[i] = make_float2((float)rr, (float)ri);
   }
