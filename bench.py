@@ -78,6 +78,12 @@ def pass_real_bytes(nvox, A, M, v_bytes=8):
     return nvox * np.array([one(jj) for jj in range(0, M + 1)], dtype=np.float64)
 
 
+def v_bytes_of(plan, RI):
+    """Bytes of V per voxel that the real-space pass reads: 4 in the fused form for a real
+    RI (its potential's real part as floats; the imaginary part is -eps everywhere), else 8."""
+    return 4 if plan['form'] == 'fused' and np.isrealobj(RI) else 8
+
+
 def measured_traffic(sim, A, plan, alg_per_launch, v_bytes=8):
     """HBM bytes per pass_real launch from the calibrated FETCH_SIZE/WRITE_SIZE runs
     recorded in profiles/ for this exact workload and plan (None otherwise)."""
@@ -684,7 +690,7 @@ def workload_vector(args):
         log('vector step', k + 1, '%.1f ms' % step_ms[-1])
     ms = float(np.mean(step_ms))
     pr_ms, pk_ms = pk[0] / max(pk[1], 1), pk[2] / max(pk[3], 1)
-    vb = 4 if s.plan()['form'] == 'fused' and np.isrealobj(RI) else 8
+    vb = v_bytes_of(s.plan(), RI)
     bytes_pr = float(pass_real_bytes(nvox, 3 * A, M, vb).sum()) / (pk[1] / args.steps)   # 3A component slots share V
     bytes_pk = 16.0 * 3 * A * nvox
     out = d_out.to_host()
@@ -831,7 +837,7 @@ def main():
     p50, p99 = float(np.percentile(step_ms, 50)), float(np.percentile(step_ms, 99))
 
     # roofline of the dominant kernel
-    vb = 4 if plan['form'] == 'fused' and np.isrealobj(RI) else 8   # a real RI's potential: 4 bytes a voxel
+    vb = v_bytes_of(plan, RI)
     bytes_pr = float(pass_real_bytes(nvox, A, M, vb).sum())   # per solve
     pr_avg_ms = pr_ms / max(pr_n, 1)
     pr_launches = pr_n / args.steps
