@@ -32,6 +32,10 @@ def test_argument_errors_without_gpu():
     assert lib.cbs_destroy(C.c_void_p(12345)) == _lib.CBS_ERR_HANDLE
     assert 'handle' in lib.cbs_last_error().decode()
     assert lib.cbs_solve_3d(None, None, 1, None, 0) == _lib.CBS_ERR_HANDLE
+    # the scalar field entry points check the handle before anything else
+    assert lib.cbs_solve_field(None, None, 1, None, None, None, None, None, 0) == _lib.CBS_ERR_HANDLE
+    assert lib.cbs_field_pupil(C.c_void_p(12345), None, None, 1, None, None, 0) == _lib.CBS_ERR_HANDLE
+    assert 'handle' in lib.cbs_last_error().decode()
 
 
 def test_bench_refuses_more_than_one_gpu():
